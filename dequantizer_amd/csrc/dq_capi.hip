@@ -1,0 +1,408 @@
+// C ABI of libdq: graph handles, workspace management, argument checks and
+// dispatch to the kernel launchers.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/dq.h"
+#include "dq_launch.hpp"
+
+using namespace dq;
+
+static thread_local std::string g_last_error;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                             \
+  do {                                                                                            \
+    hipError_t _e = (expr);                                                                       \
+    if (_e != hipSuccess) return fail(DQ_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                                      __FILE__, __LINE__);                                        \
+  } while (0)
+
+struct dq_graph {
+  DevGraph dev;
+  int max_deg = 0;
+  int n_inst = 0;
+  long long tensor_elems = 0;
+  std::vector<long long> toff_host;
+  // device topology
+  int *d_deg = nullptr, *d_adj_ptr = nullptr, *d_adj_edge = nullptr, *d_adj_side = nullptr;
+  long long* d_toff = nullptr;
+  int *d_edge_node = nullptr, *d_node_inst = nullptr, *d_edge_inst = nullptr;
+  int *d_inst_node_ptr = nullptr, *d_inst_edge_ptr = nullptr;
+  // BP schedule
+  int n_levels = 0;
+  std::vector<int> level_ptr, level_nodes;
+  int* d_level_nodes = nullptr;
+  uint8_t *d_node_src = nullptr, *d_edge_src = nullptr;
+  int max_level = 0;
+  // owned scratch
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  cd* X = nullptr;             // orthogonalisers 2E chi^2
+  double* node_dist = nullptr;  // n_nodes
+};
+
+static int ws_ensure(dq_graph* g, size_t bytes) {
+  if (bytes <= g->ws_bytes) return DQ_OK;
+  if (g->ws) HIP_TRY(hipFree(g->ws));
+  g->ws = nullptr;
+  g->ws_bytes = 0;
+  HIP_TRY(hipMalloc(&g->ws, bytes));
+  g->ws_bytes = bytes;
+  return DQ_OK;
+}
+
+template <typename T>
+static int upload(T** dst, const T* src, size_t n) {
+  if (n == 0) n = 1;
+  HIP_TRY(hipMalloc((void**)dst, n * sizeof(T)));
+  if (src) HIP_TRY(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return DQ_OK;
+}
+
+static int check_dtype(int dtype, int* rb) {
+  if (dtype == DQ_C64) {
+    *rb = 4;
+    return DQ_OK;
+  }
+  if (dtype == DQ_C128) {
+    *rb = 8;
+    return DQ_OK;
+  }
+  return fail(DQ_ERR_INVALID, "unknown dtype %d", dtype);
+}
+
+static inline hipStream_t S(void* s) { return (hipStream_t)s; }
+
+#define RET_LAUNCH(expr)                                                                              \
+  do {                                                                                                \
+    hipError_t _e = (expr);                                                                           \
+    if (_e != hipSuccess) return fail(DQ_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(_e)); \
+  } while (0)
+
+extern "C" {
+
+int dq_version(void) { return 100; }
+
+const char* dq_last_error(void) { return g_last_error.c_str(); }
+
+int dq_graph_create(dq_graph** out, int n_nodes, int n_edges, int chi, int phys, const int* deg,
+                    const int* adj_edge, const int* adj_side, const int* edge_node, const int* node_inst,
+                    int n_inst) {
+  if (!out) return fail(DQ_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (n_nodes < 0 || n_edges < 0 || chi < 1 || phys < 1 || phys > 4 || n_inst < 1)
+    return fail(DQ_ERR_INVALID, "bad sizes: n_nodes=%d n_edges=%d chi=%d phys=%d n_inst=%d", n_nodes, n_edges, chi,
+                phys, n_inst);
+  if (n_nodes > 0 && (!deg || !node_inst)) return fail(DQ_ERR_INVALID, "deg/node_inst NULL");
+  if (n_edges > 0 && !edge_node) return fail(DQ_ERR_INVALID, "edge_node NULL");
+  std::vector<int> adj_ptr(n_nodes + 1, 0);
+  int max_deg = 0;
+  for (int n = 0; n < n_nodes; ++n) {
+    if (deg[n] < 0 || deg[n] > kMaxDeg) return fail(DQ_ERR_UNSUPPORTED, "node %d degree %d > %d", n, deg[n], kMaxDeg);
+    adj_ptr[n + 1] = adj_ptr[n] + deg[n];
+    if (deg[n] > max_deg) max_deg = deg[n];
+  }
+  const int nadj = adj_ptr[n_nodes];
+  if (nadj != 2 * n_edges) return fail(DQ_ERR_UNSUPPORTED, "sum of degrees %d != 2 * edges (%d): hyperedges?", nadj,
+                                       2 * n_edges);
+  std::vector<int> seen(2 * (size_t)n_edges, 0);
+  for (int n = 0; n < n_nodes; ++n) {
+    if (node_inst[n] < 0 || node_inst[n] >= n_inst || (n > 0 && node_inst[n] < node_inst[n - 1]))
+      return fail(DQ_ERR_INVALID, "node_inst must be non-decreasing in [0, n_inst)");
+    for (int k = adj_ptr[n]; k < adj_ptr[n + 1]; ++k) {
+      const int e = adj_edge[k], s = adj_side[k];
+      if (e < 0 || e >= n_edges || s < 0 || s > 1) return fail(DQ_ERR_INVALID, "bad adjacency slot %d", k);
+      if (edge_node[2 * e + s] != n) return fail(DQ_ERR_INVALID, "adjacency slot %d disagrees with edge_node", k);
+      if (seen[2 * e + s]++) return fail(DQ_ERR_INVALID, "edge %d side %d listed twice", e, s);
+    }
+  }
+  std::vector<int> edge_inst(n_edges);
+  for (int e = 0; e < n_edges; ++e) {
+    const int a = edge_node[2 * e], b = edge_node[2 * e + 1];
+    if (node_inst[a] != node_inst[b]) return fail(DQ_ERR_INVALID, "edge %d joins two instances", e);
+    edge_inst[e] = node_inst[a];
+    if (e > 0 && edge_inst[e] < edge_inst[e - 1]) return fail(DQ_ERR_INVALID, "edges must be grouped by instance");
+  }
+  std::vector<int> inode(n_inst + 1, 0), iedge(n_inst + 1, 0);
+  for (int n = 0; n < n_nodes; ++n) inode[node_inst[n] + 1]++;
+  for (int e = 0; e < n_edges; ++e) iedge[edge_inst[e] + 1]++;
+  for (int i = 0; i < n_inst; ++i) {
+    inode[i + 1] += inode[i];
+    iedge[i + 1] += iedge[i];
+  }
+  dq_graph* g = new dq_graph();
+  g->max_deg = max_deg;
+  g->n_inst = n_inst;
+  g->toff_host.resize(n_nodes);
+  long long off = 0;
+  for (int n = 0; n < n_nodes; ++n) {
+    g->toff_host[n] = off;
+    off += ipow(chi, deg[n]) * phys;
+  }
+  g->tensor_elems = off;
+  int rc;
+#define UP(dst, src, n)                   \
+  if ((rc = upload(dst, src, n)) != DQ_OK) { \
+    dq_graph_destroy(g);                 \
+    return rc;                           \
+  }
+  UP(&g->d_deg, deg, n_nodes);
+  UP(&g->d_adj_ptr, adj_ptr.data(), n_nodes + 1);
+  UP(&g->d_adj_edge, adj_edge, nadj);
+  UP(&g->d_adj_side, adj_side, nadj);
+  UP(&g->d_toff, g->toff_host.data(), n_nodes);
+  UP(&g->d_edge_node, edge_node, 2 * (size_t)n_edges);
+  UP(&g->d_node_inst, node_inst, n_nodes);
+  UP(&g->d_edge_inst, edge_inst.data(), n_edges);
+  UP(&g->d_inst_node_ptr, inode.data(), n_inst + 1);
+  UP(&g->d_inst_edge_ptr, iedge.data(), n_inst + 1);
+  UP(&g->X, (const cd*)nullptr, 2 * (size_t)n_edges * chi * chi);
+  UP(&g->node_dist, (const double*)nullptr, n_nodes);
+#undef UP
+  DevGraph& d = g->dev;
+  d.n_nodes = n_nodes;
+  d.n_edges = n_edges;
+  d.chi = chi;
+  d.phys = phys;
+  d.deg = g->d_deg;
+  d.adj_ptr = g->d_adj_ptr;
+  d.adj_edge = g->d_adj_edge;
+  d.adj_side = g->d_adj_side;
+  d.toff = g->d_toff;
+  d.edge_node = g->d_edge_node;
+  d.node_inst = g->d_node_inst;
+  d.edge_inst = g->d_edge_inst;
+  // default schedule: synchronous, one level with every node
+  std::vector<int> lp = {0, n_nodes}, ln(n_nodes);
+  for (int n = 0; n < n_nodes; ++n) ln[n] = n;
+  std::vector<unsigned char> ns(nadj, 0), es(2 * (size_t)n_edges, 1);
+  rc = dq_bp_set_schedule(g, 1, lp.data(), ln.data(), ns.data(), es.data());
+  if (rc != DQ_OK) {
+    dq_graph_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return DQ_OK;
+}
+
+int dq_graph_destroy(dq_graph* g) {
+  if (!g) return DQ_OK;
+  void* ptrs[] = {g->d_deg,       g->d_adj_ptr,       g->d_adj_edge,      g->d_adj_side,   g->d_toff,
+                  g->d_edge_node, g->d_node_inst,     g->d_edge_inst,     g->d_inst_node_ptr,
+                  g->d_inst_edge_ptr, g->d_level_nodes, g->d_node_src, g->d_edge_src, g->ws, g->X, g->node_dist};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete g;
+  return DQ_OK;
+}
+
+long long dq_graph_tensor_elems(const dq_graph* g) { return g ? g->tensor_elems : -1; }
+
+int dq_graph_tensor_offsets(const dq_graph* g, long long* offsets) {
+  if (!g || !offsets) return fail(DQ_ERR_INVALID, "NULL argument");
+  memcpy(offsets, g->toff_host.data(), g->toff_host.size() * sizeof(long long));
+  return DQ_OK;
+}
+
+int dq_bp_set_schedule(dq_graph* g, int n_levels, const int* level_ptr, const int* level_nodes,
+                       const unsigned char* node_src, const unsigned char* edge_src) {
+  if (!g || n_levels < 1 || !level_ptr || !level_nodes || !node_src || !edge_src)
+    return fail(DQ_ERR_INVALID, "bad schedule arguments");
+  const int nn = g->dev.n_nodes;
+  if (level_ptr[0] != 0 || level_ptr[n_levels] != nn)
+    return fail(DQ_ERR_INVALID, "schedule must list every node exactly once");
+  std::vector<int> count(nn, 0);
+  int max_level = 0;
+  for (int l = 0; l < n_levels; ++l) {
+    if (level_ptr[l + 1] < level_ptr[l]) return fail(DQ_ERR_INVALID, "level_ptr not monotone");
+    const int sz = level_ptr[l + 1] - level_ptr[l];
+    if (sz > max_level) max_level = sz;
+  }
+  for (int i = 0; i < nn; ++i) {
+    if (level_nodes[i] < 0 || level_nodes[i] >= nn || count[level_nodes[i]]++)
+      return fail(DQ_ERR_INVALID, "schedule must list every node exactly once");
+  }
+  const int nadj = 2 * g->dev.n_edges;
+  for (int i = 0; i < nadj; ++i)
+    if (node_src[i] > 2 || edge_src[i] > 1) return fail(DQ_ERR_INVALID, "bad source code");
+  if (g->d_level_nodes) HIP_TRY(hipFree(g->d_level_nodes));
+  if (g->d_node_src) HIP_TRY(hipFree(g->d_node_src));
+  if (g->d_edge_src) HIP_TRY(hipFree(g->d_edge_src));
+  g->d_level_nodes = nullptr;
+  g->d_node_src = g->d_edge_src = nullptr;
+  int rc;
+  if ((rc = upload(&g->d_level_nodes, level_nodes, nn)) != DQ_OK) return rc;
+  if ((rc = upload(&g->d_node_src, node_src, nadj)) != DQ_OK) return rc;
+  if ((rc = upload(&g->d_edge_src, edge_src, nadj)) != DQ_OK) return rc;
+  g->n_levels = n_levels;
+  g->level_ptr.assign(level_ptr, level_ptr + n_levels + 1);
+  g->max_level = max_level;
+  return DQ_OK;
+}
+
+static long long node_ws_bytes(const dq_graph* g, int rb) {
+  const long long S = ipow(g->dev.chi, g->max_deg) * g->dev.phys;
+  return 2 * S * 2 * rb + (long long)g->dev.chi * g->dev.chi * (long long)sizeof(cd) + 256;
+}
+
+int dq_bp_sweep(dq_graph* g, int dtype, const void* T, const void* n2e_in, const void* e2n_in, void* n2e_out,
+                void* e2n_out, const int* inst_active, void* stream) {
+  int rb, rc;
+  if (!g) return fail(DQ_ERR_INVALID, "NULL graph");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  if (n2e_out == n2e_in || e2n_out == e2n_in || n2e_out == e2n_in || e2n_out == n2e_in)
+    return fail(DQ_ERR_INVALID, "bp sweep outputs must not alias inputs");
+  const long long wsb = (node_ws_bytes(g, rb) + 255) / 256 * 256;
+  if ((rc = ws_ensure(g, (size_t)wsb * (g->max_level > 0 ? g->max_level : 1))) != DQ_OK) return rc;
+  for (int l = 0; l < g->n_levels; ++l) {
+    const int b = g->level_ptr[l], cnt = g->level_ptr[l + 1] - b;
+    RET_LAUNCH(launch_bp_nodes(rb, g->dev, g->d_level_nodes + b, cnt, T, n2e_in, e2n_in, n2e_out, g->d_node_src,
+                               g->ws, wsb, inst_active, g->max_deg, S(stream)));
+  }
+  RET_LAUNCH(launch_bp_edges(rb, g->dev, n2e_out, n2e_in, e2n_out, g->d_edge_src, inst_active, S(stream)));
+  return DQ_OK;
+}
+
+int dq_messages_distance(dq_graph* g, int dtype, const void* n2e_a, const void* e2n_a, const void* n2e_b,
+                         const void* e2n_b, double* out, void* stream) {
+  int rb, rc;
+  if (!g || !out) return fail(DQ_ERR_INVALID, "NULL argument");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  RET_LAUNCH(launch_messages_distance(rb, g->dev, g->d_inst_edge_ptr, g->n_inst, n2e_a, e2n_a, n2e_b, e2n_b, out,
+                                      S(stream)));
+  return DQ_OK;
+}
+
+int dq_lambdas_to_messages(dq_graph* g, int dtype, const double* lam, void* n2e, void* e2n, void* stream) {
+  int rb, rc;
+  if (!g) return fail(DQ_ERR_INVALID, "NULL graph");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  RET_LAUNCH(launch_lambdas_to_messages(rb, g->dev, lam, n2e, e2n, S(stream)));
+  return DQ_OK;
+}
+
+int dq_vidal_gauge(dq_graph* g, int dtype, const void* T, const void* n2e, double eps, void* gammas, double* lam,
+                   const int* inst_active, void* stream) {
+  int rb, rc;
+  if (!g) return fail(DQ_ERR_INVALID, "NULL graph");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  if (gammas == T) return fail(DQ_ERR_INVALID, "gammas must not alias tensors");
+  const long long wse = (edge_svd_ws_bytes_host(g->dev.chi) + 255) / 256 * 256;
+  const long long wsn = (node_ws_bytes(g, rb) + 255) / 256 * 256;
+  const size_t need_e = (size_t)wse * (g->dev.n_edges > 0 ? g->dev.n_edges : 1);
+  const size_t need_n = (size_t)wsn * (g->dev.n_nodes > 0 ? g->dev.n_nodes : 1);
+  if ((rc = ws_ensure(g, need_e > need_n ? need_e : need_n)) != DQ_OK) return rc;
+  RET_LAUNCH(launch_edge_svd(rb, g->dev, n2e, eps, g->X, lam, g->ws, wse, inst_active, S(stream)));
+  RET_LAUNCH(launch_absorb(rb, g->dev, T, g->X, gammas, g->ws, wsn, inst_active, S(stream)));
+  return DQ_OK;
+}
+
+int dq_vidal_distance(dq_graph* g, int dtype, const void* gammas, const double* lam, double eps, double* out,
+                      const int* inst_active, void* stream) {
+  int rb, rc;
+  if (!g || !out) return fail(DQ_ERR_INVALID, "NULL argument");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  const long long cc = (long long)g->dev.chi * g->dev.chi;
+  const long long wsv =
+      ((3 * cc * (long long)sizeof(cd) + ((long long)g->max_deg * g->dev.chi + 4 * g->dev.chi + 16) * 8 + 16) + 255) /
+      256 * 256;
+  if ((rc = ws_ensure(g, (size_t)wsv * (g->dev.n_nodes > 0 ? g->dev.n_nodes : 1))) != DQ_OK) return rc;
+  RET_LAUNCH(launch_vidal_distance(rb, g->dev, gammas, lam, eps, g->node_dist, g->d_inst_node_ptr,
+                                   g->d_inst_edge_ptr, g->n_inst, out, g->ws, wsv, inst_active, S(stream)));
+  return DQ_OK;
+}
+
+int dq_symmetric_gauge(dq_graph* g, int dtype, const void* gammas, const double* lam, double eps, void* out,
+                       void* stream) {
+  int rb, rc;
+  if (!g) return fail(DQ_ERR_INVALID, "NULL graph");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  RET_LAUNCH(launch_symmetric_gauge(rb, g->dev, gammas, lam, eps, out, S(stream)));
+  return DQ_OK;
+}
+
+int dq_apply1(dq_graph* g, int dtype, void* T, const int* nodes, int count, const void* gates, void* stream) {
+  int rb, rc;
+  if (!g || count < 0) return fail(DQ_ERR_INVALID, "bad argument");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  RET_LAUNCH(launch_apply1(rb, g->dev, T, nodes, count, (const cd*)gates, S(stream)));
+  return DQ_OK;
+}
+
+int dq_simple_update(dq_graph* g, int dtype, void* T, double* lam, const int* gates, int count, const void* halves,
+                     double eps, int max_rank, double* err, double* entropy, void* stream) {
+  int rb, rc;
+  if (!g || count < 0 || max_rank < 1) return fail(DQ_ERR_INVALID, "bad argument");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  if (count == 0) return DQ_OK;
+  const int chi = g->dev.chi, p = g->dev.phys;
+  const long long m = ipow(chi, g->max_deg > 0 ? g->max_deg - 1 : 0) * p;
+  const long long n = (long long)chi * p * p;
+  const long long k = m < n ? m : n;
+  const long long cdn = 2 * m * n + 2 * n + 3 * k * k + 2 * m * chi + 16;
+  const long long wsb = (cdn * (long long)sizeof(cd) + (4 * k + 64) * 8 + (k + 8) * 4 + 64 + 255) / 256 * 256;
+  if ((rc = ws_ensure(g, (size_t)wsb * count)) != DQ_OK) return rc;
+  RET_LAUNCH(launch_su(rb, g->dev, T, lam, (const SuGate*)gates, count, (const cd*)halves, eps, max_rank, err,
+                       entropy, g->ws, wsb, S(stream)));
+  return DQ_OK;
+}
+
+int dq_rdm1(dq_graph* g, int dtype, const void* T, const void* e2n, const int* nodes, int count, void* rho,
+            void* stream) {
+  int rb, rc;
+  if (!g || count < 0) return fail(DQ_ERR_INVALID, "bad argument");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  if (count == 0) return DQ_OK;
+  const int chi = g->dev.chi, p = g->dev.phys;
+  const long long Sm = ipow(chi, g->max_deg) * p;
+  const long long wsb = (2 * Sm * 16 + 2LL * chi * p * chi * p * 16 + 64 + 255) / 256 * 256;
+  if ((rc = ws_ensure(g, (size_t)wsb * count)) != DQ_OK) return rc;
+  RET_LAUNCH(launch_rdm1(rb, g->dev, T, e2n, nodes, count, (cd*)rho, g->ws, wsb, S(stream)));
+  return DQ_OK;
+}
+
+int dq_rdm2(dq_graph* g, int dtype, const void* T, const void* e2n, const int* edges, int count, void* rho,
+            void* stream) {
+  int rb, rc;
+  if (!g || count < 0) return fail(DQ_ERR_INVALID, "bad argument");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  if (count == 0) return DQ_OK;
+  const int chi = g->dev.chi, p = g->dev.phys;
+  const long long Sm = ipow(chi, g->max_deg) * p;
+  const long long wsb = (2 * Sm * 16 + 2LL * chi * p * chi * p * 16 + 64 + 255) / 256 * 256;
+  if ((rc = ws_ensure(g, (size_t)wsb * count)) != DQ_OK) return rc;
+  RET_LAUNCH(launch_rdm2(rb, g->dev, T, e2n, edges, count, (cd*)rho, g->ws, wsb, S(stream)));
+  return DQ_OK;
+}
+
+int dq_update_active(const double* dist, double acc, int* iters, int max_iters, int* active, int n_inst,
+                     void* stream) {
+  RET_LAUNCH(launch_update_active(dist, acc, iters, max_iters, active, n_inst, S(stream)));
+  return DQ_OK;
+}
+
+int dq_repad(const dq_graph* old, dq_graph* g, int dtype, const void* T_old, void* T_new, void* stream) {
+  int rb, rc;
+  if (!old || !g) return fail(DQ_ERR_INVALID, "NULL graph");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  if (old->dev.n_nodes != g->dev.n_nodes || old->dev.chi > g->dev.chi || old->dev.phys != g->dev.phys)
+    return fail(DQ_ERR_INVALID, "repad needs the same topology and a chi that does not shrink");
+  RET_LAUNCH(launch_repad(rb, g->dev, old->d_toff, old->dev.chi, T_old, T_new, S(stream)));
+  return DQ_OK;
+}
+
+}  // extern "C"

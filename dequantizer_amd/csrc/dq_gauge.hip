@@ -1,0 +1,308 @@
+// Vidal gauge fixing, Vidal distance and the symmetric gauge.
+//
+// edge_svd      reference dequantizer/src/tensor_ops.py:108-130
+// absorb        tensor_ops.py:88-93
+// vidal_dist    tensor_ops.py:143-175, mappings/vidal_distance.py:17-42
+// canonicalize  tensor_ops.py:189-199
+// All small eigen/singular decompositions run in FP64 (dq_linalg.hpp).
+#include <float.h>
+
+#include "dq_launch.hpp"
+#include "dq_linalg.hpp"
+
+namespace dq {
+
+__device__ __forceinline__ double safe_inverse(double l, double eps) {
+  return 1.0 / ((l < eps ? (double)FLT_MAX : 0.0) + l);
+}
+__device__ __forceinline__ double safe_truncate(double l, double eps) { return l > eps ? l : 0.0; }
+
+// Workspace of one edge block: 11 chi x chi complex matrices + scalars.
+__host__ __device__ inline long long edge_svd_ws_bytes(int chi) {
+  const long long cc = (long long)chi * chi;
+  return 11 * cc * (long long)sizeof(cd) + (6LL * chi + 16) * sizeof(double) + (chi + 4) * sizeof(int);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_edge_svd(DevGraph g, const cx<R>* n2e, double eps, cd* X, double* lam, char* ws, long long wsb,
+               const int* inst_active) {
+  const int e = blockIdx.x;
+  if (inst_active && !inst_active[g.edge_inst[e]]) return;
+  const int n = g.chi;
+  const int cc = n * n;
+  cd* Mf = (cd*)(ws + (long long)blockIdx.x * wsb);
+  cd *Mb = Mf + cc, *Vf = Mb + cc, *Vb = Vf + cc, *Sf = Vb + cc, *Sb = Sf + cc, *If = Sb + cc, *Ib = If + cc;
+  cd *G = Ib + cc, *U = G + cc, *V = U + cc;
+  double* rot = (double*)(V + cc);
+  double* sigma = rot + 4 * n + 4;
+  double* red = sigma + n;
+  int* flag = (int*)(red + n + 8);
+  int* perm = flag + 2;
+
+  const cx<R>* m0 = n2e + (2LL * e) * cc;
+  const cx<R>* m1 = n2e + (2LL * e + 1) * cc;
+  // Hermitian part (x + x^H) / 2, as the reference's eigh symmetrises its input
+  for (int q = threadIdx.x; q < cc; q += blockDim.x) {
+    const int k = q / n, l = q % n;
+    const cd a = cvt<double>(m0[q]), at = cvt<double>(m0[l * n + k]);
+    const cd b = cvt<double>(m1[q]), bt = cvt<double>(m1[l * n + k]);
+    Mf[q] = mk<double>(0.5 * (a.re + at.re), 0.5 * (a.im - at.im));
+    Mb[q] = mk<double>(0.5 * (b.re + bt.re), 0.5 * (b.im - bt.im));
+  }
+  __syncthreads();
+  jacobi_eigh(Mf, Vf, n, rot, flag);
+  jacobi_eigh(Mb, Vb, n, rot, flag);
+  // m^{1/2} and m^{-1/2} (safe) from the eigendecompositions
+  for (int q = threadIdx.x; q < cc; q += blockDim.x) {
+    const int r = q / n, c = q % n;
+    cd sf = mk<double>(0, 0), sb = sf, inf = sf, inb = sf;
+    for (int i = 0; i < n; ++i) {
+      const double wf = Mf[i * n + i].re, wb = Mb[i * n + i].re;
+      const cd pf = Vf[r * n + i] * conj(Vf[c * n + i]);
+      const cd pb = Vb[r * n + i] * conj(Vb[c * n + i]);
+      const double a = sqrt(safe_truncate(wf, eps)), b = sqrt(safe_truncate(wb, eps));
+      const double ai = sqrt(safe_inverse(wf, eps)), bi = sqrt(safe_inverse(wb, eps));
+      sf = sf + a * pf;
+      sb = sb + b * pb;
+      inf = inf + ai * pf;
+      inb = inb + bi * pb;
+    }
+    Sf[q] = sf;
+    Sb[q] = sb;
+    If[q] = inf;
+    Ib[q] = inb;
+  }
+  __syncthreads();
+  // G (column-major) = Sf Sb^T
+  for (int q = threadIdx.x; q < cc; q += blockDim.x) {
+    const int c = q / n, r = q % n;
+    cd acc = mk<double>(0, 0);
+    for (int k = 0; k < n; ++k) fma_acc(acc, Sf[r * n + k], Sb[c * n + k]);
+    G[q] = acc;
+  }
+  __syncthreads();
+  jacobi_svd(G, n, n, n, U, n, V, sigma, rot, flag, perm);
+  double nrm = 0.0;
+  for (int j = 0; j < n; ++j) nrm += sigma[j] * sigma[j];
+  nrm = sqrt(nrm);
+  // masked Frobenius norms of U and Vh
+  double nu = 0.0, nv = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const double lj = nrm > 0.0 ? sigma[j] / nrm : 0.0;
+    if (safe_truncate(lj, eps) > eps) {
+      for (int i = 0; i < n; ++i) {
+        nu += norm2(U[j * n + i]);
+        nv += norm2(V[perm[j] * n + i]);
+      }
+    }
+  }
+  nu = sqrt(nu);
+  nv = sqrt(nv);
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const double lj = nrm > 0.0 ? safe_truncate(sigma[j] / nrm, eps) : 0.0;
+    lam[(long long)e * n + j] = lj;
+  }
+  cd* Xf = X + (2LL * e) * cc;
+  cd* Xb = X + (2LL * e + 1) * cc;
+  for (int q = threadIdx.x; q < cc; q += blockDim.x) {
+    const int b = q / n, k = q % n;
+    const double lk = nrm > 0.0 ? safe_truncate(sigma[k] / nrm, eps) : 0.0;
+    cd xf = mk<double>(0, 0), xb = xf;
+    if (lk > eps) {
+      for (int i = 0; i < n; ++i) {
+        fma_acc(xf, If[b * n + i], U[k * n + i]);
+        fma_acc(xb, Ib[b * n + i], conj(V[perm[k] * n + i]));
+      }
+      xf = (1.0 / nu) * xf;
+      xb = (1.0 / nv) * xb;
+    }
+    Xf[q] = xf;
+    Xb[q] = xb;
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_absorb(DevGraph g, const cx<R>* T, const cd* X, cx<R>* gam, char* ws, long long wsb, const int* inst_active) {
+  __shared__ double red[8];
+  const int node = blockIdx.x;
+  if (inst_active && !inst_active[g.node_inst[node]]) return;
+  const int d = g.deg[node], chi = g.chi, phys = g.phys;
+  const long long cc = (long long)chi * chi;
+  const long long S = ipow(chi, d) * phys;
+  const int base = g.adj_ptr[node];
+  cx<R>* w0 = (cx<R>*)(ws + (long long)blockIdx.x * wsb);
+  cx<R>* w1 = w0 + S;
+  const cx<R>* src = T + g.toff[node];
+  int buf = 0;
+  for (int i = 0; i < d; ++i) {
+    const int e = g.adj_edge[base + i], s = g.adj_side[base + i];
+    cx<R>* dst = buf ? w1 : w0;
+    mode_product<R, double>(dst, src, X + (2LL * e + s) * cc, chi, d, phys, i, true);
+    __syncthreads();
+    src = dst;
+    buf ^= 1;
+  }
+  double acc = 0.0;
+  for (long long f = threadIdx.x; f < S; f += blockDim.x) acc += (double)norm2(src[f]);
+  const double nrm = sqrt(block_sum(acc, red));
+  const R inv = (R)(1.0 / nrm);
+  cx<R>* out = gam + g.toff[node];
+  for (long long f = threadIdx.x; f < S; f += blockDim.x) out[f] = inv * src[f];
+}
+
+// lexicographic complex comparison z > eps (real part first)
+__device__ __forceinline__ bool complex_gt(cd z, double eps) { return z.re == eps ? z.im > 0.0 : z.re > eps; }
+
+__host__ __device__ inline long long vidal_ws_bytes(int chi, int max_deg) {
+  const long long cc = (long long)chi * chi;
+  return 3 * cc * (long long)sizeof(cd) + ((long long)max_deg * chi + 4 * chi + 16) * sizeof(double) + 16;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_vidal_node(DevGraph g, const cx<R>* gam, const double* lam, double eps, double* node_dist, char* ws,
+                 long long wsb, const int* inst_active) {
+  __shared__ const double* wp[kMaxDeg];
+  const int node = blockIdx.x;
+  if (inst_active && !inst_active[g.node_inst[node]]) return;
+  const int d = g.deg[node], chi = g.chi, phys = g.phys;
+  const int cc = chi * chi;
+  const int base = g.adj_ptr[node];
+  cd* Rm = (cd*)(ws + (long long)blockIdx.x * wsb);
+  cd* A = Rm + cc;
+  double* w2 = (double*)(A + cc);
+  double* rot = w2 + (long long)d * chi;
+  int* flag = (int*)(rot + 4 * chi + 8);
+  for (int t = threadIdx.x; t < d * chi; t += blockDim.x) {
+    const int j = t / chi, b = t % chi;
+    const double l = lam[(long long)g.adj_edge[base + j] * chi + b];
+    w2[t] = l * l;
+  }
+  if (threadIdx.x < d) wp[threadIdx.x] = w2 + threadIdx.x * chi;
+  __syncthreads();
+  const cx<R>* t = gam + g.toff[node];
+  double dist = 0.0;
+  for (int idx = 0; idx < d; ++idx) {
+    bond_gram<R>(Rm, t, t, chi, d, phys, idx, wp);
+    __syncthreads();
+    cd tr = mk<double>(0, 0);
+    int cnt = 0;
+    for (int k = 0; k < chi; ++k) tr = tr + Rm[k * chi + k];
+    for (int k = 0; k < chi; ++k) cnt += complex_gt(cdiv(Rm[k * chi + k], tr), eps) ? 1 : 0;
+    for (int q = threadIdx.x; q < cc; q += blockDim.x) {
+      const int k = q / chi, l = q % chi;
+      cd a = cdiv(Rm[q], tr), at = cdiv(Rm[l * chi + k], tr);
+      cd h = mk<double>(0.5 * (a.re + at.re), 0.5 * (a.im - at.im));
+      if (k == l && complex_gt(a, eps)) h.re -= 1.0 / cnt;
+      A[q] = h;
+    }
+    __syncthreads();
+    jacobi_eigh(A, nullptr, chi, rot, flag);
+    for (int k = 0; k < chi; ++k) dist += fabs(A[k * chi + k].re);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) node_dist[node] = dist;
+}
+
+__global__ void __launch_bounds__(256)
+    k_vidal_reduce(const double* node_dist, const int* inst_node_ptr, const int* inst_edge_ptr, double* out,
+                   const int* inst_active) {
+  __shared__ double red[8];
+  const int inst = blockIdx.x;
+  if (inst_active && !inst_active[inst]) return;
+  double acc = 0.0;
+  for (int n = inst_node_ptr[inst] + threadIdx.x; n < inst_node_ptr[inst + 1]; n += blockDim.x) acc += node_dist[n];
+  const double s = block_sum(acc, red);
+  const int ne = inst_edge_ptr[inst + 1] - inst_edge_ptr[inst];
+  if (threadIdx.x == 0) out[inst] = s / (2.0 * ne);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_symmetric(DevGraph g, const cx<R>* gam, const double* lam, double eps, cx<R>* out) {
+  __shared__ double red[8];
+  const int node = blockIdx.x;
+  const int d = g.deg[node], chi = g.chi, phys = g.phys;
+  const long long S = ipow(chi, d) * phys;
+  const int base = g.adj_ptr[node];
+  const cx<R>* t = gam + g.toff[node];
+  cx<R>* o = out + g.toff[node];
+  double acc = 0.0;
+  for (long long f = threadIdx.x; f < S; f += blockDim.x) {
+    long long r = f / phys;
+    double fac = 1.0;
+    for (int j = d - 1; j >= 0; --j) {
+      const int b = (int)(r % chi);
+      r /= chi;
+      fac *= sqrt(safe_truncate(lam[(long long)g.adj_edge[base + j] * chi + b], eps));
+    }
+    const cx<R> v = (R)fac * t[f];
+    o[f] = v;
+    acc += (double)norm2(v);
+  }
+  const double nrm = sqrt(block_sum(acc, red));
+  const R inv = (R)(1.0 / nrm);
+  for (long long f = threadIdx.x; f < S; f += blockDim.x) o[f] = inv * o[f];
+}
+
+// ------------------------------------------------------------- launchers
+long long edge_svd_ws_bytes_host(int chi) { return edge_svd_ws_bytes(chi); }
+
+hipError_t launch_edge_svd(int rb, const DevGraph& g, const void* n2e, double eps, cd* X, double* lam, void* ws,
+                           long long wsb, const int* inst_active, hipStream_t s) {
+  if (g.n_edges <= 0) return hipSuccess;
+  if (rb == 4)
+    hipLaunchKernelGGL(k_edge_svd<float>, dim3(g.n_edges), dim3(256), 0, s, g, (const cx<float>*)n2e, eps, X, lam,
+                       (char*)ws, wsb, inst_active);
+  else
+    hipLaunchKernelGGL(k_edge_svd<double>, dim3(g.n_edges), dim3(256), 0, s, g, (const cx<double>*)n2e, eps, X,
+                       lam, (char*)ws, wsb, inst_active);
+  return hipGetLastError();
+}
+
+hipError_t launch_absorb(int rb, const DevGraph& g, const void* T, const cd* X, void* gam, void* ws, long long wsb,
+                         const int* inst_active, hipStream_t s) {
+  if (g.n_nodes <= 0) return hipSuccess;
+  if (rb == 4)
+    hipLaunchKernelGGL(k_absorb<float>, dim3(g.n_nodes), dim3(256), 0, s, g, (const cx<float>*)T, X,
+                       (cx<float>*)gam, (char*)ws, wsb, inst_active);
+  else
+    hipLaunchKernelGGL(k_absorb<double>, dim3(g.n_nodes), dim3(256), 0, s, g, (const cx<double>*)T, X,
+                       (cx<double>*)gam, (char*)ws, wsb, inst_active);
+  return hipGetLastError();
+}
+
+hipError_t launch_vidal_distance(int rb, const DevGraph& g, const void* gam, const double* lam, double eps,
+                                 double* node_dist, const int* inst_node_ptr, const int* inst_edge_ptr, int n_inst,
+                                 double* inst_dist, void* ws, long long wsb, const int* inst_active, hipStream_t s) {
+  if (g.n_nodes > 0) {
+    if (rb == 4)
+      hipLaunchKernelGGL(k_vidal_node<float>, dim3(g.n_nodes), dim3(256), 0, s, g, (const cx<float>*)gam, lam, eps,
+                         node_dist, (char*)ws, wsb, inst_active);
+    else
+      hipLaunchKernelGGL(k_vidal_node<double>, dim3(g.n_nodes), dim3(256), 0, s, g, (const cx<double>*)gam, lam,
+                         eps, node_dist, (char*)ws, wsb, inst_active);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+  }
+  if (n_inst > 0)
+    hipLaunchKernelGGL(k_vidal_reduce, dim3(n_inst), dim3(256), 0, s, node_dist, inst_node_ptr, inst_edge_ptr,
+                       inst_dist, inst_active);
+  return hipGetLastError();
+}
+
+hipError_t launch_symmetric_gauge(int rb, const DevGraph& g, const void* gam, const double* lam, double eps, void* out,
+                                  hipStream_t s) {
+  if (g.n_nodes <= 0) return hipSuccess;
+  if (rb == 4)
+    hipLaunchKernelGGL(k_symmetric<float>, dim3(g.n_nodes), dim3(256), 0, s, g, (const cx<float>*)gam, lam, eps,
+                       (cx<float>*)out);
+  else
+    hipLaunchKernelGGL(k_symmetric<double>, dim3(g.n_nodes), dim3(256), 0, s, g, (const cx<double>*)gam, lam, eps,
+                       (cx<double>*)out);
+  return hipGetLastError();
+}
+
+}  // namespace dq

@@ -1,0 +1,175 @@
+// Internal declarations shared by the HIP translation units of libdq.
+//
+// Data layout in HBM (all complex values interleaved re/im, like torch
+// complex64 / complex128):
+//   node tensors  T[n] at toff[n], row-major (bond_0 .. bond_{d-1}, phys),
+//                 every bond padded to the graph's chi (zeros beyond the true
+//                 bond dimension), bonds in the node's adjacency order;
+//   messages      n2e[(e*2+s)*chi*chi]  node (side s of edge e) -> edge e,
+//                 e2n[(e*2+s)*chi*chi]  edge e -> node (side s), M[bra][ket];
+//   lambdas       lam[e*chi + k]  (double, zeros beyond the true dimension).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dq {
+
+constexpr int kMaxDeg = 8;
+
+template <typename R>
+struct cx {
+  R re, im;
+};
+
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> mk(R a, R b) {
+  cx<R> z;
+  z.re = a;
+  z.im = b;
+  return z;
+}
+template <typename R>
+__device__ __forceinline__ cx<R> operator+(cx<R> a, cx<R> b) { return mk<R>(a.re + b.re, a.im + b.im); }
+template <typename R>
+__device__ __forceinline__ cx<R> operator-(cx<R> a, cx<R> b) { return mk<R>(a.re - b.re, a.im - b.im); }
+template <typename R>
+__device__ __forceinline__ cx<R> operator*(cx<R> a, cx<R> b) {
+  return mk<R>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <typename R>
+__device__ __forceinline__ cx<R> operator*(R s, cx<R> a) { return mk<R>(s * a.re, s * a.im); }
+template <typename R>
+__device__ __forceinline__ cx<R> conj(cx<R> a) { return mk<R>(a.re, -a.im); }
+template <typename R>
+__device__ __forceinline__ R norm2(cx<R> a) { return a.re * a.re + a.im * a.im; }
+// a += b * c
+template <typename R>
+__device__ __forceinline__ void fma_acc(cx<R>& a, cx<R> b, cx<R> c) {
+  a.re = fma(b.re, c.re, a.re);
+  a.re = fma(-b.im, c.im, a.re);
+  a.im = fma(b.re, c.im, a.im);
+  a.im = fma(b.im, c.re, a.im);
+}
+// a += conj(b) * c
+template <typename R>
+__device__ __forceinline__ void fma_cacc(cx<R>& a, cx<R> b, cx<R> c) {
+  a.re = fma(b.re, c.re, a.re);
+  a.re = fma(b.im, c.im, a.re);
+  a.im = fma(b.re, c.im, a.im);
+  a.im = fma(-b.im, c.re, a.im);
+}
+template <typename R, typename S>
+__device__ __forceinline__ cx<R> cvt(cx<S> a) { return mk<R>((R)a.re, (R)a.im); }
+template <typename R>
+__device__ __forceinline__ cx<R> cdiv(cx<R> a, cx<R> b) {
+  R d = b.re * b.re + b.im * b.im;
+  return mk<R>((a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d);
+}
+
+typedef cx<double> cd;
+
+// Graph descriptor passed by value to kernels (all pointers are device memory).
+struct DevGraph {
+  int n_nodes;
+  int n_edges;
+  int chi;   // padded bond dimension
+  int phys;  // physical dimension (uniform)
+  const int* deg;         // [n_nodes]
+  const int* adj_ptr;     // [n_nodes+1]
+  const int* adj_edge;    // [adj_ptr[n]] edge of each bond slot
+  const int* adj_side;    // [adj_ptr[n]] side (0/1) of the node in that edge
+  const long long* toff;  // [n_nodes] node tensor offsets in complex elements
+  const int* edge_node;   // [2*n_edges]
+  const int* node_inst;   // [n_nodes]
+  const int* edge_inst;   // [n_edges]
+};
+
+__host__ __device__ inline long long ipow(long long b, int e) {
+  long long r = 1;
+  for (int i = 0; i < e; ++i) r *= b;
+  return r;
+}
+
+// ---------------------------------------------------------------- reductions
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide sum of a double; `red` must hold blockDim.x/64 doubles.  All
+// threads get the result.
+__device__ inline double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < nw; ++i) s += red[i];
+  __syncthreads();
+  return s;
+}
+
+// ----------------------------------------------------- generic tensor helpers
+// dst = src with matrix M applied on axis j of a tensor with `d` bonds of
+// size chi and a trailing phys axis: dst[o, a, in] = sum_b M[a*chi+b] src[o, b, in]
+// (transposed=true uses M[b*chi+a]).  Whole block cooperates.
+template <typename R, typename MT>
+__device__ void mode_product(cx<R>* dst, const cx<R>* src, const cx<MT>* M, int chi, int d, int phys, int j,
+                             bool transposed) {
+  const long long inner = ipow(chi, d - 1 - j) * phys;
+  const long long total = ipow(chi, d) * phys;
+  for (long long f = threadIdx.x; f < total; f += blockDim.x) {
+    const long long in = f % inner;
+    const long long t = f / inner;
+    const int a = (int)(t % chi);
+    const long long o = t / chi;
+    const cx<R>* s = src + o * chi * inner + in;
+    cx<R> acc = mk<R>(0, 0);
+    for (int b = 0; b < chi; ++b) {
+      cx<MT> m = transposed ? M[(long long)b * chi + a] : M[(long long)a * chi + b];
+      fma_acc(acc, cvt<R>(m), s[b * inner]);
+    }
+    dst[f] = acc;
+  }
+}
+
+// out[k*chi+l] = sum over all axes but bond idx of conj(A[..k..]) * B[..l..]
+// * prod_{j != idx} w_j[b_j] (w may be null).  Accumulates in double.
+// Results written to `out` (double complex).  Whole block cooperates.
+template <typename R>
+__device__ void bond_gram(cd* out, const cx<R>* A, const cx<R>* B, int chi, int d, int phys, int idx,
+                          const double* const* w) {
+  const long long inner = ipow(chi, d - 1 - idx) * phys;
+  const long long rest = ipow(chi, d - 1) * phys;
+  const int pairs = chi * chi;
+  for (int q = threadIdx.x; q < pairs; q += blockDim.x) {
+    const int k = q / chi, l = q % chi;
+    double sr = 0.0, si = 0.0;
+    for (long long r = 0; r < rest; ++r) {
+      const long long o = r / inner, in = r % inner;
+      double wt = 1.0;
+      if (w) {
+        // decode bonds of the rest index (outer bonds 0..idx-1, inner bonds idx+1..d-1)
+        long long oo = o;
+        for (int j = idx - 1; j >= 0; --j) {
+          wt *= w[j][oo % chi];
+          oo /= chi;
+        }
+        long long ii = in / phys;
+        for (int j = d - 1; j > idx; --j) {
+          wt *= w[j][ii % chi];
+          ii /= chi;
+        }
+      }
+      const cx<R> a = A[(o * chi + k) * inner + in];
+      const cx<R> b = B[(o * chi + l) * inner + in];
+      sr += wt * ((double)a.re * b.re + (double)a.im * b.im);
+      si += wt * ((double)a.re * b.im - (double)a.im * b.re);
+    }
+    out[q] = mk<double>(sr, si);
+  }
+}
+
+}  // namespace dq

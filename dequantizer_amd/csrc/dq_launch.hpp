@@ -1,0 +1,85 @@
+// Host-side launchers of the libdq kernels (one per kernel family).  Each
+// returns the hipError_t of its launch.  `real_bytes` selects the bulk
+// arithmetic: 4 = complex64 (float accumulation), 8 = complex128.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dq_internal.hpp"
+
+namespace dq {
+
+// Per-level BP node update.  node_src codes per adjacency slot:
+// 0 = e2n_old[e][side], 1 = n2e_new[e][1-side], 2 = n2e_old[e][1-side].
+hipError_t launch_bp_nodes(int real_bytes, const DevGraph& g, const int* nodes, int count, const void* T,
+                           const void* n2e_old, const void* e2n_old, void* n2e_new, const uint8_t* node_src,
+                           void* ws, long long ws_bytes_per_block, const int* inst_active, int max_deg,
+                           hipStream_t s);
+
+// e2n_new[e][s] = (edge_src[2e+s] == 0 ? n2e_new : n2e_old)[e][1-s].
+hipError_t launch_bp_edges(int real_bytes, const DevGraph& g, const void* n2e_new, const void* n2e_old, void* e2n_new,
+                           const uint8_t* edge_src, const int* inst_active, hipStream_t s);
+
+// n2e[e][s] = e2n[e][s] = diag(lam[e]).
+hipError_t launch_lambdas_to_messages(int real_bytes, const DevGraph& g, const double* lam, void* n2e, void* e2n,
+                                      hipStream_t s);
+
+// out[i] = sum over the messages of instance i of |A - B|_F, divided by the
+// message count (4 per edge): A/B = (n2e, e2n) pairs.
+hipError_t launch_messages_distance(int real_bytes, const DevGraph& g, const int* inst_edge_ptr, int n_inst,
+                                    const void* n2e_a, const void* e2n_a, const void* n2e_b, const void* e2n_b,
+                                    double* out, hipStream_t s);
+
+// Vidal gauge, edge part: orthogonalisers X[e][s] (complex double, chi x chi)
+// and lambda[e] from the node->edge messages.
+hipError_t launch_edge_svd(int real_bytes, const DevGraph& g, const void* n2e, double eps, cd* X, double* lam,
+                           void* ws, long long ws_bytes_per_block, const int* inst_active, hipStream_t s);
+
+long long edge_svd_ws_bytes_host(int chi);
+
+// Vidal gauge, node part: Gamma = T x_i X_i^T normalised.
+hipError_t launch_absorb(int real_bytes, const DevGraph& g, const void* T, const cd* X, void* gammas, void* ws,
+                         long long ws_bytes_per_block, const int* inst_active, hipStream_t s);
+
+// Per-node Vidal distance, then per-instance sum / (2 |E_i|).
+hipError_t launch_vidal_distance(int real_bytes, const DevGraph& g, const void* gammas, const double* lam, double eps,
+                                 double* node_dist, const int* inst_node_ptr, const int* inst_edge_ptr, int n_inst,
+                                 double* inst_dist, void* ws, long long ws_bytes_per_block, const int* inst_active,
+                                 hipStream_t s);
+
+// Symmetric gauge: out = Gamma * prod_i sqrt(lam_i > eps) normalised.
+hipError_t launch_symmetric_gauge(int real_bytes, const DevGraph& g, const void* gammas, const double* lam, double eps,
+                                  void* out, hipStream_t s);
+
+// One-site gates: T[nodes[i]] <- normalise(T . G_i^T), G_i = gates[i] (phys x phys complex double).
+hipError_t launch_apply1(int real_bytes, const DevGraph& g, void* T, const int* nodes, int count, const cd* gates,
+                         hipStream_t s);
+
+// Simple update for `count` gates on pairwise disjoint edges.
+struct SuGate {
+  int edge;   // edge index
+  int first;  // side of the controlling node in that edge
+  int rank;   // operator-Schmidt rank r of the gate
+  int pad;
+};
+hipError_t launch_su(int real_bytes, const DevGraph& g, void* T, double* lam, const SuGate* gates, int count,
+                     const cd* halves /* per gate: 2 halves of phys^4 (r <= phys^2, each (r, phys, phys)) */, double eps, int max_rank, double* err,
+                     double* entropy, void* ws, long long ws_bytes_per_block, hipStream_t s);
+
+// One-site density matrices with messages e2n: rho[i] (phys x phys).
+hipError_t launch_rdm1(int real_bytes, const DevGraph& g, const void* T, const void* e2n, const int* nodes, int count,
+                       cd* rho, void* ws, long long ws_bytes_per_block, hipStream_t s);
+
+// Two-site density matrices on edges: rho[i] (phys^2 x phys^2), rows (s_side0, s_side1).
+hipError_t launch_rdm2(int real_bytes, const DevGraph& g, const void* T, const void* e2n, const int* edges, int count,
+                       cd* rho, void* ws, long long ws_bytes_per_block, hipStream_t s);
+
+// Re-pad node tensors from chi_old to the graph's chi (zeros in new slots).
+hipError_t launch_repad(int real_bytes, const DevGraph& g_new, const long long* toff_old, int chi_old,
+                        const void* T_old, void* T_new, hipStream_t s);
+
+// Per-instance convergence bookkeeping of the BP-to-Vidal loop.
+hipError_t launch_update_active(const double* dist, double acc, int* iters, int max_iters, int* active, int n_inst,
+                                hipStream_t s);
+
+}  // namespace dq

@@ -1,0 +1,306 @@
+// FP64 small dense linear algebra, one thread block per problem.
+//
+//   jacobi_eigh     Hermitian eigendecomposition, cyclic two-sided Jacobi
+//                   (Golub & Van Loan, "Matrix Computations", 4th ed., 8.5)
+//                   with the round-robin parallel ordering (Brent & Luk 1985)
+//                   and a complex phase reduction of each 2x2 block.
+//   jacobi_svd      one-sided (Hestenes 1958) Jacobi SVD, same ordering.
+//   householder_qr  Householder QR with reflectors H = I - u u^H (|u|^2 = 2).
+//
+// All matrices live in memory visible to the whole block (LDS or global);
+// every function is called by all threads of the block.
+#pragma once
+#include "dq_internal.hpp"
+
+namespace dq {
+
+// Round-robin pair k of round r among n2 (even) players; every unordered pair
+// appears exactly once over the n2-1 rounds.
+__device__ __forceinline__ void rr_pair(int r, int k, int n2, int& p, int& q) {
+  if (k == 0) {
+    p = n2 - 1;
+    q = r;
+  } else {
+    p = (r + k) % (n2 - 1);
+    q = (r - k + n2 - 1) % (n2 - 1);
+  }
+  if (p > q) {
+    int t = p;
+    p = q;
+    q = t;
+  }
+}
+
+// Jacobi rotation for the Hermitian 2x2 block [[app, apq], [conj(apq), aqq]]:
+// J = [[c, s], [-s e, c e]] with e = exp(-i arg apq) makes J^H B J diagonal.
+// Returns false when the block is already diagonal to working precision.
+__device__ __forceinline__ bool jacobi_params(double app, double aqq, cd apq, double& c, double& s, double& er,
+                                              double& ei) {
+  const double mag = sqrt(norm2(apq));
+  if (!(mag > 1e-15 * sqrt(fabs(app * aqq))) || mag < 1e-300) return false;
+  er = apq.re / mag;
+  ei = -apq.im / mag;
+  const double tau = (aqq - app) / (2.0 * mag);
+  const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+  c = 1.0 / sqrt(1.0 + t * t);
+  s = t * c;
+  return true;
+}
+
+// A (n x n row-major Hermitian) -> eigenvalues on the diagonal; V columns =
+// eigenvectors.  rot: 4*ceil(n/2) doubles; flag: one int.
+__device__ inline void jacobi_eigh(cd* A, cd* V, int n, double* rot, int* flag) {
+  const int n2 = n + (n & 1), np = n2 / 2;
+  if (V)
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x) V[i] = mk<double>((i / n) == (i % n) ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    if (threadIdx.x == 0) *flag = 0;
+    __syncthreads();
+    for (int r = 0; r < n2 - 1; ++r) {
+      for (int k = threadIdx.x; k < np; k += blockDim.x) {
+        int p, q;
+        rr_pair(r, k, n2, p, q);
+        double c = 1.0, s = 0.0, er = 1.0, ei = 0.0;
+        if (q < n && jacobi_params(A[p * n + p].re, A[q * n + q].re, A[p * n + q], c, s, er, ei)) *flag = 1;
+        rot[4 * k + 0] = c;
+        rot[4 * k + 1] = s;
+        rot[4 * k + 2] = er;
+        rot[4 * k + 3] = ei;
+      }
+      __syncthreads();
+      // A <- A J and V <- V J (columns p, q)
+      for (int t = threadIdx.x; t < n * np; t += blockDim.x) {
+        const int i = t / np, k = t % np;
+        const double s = rot[4 * k + 1];
+        if (s == 0.0) continue;
+        int p, q;
+        rr_pair(r, k, n2, p, q);
+        const double c = rot[4 * k];
+        const cd e = mk<double>(rot[4 * k + 2], rot[4 * k + 3]);
+        const cd x = A[i * n + p], y = A[i * n + q];
+        const cd ey = e * y;
+        A[i * n + p] = mk<double>(c * x.re - s * ey.re, c * x.im - s * ey.im);
+        A[i * n + q] = mk<double>(s * x.re + c * ey.re, s * x.im + c * ey.im);
+        if (V) {
+          const cd vx = V[i * n + p], vy = V[i * n + q];
+          const cd evy = e * vy;
+          V[i * n + p] = mk<double>(c * vx.re - s * evy.re, c * vx.im - s * evy.im);
+          V[i * n + q] = mk<double>(s * vx.re + c * evy.re, s * vx.im + c * evy.im);
+        }
+      }
+      __syncthreads();
+      // A <- J^H A (rows p, q)
+      for (int t = threadIdx.x; t < n * np; t += blockDim.x) {
+        const int j = t / np, k = t % np;
+        const double s = rot[4 * k + 1];
+        if (s == 0.0) continue;
+        int p, q;
+        rr_pair(r, k, n2, p, q);
+        const double c = rot[4 * k];
+        const cd ec = mk<double>(rot[4 * k + 2], -rot[4 * k + 3]);
+        const cd x = A[p * n + j], y = A[q * n + j];
+        const cd ey = ec * y;
+        A[p * n + j] = mk<double>(c * x.re - s * ey.re, c * x.im - s * ey.im);
+        A[q * n + j] = mk<double>(s * x.re + c * ey.re, s * x.im + c * ey.im);
+      }
+      __syncthreads();
+      for (int k = threadIdx.x; k < np; k += blockDim.x) {
+        if (rot[4 * k + 1] == 0.0) continue;
+        int p, q;
+        rr_pair(r, k, n2, p, q);
+        A[p * n + q] = mk<double>(0.0, 0.0);
+        A[q * n + p] = mk<double>(0.0, 0.0);
+        A[p * n + p].im = 0.0;
+        A[q * n + q].im = 0.0;
+      }
+      __syncthreads();
+    }
+    const int any = *flag;
+    __syncthreads();
+    if (!any) break;
+  }
+}
+
+// One-sided Jacobi SVD.  G: m x n column-major (ldg), m >= n, overwritten.
+// Outputs sigma (descending), U (m x n, column-major, ldu; zero columns for
+// sigma == 0) and the UNSORTED right vectors V (n x n column-major): with
+// perm from this call, G_in = sum_j sigma[j] U[:, j] V[:, perm[j]]^H.
+// rot: >= n doubles scratch (norms); flag: 1 int; perm: n ints.
+__device__ inline void jacobi_svd(cd* G, int ldg, int m, int n, cd* U, int ldu, cd* V, double* sigma, double* rot,
+                                  int* flag, int* perm) {
+  const int n2 = n + (n & 1), np = n2 / 2;
+  for (int i = threadIdx.x; i < n * n; i += blockDim.x) V[i] = mk<double>((i / n) == (i % n) ? 1.0 : 0.0, 0.0);
+  // threads per pair: power of two, <= 64, groups aligned inside waves
+  int tpp = 1;
+  while (tpp * 2 <= 64 && tpp * 2 * np <= (int)blockDim.x) tpp *= 2;
+  const int grp = threadIdx.x / tpp, lane = threadIdx.x % tpp;
+  __syncthreads();
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    if (threadIdx.x == 0) *flag = 0;
+    __syncthreads();
+    for (int r = 0; r < n2 - 1; ++r) {
+      for (int kb = 0; kb < np; kb += (int)blockDim.x / tpp) {
+        const int k = kb + grp;
+        int p = 0, q = 0;
+        const bool live = (grp < (int)blockDim.x / tpp) && (k < np);
+        if (live) rr_pair(r, k, n2, p, q);
+        const bool act = live && q < n;
+        double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+        if (act) {
+          const cd* gp = G + (long long)p * ldg;
+          const cd* gq = G + (long long)q * ldg;
+          for (int i = lane; i < m; i += tpp) {
+            const cd x = gp[i], y = gq[i];
+            al += norm2(x);
+            be += norm2(y);
+            gr += x.re * y.re + x.im * y.im;
+            gi += x.re * y.im - x.im * y.re;
+          }
+        }
+        for (int o = tpp / 2; o > 0; o >>= 1) {
+          al += __shfl_xor(al, o, 64);
+          be += __shfl_xor(be, o, 64);
+          gr += __shfl_xor(gr, o, 64);
+          gi += __shfl_xor(gi, o, 64);
+        }
+        double c, s, er, ei;
+        if (act && jacobi_params(al, be, mk<double>(gr, gi), c, s, er, ei)) {
+          if (lane == 0) *flag = 1;
+          const cd e = mk<double>(er, ei);
+          cd* gp = G + (long long)p * ldg;
+          cd* gq = G + (long long)q * ldg;
+          for (int i = lane; i < m; i += tpp) {
+            const cd x = gp[i], ey = e * gq[i];
+            gp[i] = mk<double>(c * x.re - s * ey.re, c * x.im - s * ey.im);
+            gq[i] = mk<double>(s * x.re + c * ey.re, s * x.im + c * ey.im);
+          }
+          cd* vp = V + (long long)p * n;
+          cd* vq = V + (long long)q * n;
+          for (int i = lane; i < n; i += tpp) {
+            const cd x = vp[i], ey = e * vq[i];
+            vp[i] = mk<double>(c * x.re - s * ey.re, c * x.im - s * ey.im);
+            vq[i] = mk<double>(s * x.re + c * ey.re, s * x.im + c * ey.im);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    const int any = *flag;
+    __syncthreads();
+    if (!any) break;
+  }
+  // column norms -> sigma, then sort descending
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    double s = 0.0;
+    for (int i = 0; i < m; ++i) s += norm2(G[(long long)j * ldg + i]);
+    rot[j] = sqrt(s);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < n; ++j) perm[j] = j;
+    for (int a = 0; a < n; ++a) {
+      int best = a;
+      for (int b = a + 1; b < n; ++b)
+        if (rot[perm[b]] > rot[perm[best]]) best = b;
+      int t = perm[a];
+      perm[a] = perm[best];
+      perm[best] = t;
+    }
+    for (int j = 0; j < n; ++j) sigma[j] = rot[perm[j]];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < m * n; t += blockDim.x) {
+    const int j = t / m, i = t % m;
+    const double s = sigma[j];
+    const cd g = G[(long long)perm[j] * ldg + i];
+    U[(long long)j * ldu + i] = s > 0.0 ? mk<double>(g.re / s, g.im / s) : mk<double>(0.0, 0.0);
+  }
+  __syncthreads();
+}
+
+// V columns permuted by perm: out[:, j] = V[:, perm[j]] (separate buffer).
+__device__ inline void permute_cols(const cd* V, cd* out, int n, const int* perm) {
+  for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+    const int j = t / n, i = t % n;
+    out[(long long)j * n + i] = V[(long long)perm[j] * n + i];
+  }
+  __syncthreads();
+}
+
+// Householder QR in place (column-major A, lda).  beta[j] = R_jj; u_j stored
+// in A[j.., j] (|u_j|^2 = 2, H_j = I - u_j u_j^H, Hermitian and unitary);
+// strict upper triangle = R.  red: blockDim/64 doubles.
+__device__ inline void householder_qr(cd* A, int lda, int m, int n, cd* beta, double* red) {
+  const int k = m < n ? m : n;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
+  for (int j = 0; j < k; ++j) {
+    cd* col = A + (long long)j * lda;
+    double s = 0.0;
+    for (int i = j + threadIdx.x; i < m; i += blockDim.x) s += norm2(col[i]);
+    const double nrm2 = block_sum(s, red);
+    const cd alpha = col[j];
+    const double nrm = sqrt(nrm2);
+    const double aa = sqrt(norm2(alpha));
+    cd b;
+    double scale = 0.0;
+    if (nrm2 > 0.0) {
+      // beta = -e^{i arg alpha} |x|;  u = x - beta e_j;  |u|^2 = 2 |x| (|x| + |alpha|)
+      if (aa > 0.0)
+        b = mk<double>(-alpha.re / aa * nrm, -alpha.im / aa * nrm);
+      else
+        b = mk<double>(-nrm, 0.0);
+      scale = sqrt(2.0) / sqrt(2.0 * nrm * (nrm + aa));
+    } else {
+      b = mk<double>(0.0, 0.0);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) beta[j] = b;
+    for (int i = j + threadIdx.x; i < m; i += blockDim.x) {
+      cd x = col[i];
+      if (i == j) x = x - b;
+      col[i] = scale * x;
+    }
+    __syncthreads();
+    if (nrm2 > 0.0) {
+      for (int c = j + 1 + w; c < n; c += nw) {
+        cd* y = A + (long long)c * lda;
+        double sr = 0.0, si = 0.0;
+        for (int i = j + lane; i < m; i += 64) {
+          const cd u = col[i], v = y[i];
+          sr += u.re * v.re + u.im * v.im;
+          si += u.re * v.im - u.im * v.re;
+        }
+        sr = wave_sum(sr);
+        si = wave_sum(si);
+        const cd dot = mk<double>(sr, si);
+        for (int i = j + lane; i < m; i += 64) y[i] = y[i] - col[i] * dot;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Y (m x c column-major, ldy) <- H_0 H_1 ... H_{k-1} Y.
+__device__ inline void householder_apply_q(const cd* A, int lda, int m, int k, cd* Y, int ldy, int c) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
+  for (int j = k - 1; j >= 0; --j) {
+    const cd* col = A + (long long)j * lda;
+    for (int cc = w; cc < c; cc += nw) {
+      cd* y = Y + (long long)cc * ldy;
+      double sr = 0.0, si = 0.0;
+      for (int i = j + lane; i < m; i += 64) {
+        const cd u = col[i], v = y[i];
+        sr += u.re * v.re + u.im * v.im;
+        si += u.re * v.im - u.im * v.re;
+      }
+      sr = wave_sum(sr);
+      si = wave_sum(si);
+      const cd dot = mk<double>(sr, si);
+      for (int i = j + lane; i < m; i += 64) y[i] = y[i] - col[i] * dot;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace dq

@@ -1,0 +1,154 @@
+/* libdq -- MI355X (gfx950) belief-propagation / Vidal-gauge / simple-update
+ * engine for graph tensor networks.  C ABI: plain pointers and sizes.
+ *
+ * Conventions
+ *   - Every compute entry point takes caller-owned DEVICE buffers, a stream
+ *     (hipStream_t passed as void*, NULL = default stream) and returns a
+ *     status (DQ_OK = 0).  On error, dq_last_error() returns a thread-local
+ *     description.  Nothing is synchronised unless stated.
+ *   - dtype: DQ_C64 (complex64 storage, float arithmetic for the bulk
+ *     contractions) or DQ_C128 (complex128).  Eigen/singular decompositions,
+ *     QR, lambdas and residuals are always FP64.
+ *   - A graph handle fixes the topology, a padded bond dimension chi and a
+ *     uniform physical dimension phys.  Node tensors live in one buffer at
+ *     offsets from dq_graph_tensor_offsets, row-major (bond_0..bond_{d-1},
+ *     phys) with bonds in the node's adjacency order, each padded to chi.
+ *     Messages: n2e/e2n buffers of 2*n_edges chi x chi matrices, index
+ *     (edge*2 + side) where side is the node's position in the edge; entries
+ *     M[bra][ket].  Lambdas: n_edges x chi doubles.
+ *   - Several graphs may be packed into one handle as instances (nodes and
+ *     edges of an instance contiguous); per-instance results are indexed by
+ *     instance.  inst_active (device int[n_inst], may be NULL) masks work of
+ *     inactive instances (their outputs are left untouched).
+ *   - A handle owns its topology arrays and a scratch workspace; it is not
+ *     thread-safe, distinct handles are.  The library never frees caller
+ *     memory.
+ */
+#ifndef DQ_H
+#define DQ_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DQ_OK 0
+#define DQ_ERR_INVALID 1     /* bad argument / shape */
+#define DQ_ERR_HIP 2         /* HIP runtime error */
+#define DQ_ERR_UNSUPPORTED 3 /* valid request outside what is implemented */
+
+#define DQ_C64 0
+#define DQ_C128 1
+
+typedef struct dq_graph dq_graph;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int dq_version(void);
+
+/* Description of the last error raised on the calling thread. */
+const char* dq_last_error(void);
+
+/* Topology (host arrays, copied):
+ *   deg[n_nodes], adj_edge/adj_side[sum deg] (bond slots in adjacency order),
+ *   edge_node[2*n_edges] (node at side 0, side 1), node_inst[n_nodes]
+ *   (non-decreasing instance ids in [0, n_inst)).
+ * Replaces the graph container of dequantizer/src/graph/tensor_graph.py:31-94
+ * (TensorGraph.add_node/add_edge; bond order = insertion order, node.py:19-27). */
+int dq_graph_create(dq_graph** out, int n_nodes, int n_edges, int chi, int phys, const int* deg,
+                    const int* adj_edge, const int* adj_side, const int* edge_node, const int* node_inst,
+                    int n_inst);
+int dq_graph_destroy(dq_graph* g);
+
+/* Total complex elements of the padded node-tensor buffer; per-node offsets
+ * (host long long[n_nodes]). */
+long long dq_graph_tensor_elems(const dq_graph* g);
+int dq_graph_tensor_offsets(const dq_graph* g, long long* offsets);
+
+/* BP sweep schedule (host arrays, copied).  Nodes are processed level by
+ * level; node_src[slot] selects each incoming message: 0 = e2n_in of that
+ * slot, 1 = this sweep's n2e_out of the partner node, 2 = n2e_in of the
+ * partner; edge_src[2*e+s] selects the source of e2n_out[e][s]: 0 = n2e_out,
+ * 1 = n2e_in (of the partner side).  All-zero node_src, all-one edge_src and
+ * one level reproduce the synchronous sweep; the asynchronous traversal-order
+ * sweep of dequantizer/src/mappings/belief_propagation.py:24-64 is expressed
+ * by the levels of its dependency DAG. */
+int dq_bp_set_schedule(dq_graph* g, int n_levels, const int* level_ptr, const int* level_nodes,
+                       const unsigned char* node_src, const unsigned char* edge_src);
+
+/* One BP sweep: replaces get_belief_propagation_map(...)(tensors, messages)
+ * (dequantizer/src/mappings/belief_propagation.py:24-64), node update
+ * pass_through_node (tensor_ops.py:36-60), edge update pass_through_edge
+ * (tensor_ops.py:70-76).  Output buffers must not alias inputs. */
+int dq_bp_sweep(dq_graph* g, int dtype, const void* tensors, const void* n2e_in, const void* e2n_in, void* n2e_out,
+                void* e2n_out, const int* inst_active, void* stream);
+
+/* Mean Frobenius distance between two message sets, per instance (device
+ * double[n_inst]).  Replaces messages_frob_distance
+ * (dequantizer/src/mappings/messages_distance.py:14-22). */
+int dq_messages_distance(dq_graph* g, int dtype, const void* n2e_a, const void* e2n_a, const void* n2e_b,
+                         const void* e2n_b, double* out, void* stream);
+
+/* n2e = e2n = diag(lambda) on every edge.  Replaces lambdas2messages
+ * (dequantizer/src/mappings/symmetric_gauge.py:50-61). */
+int dq_lambdas_to_messages(dq_graph* g, int dtype, const double* lam, void* n2e, void* e2n, void* stream);
+
+/* Vidal gauge from converged node->edge messages: gammas (tensor buffer) and
+ * lambdas.  Replaces get_vidal_gauge_fixing_map (mappings/vidal_gauge.py:21-63)
+ * = edge_svd (tensor_ops.py:108-130) + absorb_by_node (tensor_ops.py:88-93). */
+int dq_vidal_gauge(dq_graph* g, int dtype, const void* tensors, const void* n2e, double eps, void* gammas,
+                   double* lam, const int* inst_active, void* stream);
+
+/* Distance to the Vidal gauge per instance (device double[n_inst]).  Replaces
+ * get_vidal_gauge_distance_map (mappings/vidal_distance.py:17-42) = vidal_dist
+ * (tensor_ops.py:143-175). */
+int dq_vidal_distance(dq_graph* g, int dtype, const void* gammas, const double* lam, double eps, double* out,
+                      const int* inst_active, void* stream);
+
+/* Symmetric gauge.  Replaces get_symmetric_gauge_fixing_map
+ * (mappings/symmetric_gauge.py:17-39) = canonicalize (tensor_ops.py:189-199). */
+int dq_symmetric_gauge(dq_graph* g, int dtype, const void* gammas, const double* lam, double eps, void* out,
+                       void* stream);
+
+/* One-site gates on `count` nodes (device int nodes[count], device complex128
+ * gates[count][phys][phys], out x in).  Replaces TensorGraph.apply1
+ * (dequantizer/src/graph/tensor_graph.py:268-275). */
+int dq_apply1(dq_graph* g, int dtype, void* tensors, const int* nodes, int count, const void* gates, void* stream);
+
+/* Simple update of `count` two-site gates on pairwise node-disjoint edges
+ * (device int gates[count][4] = {edge, side of the controlling node, gate rank
+ * r, 0}; device complex128 halves[count][2][phys^4], each half (r, out, in)
+ * from the operator-Schmidt split).  Updates tensors (Vidal Gammas) and
+ * lambdas in place; per-gate truncation error and entropy (device double).
+ * Replaces TensorGraph.apply2_to_vidal_canonical (tensor_graph.py:294-375) =
+ * simple_update (tensor_ops.py:318-389) with accuracy None and
+ * max_rank = max_rank (capped at chi). */
+int dq_simple_update(dq_graph* g, int dtype, void* tensors, double* lam, const int* gates, int count,
+                     const void* halves, double eps, int max_rank, double* err, double* entropy, void* stream);
+
+/* One-site density matrices (device complex128 rho[count][phys][phys]) from
+ * tensors and edge->node messages.  Replaces get_one_side_density_matrix
+ * (dequantizer/src/mappings/density_matrix.py:16-37). */
+int dq_rdm1(dq_graph* g, int dtype, const void* tensors, const void* e2n, const int* nodes, int count, void* rho,
+            void* stream);
+
+/* Two-site density matrices of edges (device complex128
+ * rho[count][phys^2][phys^2], rows (s_side0, s_side1)).  No reference
+ * counterpart (the reference exports only the one-site matrix,
+ * dequantizer/src/mappings/__init__.py:1-6). */
+int dq_rdm2(dq_graph* g, int dtype, const void* tensors, const void* e2n, const int* edges, int count, void* rho,
+            void* stream);
+
+/* Convergence bookkeeping of the BP-to-Vidal loop
+ * (dequantizer/src/quantum_emulator/bp_quantum_emulator.py:253-266): for each
+ * active instance, iters += 1 and active = dist > acc && iters < max_iters
+ * (max_iters < 0: no cap).  All device arrays. */
+int dq_update_active(const double* dist, double acc, int* iters, int max_iters, int* active, int n_inst,
+                     void* stream);
+
+/* Copy node tensors of handle `old` (smaller chi) into the padded layout of
+ * `g` (same topology, chi >= old chi); new slots are zero. */
+int dq_repad(const dq_graph* old, dq_graph* g, int dtype, const void* tensors_old, void* tensors_new, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DQ_H */
