@@ -1,0 +1,226 @@
+"""Benchmark: forward BP annealing run on a 1000-spin random 3-regular graph.
+
+Workload (one GPU): N = 1000 spins, random 3-regular graph (1500 edges),
+J in {-1, +1}, h ~ N(0, 0.1^2), transverse-field annealing with dt = 0.1 over a
+200-step linear schedule, chi = 16, complex64 contractions with FP64
+decompositions, regauge (BP to Vidal distance 1e-5, at most 100 sweeps) once
+per layer.  A timed step is one Trotter step: regauge + 1500 simple updates
++ 1000 one-site gates.  The metric is edge-updates/s (one EU = one directed
+node->edge message recomputation) over the whole timed run.
+
+Prints one JSON line.  ``--gpus`` above 1 is refused: this build measures one
+GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "edge-updates/s & SU gates/s at χ=16 vs plain GPU and CPU; fwd+adjoint time"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def eu_flops(d: int, chi: int, p: int) -> int:
+    """Algorithmic FLOPs of one edge update: d-1 message contractions and one
+    Gram, each chi^(d+1) p complex multiply-adds (8 real flops)."""
+    return 8 * d * chi ** (d + 1) * p
+
+
+def eu_bytes(d: int, chi: int, p: int, elem: int = 8) -> float:
+    """Algorithmic HBM bytes of one edge update: the node tensor and its d
+    incoming messages read once and amortised over its d outputs, plus one
+    message written."""
+    return elem * (chi ** d * p + d * chi * chi) / d + elem * chi * chi
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--nodes", type=int, default=1000)
+    ap.add_argument("--chi", type=int, default=16)
+    ap.add_argument("--schedule-steps", type=int, default=200)
+    ap.add_argument("--dt", type=float, default=0.1)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--traversal", default="bfs")
+    ap.add_argument("--sync", action="store_true", help="synchronous BP sweeps")
+    ap.add_argument("--fixed-sweeps", type=int, default=10,
+                    help="BP sweeps per regauge with the convergence test off (fixed work); 0 = converge to 1e-5")
+    ap.add_argument("--cpu-baseline", type=int, default=1, help="run the oracle CPU leg (1) or not (0)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: every box in this setup has one GPU; "
+              "multi-GPU sharding is described in DESIGN.md, not run", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from dequantizer_amd.annealing import Annealer, get_scheduler, linear_schedule, random_regular
+    from dequantizer_amd import PRNGKey
+
+    torch.cuda.init()
+    ef = random_regular(args.nodes, 3, seed=args.seed, couplings="pm1", field_std=0.1)
+    schedule = get_scheduler(args.dt, linear_schedule, args.schedule_steps)
+    if args.warmup + args.steps > len(schedule):
+        raise SystemExit("warmup + steps exceeds the schedule length")
+    t_build = time.perf_counter()
+    ann = Annealer([ef], [PRNGKey(args.seed)], max_chi=args.chi, layers_per_regauging=1, accuracy=1e-5,
+                   max_belief_propagation_iterations=100, synchronous_update=args.sync,
+                   traversal_type=args.traversal, dtype=torch.complex64, chi=args.chi,
+                   fixed_sweeps=args.fixed_sweeps or None)
+    emu = ann.emu
+    for k in range(args.warmup):
+        ann.step(*schedule[k])
+        emu._flush()
+        torch.cuda.synchronize()
+        print(f"bench: warmup step {k} done, chi={emu.chi}, bp sweeps so far {sum(emu.bp_iterations[0])}",
+              file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t_build
+
+    it_before = sum(emu.bp_iterations[0])
+    emu.sweep_events = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.warmup, args.warmup + args.steps):
+        ann.step(*schedule[k])
+        if k % 2 == 0:
+            print(f"bench: timed step {k} queued", file=sys.stderr, flush=True)
+    emu._flush()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    sweeps = sum(emu.bp_iterations[0]) - it_before
+    n_eu = sweeps * 2 * ef.edges_number
+    n_gates = args.steps * ef.edges_number
+    # live per-launch duration of the BP sweep (HIP events on torch's stream)
+    durs = []
+    for ev0, ev1, act in emu.sweep_events:
+        if int(act.sum().item()) > 0:
+            durs.append(ev0.elapsed_time(ev1) * 1e-3)
+    emu.sweep_events = None
+    d, p, chi = 3, 2, args.chi
+    flops_per_sweep = 2 * ef.edges_number * eu_flops(d, chi, p)
+    bytes_per_sweep = 2 * ef.edges_number * eu_bytes(d, chi, p)
+    sweep_s = float(np.mean(durs)) if durs else float("nan")
+    achieved_tflops = flops_per_sweep / sweep_s / 1e12 if durs else None
+
+    result = {
+        "metric": METRIC,
+        "value": n_eu / elapsed,
+        "unit": "edge-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {
+            "workload": "BP annealing, random 3-regular N=%d (%d edges), chi=%d, dt=%g, %d-step linear schedule, "
+                        "steps %d..%d timed" % (args.nodes, ef.edges_number, chi, args.dt, args.schedule_steps,
+                                               args.warmup, args.warmup + args.steps - 1),
+            "nodes": args.nodes, "edges": ef.edges_number, "chi": chi, "phys": p,
+            "bp_schedule": ("sync" if args.sync else "async-" + args.traversal),
+            "bp_accuracy": 1e-5, "regauge": "every layer",
+            "sweeps_per_regauge": (args.fixed_sweeps or "to convergence"),
+        },
+        "su_gates_per_s": n_gates / elapsed,
+        "bp_sweeps_timed": int(sweeps),
+        "bp_eu_per_s_in_sweeps": (2 * ef.edges_number / sweep_s) if durs else None,
+        "setup_and_warmup_s": t_build,
+        "roofline": {
+            "bound": "mfma",
+            "kernel": "BP sweep (k_bp_nodes + k_bp_edges)",
+            "achieved": achieved_tflops,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": (achieved_tflops / PEAK_FP32_TFLOPS) if achieved_tflops else None,
+            "traffic": None,
+            "algorithmic_flops_per_launch": flops_per_sweep,
+            "algorithmic_bytes_per_launch": bytes_per_sweep,
+            "avg_launch_s": sweep_s,
+            "launches": len(durs),
+        },
+        "cpu_baseline": None,
+    }
+    if args.cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(ann, ef, schedule[args.warmup + args.steps], args)
+    print(json.dumps(result))
+
+
+def cpu_baseline(ann, ef, step_times, args):
+    """The oracle (NumPy complex128 restatement) on one Trotter step of the
+    same instance, started from the GPU state: regauge + all simple updates
+    + mixing gates.  Reports EU/s = oracle BP EU in that step / its time."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import torch
+    from oracle import annealing as oa
+    from oracle import emulator as oe
+    from oracle import graph as og
+
+    emu = ann.emu
+    emu._flush()
+    topo = emu.topo
+    chi = emu.chi
+    g = og.graph_from_pairs(ef.nodes_number, ef.coupled_spin_pairs)
+    gam = emu.gam.detach().cpu().numpy().astype(np.complex128)
+    lam = emu.lam.detach().cpu().numpy().reshape(-1, chi)
+    tensors, lambdas = {}, {}
+    for j, e in enumerate(topo.edges):
+        d = int(emu.dims[j])
+        lambdas[e.id] = lam[j, :d].copy()
+        g.edges[e.id].dimension = d
+    for i, node in enumerate(topo.nodes):
+        deg = int(topo.deg[i])
+        off = int(emu.h.offsets[i])
+        full = gam[off: off + chi ** deg * 2].reshape((chi,) * deg + (2,))
+        dims = [int(emu.dims[topo.adj_edge[k]]) for k in range(topo.adj_ptr[i], topo.adj_ptr[i + 1])]
+        tensors[node.id] = full[tuple(slice(0, x) for x in dims) + (slice(None),)].copy()
+        g.nodes[node.id].bond_shape = tuple(dims)
+    # oracle emulator with this state (constructed on |0>, then replaced)
+    o = oe.BPQuantumEmulator.__new__(oe.BPQuantumEmulator)
+    o.tensor_graph = g
+    o.prng_key = og.make_key(args.seed)
+    o.max_chi, o.max_iters, o.per_regauge, o.acc = args.chi, 100, ef.edges_number, 1e-5
+    o.gates_since = ef.edges_number
+    o.traverser = g.traversal(ordering=args.traversal)
+    from oracle import mappings as om
+    o.bp_map = om.belief_propagation_map(o.traverser, args.sync)
+    o.vg_map = om.vidal_gauge_fixing_map(o.traverser, 1e-10)
+    o.vd_map = om.vidal_gauge_distance_map(o.traverser, 1e-10)
+    o.sg_map = om.symmetric_gauge_fixing_map(o.traverser, 1e-10)
+    o.tensors, o.lambdas = tensors, lambdas
+    o.vidal_distances_after_regauging, o.truncation_affected_vidal_distances = [], []
+    o.truncation_errors, o.entropies, o.bp_iterations = [], [], []
+    o._initial_messages = None
+    o.fixed_sweeps = args.fixed_sweeps or None
+    mix, cpl = step_times
+    deg = [g.nodes[i].degree for i in range(ef.nodes_number)]
+    t0 = time.perf_counter()
+    for (a, b), jv in zip(ef.coupled_spin_pairs, ef.coupling_amplitudes):
+        a, b = int(a), int(b)
+        o.apply_q2(oa.coupling_gate(jv, ef.fields[a], ef.fields[b], deg[a], deg[b], cpl), a, b)
+    mg = oa.mixing_gate(mix)
+    for i in range(ef.nodes_number):
+        o.apply_q1(mg, i)
+    dt = time.perf_counter() - t0
+    sweeps = sum(o.bp_iterations)
+    eu = sweeps * 2 * ef.edges_number
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": eu / dt, "unit": "edge-updates/s", "cores": threads, "kind": "port",
+            "sample": "one Trotter step of the same instance on the oracle (NumPy complex128), started from the "
+                      "GPU state after the timed steps: regauge (%d BP sweeps) + %d simple updates + %d one-site "
+                      "gates, %.1f s" % (sweeps, ef.edges_number, ef.nodes_number, dt),
+            "seconds": dt, "gates_per_s": ef.edges_number / dt}
+
+
+if __name__ == "__main__":
+    main()
