@@ -139,9 +139,10 @@ static inline int grid_for(long long total, int block) {
 
 hipError_t launch_bp_nodes(int rb, const DevGraph& g, const int* nodes, int count, const void* T, const void* n2e_old,
                            const void* e2n_old, void* n2e_new, const uint8_t* node_src, void* ws, long long wsb,
-                           const int* inst_active, int max_deg, hipStream_t s) {
-  (void)max_deg;
+                           const int* inst_active, int min_deg, int max_deg, hipStream_t s) {
   if (count <= 0) return hipSuccess;
+  if (bp_fast_applicable(rb, g, min_deg, max_deg))
+    return launch_bp_nodes_fast(g, nodes, count, T, n2e_old, e2n_old, n2e_new, node_src, inst_active, s);
   if (rb == 4)
     hipLaunchKernelGGL(k_bp_nodes<float>, dim3(count), dim3(256), 0, s, g, nodes, (const cx<float>*)T,
                        (const cx<float>*)n2e_old, (const cx<float>*)e2n_old, (cx<float>*)n2e_new, node_src,

@@ -1,0 +1,174 @@
+// BP node update on CDNA4 matrix cores for degree-3 nodes, chi = 16,
+// phys = 2, complex64 (FP32 accumulation).
+//
+// Same arithmetic as the reference pass_through_node
+// (dequantizer/src/tensor_ops.py:36-60): for every bond idx,
+// U = T x_{j != idx} M_j (message j applied on the ket side), out[k,l] =
+// sum conj(T)[..k..] U[..l..] over the other axes, out /= trace(out).
+//
+// One 256-thread block (4 waves) per node.  T (16x16x16x2 complex) and one
+// work tensor live in LDS with a padded layout: element (b0, b1, b2, s) at
+// b0*529 + b1*33 + b2*2 + s (complex), so that the 16 lanes of an MFMA
+// operand fetch that walk along any bond axis hit distinct banks.
+//   mode product  X x_j M: a GEMM (16 x 16) x (16 x 512) split in 32 column
+//                 tiles; per tile 4 k-steps of V_MFMA_F32_16X16X4_F32, each
+//                 complex product = 4 real MFMAs;
+//   Gram          (16 x 512) x (512 x 16), K split over the 4 waves and
+//                 reduced through LDS.
+// Per node: 6 mode products + 3 Grams (U_0 = (T x_2 M2) x_1 M1,
+// U_1 = (T x_2 M2) x_0 M0, U_2 = (T x_0 M0) x_1 M1, the second product of each
+// in place).
+#include "dq_launch.hpp"
+
+namespace dq {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+namespace fast3 {
+constexpr int RP = 33;              // pitch of a (b0, b1) fiber of 32 complex values
+constexpr int B0P = 16 * RP + 1;    // pitch of b0
+constexpr int NPAD = 16 * B0P;      // padded complex elements of one tensor
+constexpr int S = 16 * 16 * 16 * 2;  // unpadded complex elements
+
+__device__ __forceinline__ int axis_stride(int j) { return j == 0 ? B0P : (j == 1 ? RP : 2); }
+
+// padded offset of column c (0..511) of the matrix view of mode j: columns
+// enumerate the remaining axes in row-major order.
+__device__ __forceinline__ int col_base(int j, int c) {
+  if (j == 0) return (c >> 5) * RP + (c & 31);
+  if (j == 1) return (c >> 5) * B0P + (c & 31);
+  return (c >> 5) * B0P + ((c >> 1) & 15) * RP + (c & 1);
+}
+
+__device__ __forceinline__ floatx4 mfma(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// dst = src x_j M (may be in place).  m[ks] = M[lane%16][4*ks + lane/16].
+__device__ __forceinline__ void mode_product(float2* dst, const float2* src, const float2 (&m)[4], int j, int wave,
+                                             int lane) {
+  const int st = axis_stride(j);
+  const int col = lane & 15, kq = lane >> 4;
+  for (int tile = wave; tile < 32; tile += 4) {
+    const int cb = col_base(j, tile * 16 + col);
+    floatx4 ar = {0.f, 0.f, 0.f, 0.f}, ai = {0.f, 0.f, 0.f, 0.f};
+    float2 x[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) x[ks] = src[cb + (4 * ks + kq) * st];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      ar = mfma(m[ks].x, x[ks].x, ar);
+      ar = mfma(-m[ks].y, x[ks].y, ar);
+      ai = mfma(m[ks].x, x[ks].y, ai);
+      ai = mfma(m[ks].y, x[ks].x, ai);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) dst[cb + (4 * kq + v) * st] = make_float2(ar[v], ai[v]);
+  }
+}
+
+// Partial Gram of this wave: sum over its 128 columns of conj(T[k]) U[l].
+__device__ __forceinline__ void gram_partial(const float2* t, const float2* u, int idx, int wave, int lane,
+                                             floatx4& gr, floatx4& gi) {
+  const int st = axis_stride(idx);
+  const int row = lane & 15, kq = lane >> 4;
+  gr = floatx4{0.f, 0.f, 0.f, 0.f};
+  gi = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int ks = wave * 32; ks < wave * 32 + 32; ++ks) {
+    const int off = col_base(idx, 4 * ks + kq) + row * st;
+    const float2 a = t[off], b = u[off];
+    // conj(a) * b
+    gr = mfma(a.x, b.x, gr);
+    gr = mfma(a.y, b.y, gr);
+    gi = mfma(a.x, b.y, gi);
+    gi = mfma(-a.y, b.x, gi);
+  }
+}
+}  // namespace fast3
+
+__global__ void __launch_bounds__(256)
+    k_bp_nodes_d3c16(DevGraph g, const int* nodes, const float2* T, const float2* n2e_old, const float2* e2n_old,
+                     float2* n2e_new, const uint8_t* node_src, const int* inst_active) {
+  using namespace fast3;
+  __shared__ float2 tl[NPAD];
+  __shared__ float2 wk[NPAD];
+  __shared__ float2 part[4][256];
+  const int node = nodes[blockIdx.x];
+  if (inst_active && !inst_active[g.node_inst[node]]) return;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int base = g.adj_ptr[node];
+  // stage T into LDS (padded)
+  const float2* tg = T + g.toff[node];
+  for (int f = tid; f < S; f += 256) {
+    const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
+    tl[b0 * B0P + b1 * RP + r] = tg[f];
+  }
+  // message operands in registers: m[j][ks] = M_j[lane%16][4 ks + lane/16]
+  float2 m[3][4];
+  int oe[3], os[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int e = g.adj_edge[base + k], s = g.adj_side[base + k];
+    oe[k] = e;
+    os[k] = s;
+    const int code = node_src ? node_src[base + k] : 0;
+    const float2* src = code == 0 ? e2n_old + (2LL * e + s) * 256
+                                  : (code == 1 ? n2e_new : n2e_old) + (2LL * e + 1 - s) * 256;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) m[k][ks] = src[(lane & 15) * 16 + 4 * ks + (lane >> 4)];
+  }
+  __syncthreads();
+  // (first product, second product, Gram axis) for idx = 0, 1, 2
+  const int plan[3][3] = {{2, 1, 0}, {2, 0, 1}, {0, 1, 2}};
+#pragma unroll
+  for (int step = 0; step < 3; ++step) {
+    const int j1 = plan[step][0], j2 = plan[step][1], idx = plan[step][2];
+    mode_product(wk, tl, m[j1], j1, wave, lane);
+    __syncthreads();
+    mode_product(wk, wk, m[j2], j2, wave, lane);
+    __syncthreads();
+    floatx4 gr, gi;
+    gram_partial(tl, wk, idx, wave, lane, gr, gi);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) part[wave][(4 * (lane >> 4) + v) * 16 + (lane & 15)] = make_float2(gr[v], gi[v]);
+    __syncthreads();
+    float2 sum = part[0][tid];
+    for (int w = 1; w < 4; ++w) {
+      sum.x += part[w][tid].x;
+      sum.y += part[w][tid].y;
+    }
+    float tr_r = 0.f, tr_i = 0.f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int q = k * 17;
+      float2 d = part[0][q];
+      for (int w = 1; w < 4; ++w) {
+        d.x += part[w][q].x;
+        d.y += part[w][q].y;
+      }
+      tr_r += d.x;
+      tr_i += d.y;
+    }
+    const float den = tr_r * tr_r + tr_i * tr_i;
+    float2 o;
+    o.x = (sum.x * tr_r + sum.y * tr_i) / den;
+    o.y = (sum.y * tr_r - sum.x * tr_i) / den;
+    n2e_new[(2LL * oe[idx] + os[idx]) * 256 + tid] = o;
+    __syncthreads();
+  }
+}
+
+bool bp_fast_applicable(int rb, const DevGraph& g, int min_deg, int max_deg) {
+  return rb == 4 && g.chi == 16 && g.phys == 2 && min_deg == 3 && max_deg == 3;
+}
+
+hipError_t launch_bp_nodes_fast(const DevGraph& g, const int* nodes, int count, const void* T, const void* n2e_old,
+                                const void* e2n_old, void* n2e_new, const uint8_t* node_src, const int* inst_active,
+                                hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bp_nodes_d3c16, dim3(count), dim3(256), 0, s, g, nodes, (const float2*)T,
+                     (const float2*)n2e_old, (const float2*)e2n_old, (float2*)n2e_new, node_src, inst_active);
+  return hipGetLastError();
+}
+
+}  // namespace dq

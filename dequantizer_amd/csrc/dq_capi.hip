@@ -34,6 +34,7 @@ static int fail(int code, const char* fmt, ...) {
 struct dq_graph {
   DevGraph dev;
   int max_deg = 0;
+  int min_deg = 0;
   int n_inst = 0;
   long long tensor_elems = 0;
   std::vector<long long> toff_host;
@@ -146,6 +147,9 @@ int dq_graph_create(dq_graph** out, int n_nodes, int n_edges, int chi, int phys,
   }
   dq_graph* g = new dq_graph();
   g->max_deg = max_deg;
+  g->min_deg = max_deg;
+  for (int n = 0; n < n_nodes; ++n)
+    if (deg[n] < g->min_deg) g->min_deg = deg[n];
   g->n_inst = n_inst;
   g->toff_host.resize(n_nodes);
   long long off = 0;
@@ -271,7 +275,7 @@ int dq_bp_sweep(dq_graph* g, int dtype, const void* T, const void* n2e_in, const
   for (int l = 0; l < g->n_levels; ++l) {
     const int b = g->level_ptr[l], cnt = g->level_ptr[l + 1] - b;
     RET_LAUNCH(launch_bp_nodes(rb, g->dev, g->d_level_nodes + b, cnt, T, n2e_in, e2n_in, n2e_out, g->d_node_src,
-                               g->ws, wsb, inst_active, g->max_deg, S(stream)));
+                               g->ws, wsb, inst_active, g->min_deg, g->max_deg, S(stream)));
   }
   RET_LAUNCH(launch_bp_edges(rb, g->dev, n2e_out, n2e_in, e2n_out, g->d_edge_src, inst_active, S(stream)));
   return DQ_OK;

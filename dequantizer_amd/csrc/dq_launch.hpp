@@ -13,8 +13,15 @@ namespace dq {
 // 0 = e2n_old[e][side], 1 = n2e_new[e][1-side], 2 = n2e_old[e][1-side].
 hipError_t launch_bp_nodes(int real_bytes, const DevGraph& g, const int* nodes, int count, const void* T,
                            const void* n2e_old, const void* e2n_old, void* n2e_new, const uint8_t* node_src,
-                           void* ws, long long ws_bytes_per_block, const int* inst_active, int max_deg,
+                           void* ws, long long ws_bytes_per_block, const int* inst_active, int min_deg, int max_deg,
                            hipStream_t s);
+
+// Matrix-core node update for graphs whose nodes all have degree 3, chi 16,
+// phys 2, complex64 (dq_bp_fast.hip).
+bool bp_fast_applicable(int real_bytes, const DevGraph& g, int min_deg, int max_deg);
+hipError_t launch_bp_nodes_fast(const DevGraph& g, const int* nodes, int count, const void* T, const void* n2e_old,
+                                const void* e2n_old, void* n2e_new, const uint8_t* node_src, const int* inst_active,
+                                hipStream_t s);
 
 // e2n_new[e][s] = (edge_src[2e+s] == 0 ? n2e_new : n2e_old)[e][1-s].
 hipError_t launch_bp_edges(int real_bytes, const DevGraph& g, const void* n2e_new, const void* n2e_old, void* e2n_new,

@@ -174,3 +174,21 @@ def test_ghz_tree_pipeline():
     for eid in lam:
         nz = int((host(lam[eid]) > acc).sum())
         assert nz == 3, (eid, host(lam[eid]))
+
+
+@pytest.mark.parametrize("sync,order", [(True, "bfs"), (False, "bfs"), (False, "dfs")])
+def test_bp_matrix_core_path_chi16(sync, order):
+    """Degree-3 graphs at chi=16, complex64 run the matrix-core node update;
+    compared with the complex128 oracle (FP32 accumulation, rel. 2e-5)."""
+    gp, go = regular_pair(24, 16, 11)
+    rng = np.random.default_rng(21)
+    tensors = random_tensors(gp, rng)
+    messages = random_messages(gp, rng)
+    trav_p = list(gp.get_traversal_iterator(ordering=order))
+    trav_o = go.traversal(ordering=order)
+    out_o = om.belief_propagation_map(trav_o, sync)(tensors, to_oracle_messages(messages))
+    bp_p = dq.get_belief_propagation_map(trav_p, sync)
+    out_p = bp_p({k: v.astype(np.complex64) for k, v in tensors.items()},
+                 {k: v.astype(np.complex64) for k, v in messages.items()})
+    worst = max(_rel(host(out_p[dq.MessageID(k.src, k.dst)]), v) for k, v in out_o.items())
+    assert worst < 2e-5, worst
