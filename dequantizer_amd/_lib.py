@@ -43,7 +43,7 @@ SIGNATURES = {
     "dq_vidal_distance": (c_int, [vp, c_int, vp, vp, c_dbl, vp, vp, vp]),
     "dq_symmetric_gauge": (c_int, [vp, c_int, vp, vp, c_dbl, vp, vp]),
     "dq_apply1": (c_int, [vp, c_int, vp, vp, c_int, vp, vp]),
-    "dq_simple_update": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, c_dbl, c_int, vp, vp, vp]),
+    "dq_simple_update": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, c_int, c_dbl, c_int, vp, vp, vp]),
     "dq_rdm1": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, vp]),
     "dq_rdm2": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, vp]),
     "dq_update_active": (c_int, [vp, c_dbl, vp, c_int, vp, c_int, vp]),

@@ -171,18 +171,34 @@ class Annealer:
             for i in range(self.efs[inst].nodes_number):
                 self.emu.queue_q1(inst, HADAMARD, i)
         self.steps_done = 0
+        # per-step op pattern: coupling gates in pair order, then mixing gates
+        two = [(inst, int(a), int(b)) for inst, ef in enumerate(self.efs) for a, b in ef.coupled_spin_pairs]
+        one = [(inst, i) for inst, ef in enumerate(self.efs) for i in range(ef.nodes_number)]
+        self._aligned = all(ef.edges_number * layers_per_regauging == per_i
+                            for ef, per_i in zip(self.efs, per)) and layers_per_regauging == 1
+        self.plan = self.emu.make_plan(two, one) if self._aligned else None
 
     def step(self, mixing_time: float, coupling_time: float):
         """One Trotter step on every instance."""
-        layer = []
-        for ef in self.efs:
-            gates = coupling_gates(ef, coupling_time)
-            layer.append([(gates[k], int(a), int(b)) for k, (a, b) in enumerate(ef.coupled_spin_pairs)])
-        self.emu.apply_layer(layer)
-        mg = mixing_gate(mixing_time)
-        for inst, ef in enumerate(self.efs):
-            for i in range(ef.nodes_number):
-                self.emu.queue_q1(inst, mg, i)
+        if self.plan is not None:
+            # regauge points of the reference fall on the first gate of the step
+            due = [i for i in range(len(self.efs)) if self.emu.gates_since[i] >= self.emu.per_regauge[i]]
+            self.emu._regauge(due)
+            gates = np.concatenate([coupling_gates(ef, coupling_time) for ef in self.efs])
+            halves, ranks = decompose_gates(gates)
+            mg = mixing_gate(mixing_time)
+            n1 = sum(ef.nodes_number for ef in self.efs)
+            self.emu.run_plan(self.plan, halves, ranks, np.broadcast_to(mg, (n1, 2, 2)))
+        else:
+            layer = []
+            for ef in self.efs:
+                gates = coupling_gates(ef, coupling_time)
+                layer.append([(gates[k], int(a), int(b)) for k, (a, b) in enumerate(ef.coupled_spin_pairs)])
+            self.emu.apply_layer(layer)
+            mg = mixing_gate(mixing_time)
+            for inst, ef in enumerate(self.efs):
+                for i in range(ef.nodes_number):
+                    self.emu.queue_q1(inst, mg, i)
         self.steps_done += 1
 
     def finish(self, two_site: bool = True) -> List[QuantumAnnealingResults]:

@@ -348,20 +348,22 @@ int dq_apply1(dq_graph* g, int dtype, void* T, const int* nodes, int count, cons
 }
 
 int dq_simple_update(dq_graph* g, int dtype, void* T, double* lam, const int* gates, int count, const void* halves,
-                     double eps, int max_rank, double* err, double* entropy, void* stream) {
+                     int max_gate_rank, double eps, int max_rank, double* err, double* entropy, void* stream) {
   int rb, rc;
   if (!g || count < 0 || max_rank < 1) return fail(DQ_ERR_INVALID, "bad argument");
+  if (max_gate_rank < 1 || max_gate_rank > g->dev.phys * g->dev.phys)
+    return fail(DQ_ERR_INVALID, "max_gate_rank %d outside [1, phys^2]", max_gate_rank);
   if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
   if (count == 0) return DQ_OK;
   const int chi = g->dev.chi, p = g->dev.phys;
-  const long long m = ipow(chi, g->max_deg > 0 ? g->max_deg - 1 : 0) * p;
-  const long long n = (long long)chi * p * p;
-  const long long k = m < n ? m : n;
-  const long long cdn = 2 * m * n + 2 * n + 3 * k * k + 2 * m * chi + 16;
-  const long long wsb = (cdn * (long long)sizeof(cd) + (4 * k + 64) * 8 + (k + 8) * 4 + 64 + 255) / 256 * 256;
+  if (chi * p > 64 || chi * p * p > 128)
+    return fail(DQ_ERR_UNSUPPORTED, "simple update supports chi*phys <= 64 and chi*phys^2 <= 128 (chi=%d, phys=%d)",
+                chi, p);
+  const bool lds = su_lds_fits(chi, p, max_gate_rank);
+  const long long wsb = lds ? 256 : su_ws_bytes(chi, p, max_gate_rank);
   if ((rc = ws_ensure(g, (size_t)wsb * count)) != DQ_OK) return rc;
-  RET_LAUNCH(launch_su(rb, g->dev, T, lam, (const SuGate*)gates, count, (const cd*)halves, eps, max_rank, err,
-                       entropy, g->ws, wsb, S(stream)));
+  RET_LAUNCH(launch_su(rb, g->dev, T, lam, (const SuGate*)gates, count, (const cd*)halves, max_gate_rank, eps,
+                       max_rank, err, entropy, g->ws, wsb, S(stream)));
   return DQ_OK;
 }
 

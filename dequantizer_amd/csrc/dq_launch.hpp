@@ -70,8 +70,14 @@ struct SuGate {
   int pad;
 };
 hipError_t launch_su(int real_bytes, const DevGraph& g, void* T, double* lam, const SuGate* gates, int count,
-                     const cd* halves /* per gate: 2 halves of phys^4 (r <= phys^2, each (r, phys, phys)) */, double eps, int max_rank, double* err,
+                     const cd* halves /* per gate: 2 halves of phys^4 (r <= phys^2, each (r, phys, phys)) */,
+                     int max_gate_rank, double eps, int max_rank, double* err,
                      double* entropy, void* ws, long long ws_bytes_per_block, hipStream_t s);
+
+// Per-block workspace of the simple update (bytes) and whether the small
+// FP64 matrices fit the LDS variant.
+long long su_ws_bytes(int chi, int phys, int max_r);
+bool su_lds_fits(int chi, int phys, int max_r);
 
 // One-site density matrices with messages e2n: rho[i] (phys x phys).
 hipError_t launch_rdm1(int real_bytes, const DevGraph& g, const void* T, const void* e2n, const int* nodes, int count,

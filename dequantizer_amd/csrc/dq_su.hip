@@ -51,171 +51,243 @@ __device__ inline int find_slot(const DevGraph& g, int node, int e, int side) {
   return -1;
 }
 
-// Workspace of one gate block (bytes), for nodes of degree <= max_deg.
-__host__ __device__ inline long long su_ws_bytes(int chi, int phys, int max_deg, int max_r) {
-  const long long m = ipow(chi, max_deg - 1 > 0 ? max_deg - 1 : 0) * phys;
-  const long long n = (long long)chi * max_r;
-  const long long k = m < n ? m : n;
-  const long long cdn = 2 * m * n      // A_a, A_b
-                        + 2 * n        // beta_a, beta_b
-                        + k * k        // G
-                        + k * k        // U
-                        + k * k        // V
-                        + 2 * m * chi  // Y_a, Y_b
-                        + 16;
-  return cdn * (long long)sizeof(cd) + (4 * k + 64) * sizeof(double) + (k + 8) * sizeof(int) + 64;
+// ---------------------------------------------------------------------------
+// Simple update, one block per gate.  The reference factorises each side
+// A = (T * lambda_others) x gate-half (rows: other bonds + phys, columns:
+// bond * r + q) by QR and takes the SVD of core = R_a diag(lambda) R_b^T
+// (tensor_ops.py:265-389).  Only R up to a left unitary matters, so R is taken
+// from the Gram matrix G = A^H A = V S^2 V^H as R = S V^H; G is assembled from
+// the environment E = X^H X of X = T * lambda_others (rows: other bonds,
+// columns: (bond, phys)) and the gate half.  With core = U Sigma V^H,
+//   Gamma_a = A_a Lambda R_b^T V Sigma^-1,   Gamma_b = A_b Lambda R_a^T conj(U) Sigma^-1
+// (the reference's Q_a U and Q_b conj(V)); A_a never needs to be formed:
+// Gamma_a[o, j, s'] = sum_{b,s} T[o, b, s] Z_a[(b,s), (j,s')] with
+// Z_a = sum_q half[q, s', s] W_a[(b,q), j].  Columns of Gamma for zero
+// singular values are set to zero (the reference leaves arbitrary
+// orthonormal columns there; they only ever meet a zero lambda).
+
+__device__ __forceinline__ double lam_mask(double l, double eps) {
+  // lambda * safe_inverse(lambda): 1 above eps, ~lambda / FLT_MAX below
+  return l * su_safe_inverse(l, eps);
 }
 
-// Build the normalised (others.., s_out) x (b * r + q) matrix of one side
-// (column-major) from the node tensor, the lambdas of its other bonds and the
-// gate half h[q][s_out][s_in].
+// Stage rows [r0, r0+nr) of X[o][(b,s)] = prod_{j != slot} lam_j[o_j] * T[o, b@slot, s]
+// into buf (nr x cp, row-major, complex double).  Rows enumerate the other
+// bonds in order, last one fastest.
 template <typename R>
-__device__ void su_build(const DevGraph& g, const cx<R>* t, int node, int slot, const double* lam, const cd* h, int r,
-                         cd* A, long long m, int n, double* red) {
-  const int d = g.deg[node], chi = g.chi, p = g.phys;
-  const int base = g.adj_ptr[node];
-  double acc = 0.0;
-  for (long long f = threadIdx.x; f < m * n; f += blockDim.x) {
-    const int col = (int)(f / m);
-    const long long row = f % m;
-    const int b = col / r, q = col % r;
-    const int so = (int)(row % p);
-    long long orest = row / p;
-    // decode the other bonds (last one fastest) and build the tensor index
-    long long tidx = 0, mul = p;
+__device__ void su_stage_rows(const DevGraph& g, const cx<R>* t, int node, int slot, const double* lam, bool weighted,
+                              long long r0, int nr, cd* buf) {
+  const int d = g.deg[node], chi = g.chi, p = g.phys, base = g.adj_ptr[node];
+  const int cp = chi * p;
+  for (int f = threadIdx.x; f < nr * cp; f += blockDim.x) {
+    const int rr = f / cp, col = f % cp;
+    const int b = col / p, s = col % p;
+    long long o = r0 + rr;
+    long long tidx = s, mul = p;
     double wt = 1.0;
     for (int j = d - 1; j >= 0; --j) {
       int bj;
       if (j == slot) {
         bj = b;
       } else {
-        bj = (int)(orest % chi);
-        orest /= chi;
-        wt *= lam[(long long)g.adj_edge[base + j] * chi + bj];
+        bj = (int)(o % chi);
+        o /= chi;
+        if (weighted) wt *= lam[(long long)g.adj_edge[base + j] * chi + bj];
       }
       tidx += bj * mul;
       mul *= chi;
     }
-    cd v = mk<double>(0, 0);
-    if (wt != 0.0)
-      for (int si = 0; si < p; ++si) fma_acc(v, h[(q * p + so) * p + si], cvt<double>(t[tidx + si]));
-    v = wt * v;
-    A[f] = v;
-    acc += norm2(v);
+    buf[f] = wt * cvt<double>(t[tidx]);
   }
-  const double nrm = sqrt(block_sum(acc, red));
-  const double inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
-  for (long long f = threadIdx.x; f < m * n; f += blockDim.x) A[f] = inv * A[f];
+}
+
+// E[(b,s),(b',s')] = sum_o conj(X[o][(b,s)]) X[o][(b',s')]  (cp x cp).
+template <typename R>
+__device__ void su_environment(const DevGraph& g, const cx<R>* t, int node, int slot, const double* lam, cd* E,
+                               cd* chunk, int chunk_rows) {
+  const int d = g.deg[node], chi = g.chi, p = g.phys;
+  const int cp = chi * p;
+  const long long rows = ipow(chi, d - 1);
+  const int per = (cp * cp + blockDim.x - 1) / blockDim.x;
+  cd acc[16];
+  for (int i = 0; i < per && i < 16; ++i) acc[i] = mk<double>(0, 0);
+  for (long long r0 = 0; r0 < rows; r0 += chunk_rows) {
+    const int nr = (int)((rows - r0) < chunk_rows ? (rows - r0) : chunk_rows);
+    su_stage_rows<R>(g, t, node, slot, lam, true, r0, nr, chunk);
+    __syncthreads();
+    for (int i = 0; i < per && i < 16; ++i) {
+      const int q = threadIdx.x + i * blockDim.x;
+      if (q >= cp * cp) break;
+      const int x = q / cp, y = q % cp;
+      cd a = acc[i];
+      for (int r = 0; r < nr; ++r) fma_cacc(a, chunk[r * cp + x], chunk[r * cp + y]);
+      acc[i] = a;
+    }
+    __syncthreads();
+  }
+  for (int i = 0; i < per && i < 16; ++i) {
+    const int q = threadIdx.x + i * blockDim.x;
+    if (q < cp * cp) E[q] = acc[i];
+  }
   __syncthreads();
 }
 
-// Write Gamma = Y (rows (others.., s), column = new bond) * prod safe_inv(lambda_other)
-// into the padded node tensor with the new bond at `slot`.
-template <typename R>
-__device__ void su_store(const DevGraph& g, cx<R>* t, int node, int slot, const double* lam, double eps, const cd* Y,
-                         long long m, int rank) {
-  const int d = g.deg[node], chi = g.chi, p = g.phys;
-  const int base = g.adj_ptr[node];
-  const long long S = ipow(chi, d) * p;
-  for (long long f = threadIdx.x; f < S; f += blockDim.x) {
-    const int s = (int)(f % p);
-    long long rr = f / p;
-    long long row = 0, mul = p;
-    int nb = 0;
-    double wt = 1.0;
-    for (int j = d - 1; j >= 0; --j) {
-      const int bj = (int)(rr % chi);
-      rr /= chi;
-      if (j == slot) {
-        nb = bj;
-      } else {
-        row += bj * mul;
-        mul *= chi;
-        wt *= su_safe_inverse(lam[(long long)g.adj_edge[base + j] * chi + bj], eps);
+// G = E (x) K combined with the gate half h (r, p, p): G[(b,q),(b',q')] =
+// sum_{s,s'} E[(b,s),(b',s')] sum_o conj(h[q,o,s]) h[q',o,s'];  G /= trace(G).
+// Returns trace(G) before normalisation (= |A|^2).
+__device__ double su_gram(const cd* E, const cd* h, int chi, int p, int r, cd* G, double* red) {
+  const int n = chi * r, cp = chi * p;
+  double tr = 0.0;
+  for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
+    const int x = f / n, y = f % n;
+    const int b = x / r, q = x % r, b2 = y / r, q2 = y % r;
+    cd acc = mk<double>(0, 0);
+    for (int s = 0; s < p; ++s)
+      for (int s2 = 0; s2 < p; ++s2) {
+        cd k = mk<double>(0, 0);
+        for (int o = 0; o < p; ++o) fma_cacc(k, h[(q * p + o) * p + s], h[(q2 * p + o) * p + s2]);
+        fma_acc(acc, E[(b * p + s) * cp + b2 * p + s2], k);
       }
+    G[f] = acc;
+    if (x == y) tr += acc.re;
+  }
+  tr = block_sum(tr, red);
+  const double inv = tr > 0.0 ? 1.0 / tr : 0.0;
+  for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
+    const int x = f / n, y = f % n;
+    if (x <= y) {
+      const cd a = G[f], b = G[y * n + x];
+      const cd hm = mk<double>(0.5 * (a.re + b.re) * inv, 0.5 * (a.im - b.im) * inv);
+      G[f] = hm;
+      G[y * n + x] = conj(hm);
     }
-    row += s;
-    cd v = mk<double>(0, 0);
-    if (nb < rank) v = wt * Y[(long long)nb * m + row];
-    t[f] = cvt<R>(v);
+  }
+  __syncthreads();
+  return tr;
+}
+
+// R = diag(sqrt(mu)) V^H from the eigendecomposition left in (G, V).
+__device__ void su_rfactor(const cd* G, const cd* V, int n, cd* R) {
+  for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
+    const int i = f / n, c = f % n;
+    const double mu = G[i * n + i].re;
+    const double sv = mu > 0.0 ? sqrt(mu) : 0.0;
+    R[f] = sv * conj(V[c * n + i]);
+  }
+  __syncthreads();
+}
+
+// Gamma of one side: rows o of the other bonds, new bond j at `slot`.
+// Z[(b,s),(j,s')] (cp x cp, complex double) holds the contraction with the
+// gate half and W; scale = 1 / sqrt(trace G).
+template <typename R>
+__device__ void su_write_gamma(const DevGraph& g, cx<R>* t, int node, int slot, const double* lam, double eps,
+                               const cd* Z, double scale, cd* chunk, int chunk_rows) {
+  const int d = g.deg[node], chi = g.chi, p = g.phys, base = g.adj_ptr[node];
+  const int cp = chi * p;
+  const long long rows = ipow(chi, d - 1);
+  for (long long r0 = 0; r0 < rows; r0 += chunk_rows) {
+    const int nr = (int)((rows - r0) < chunk_rows ? (rows - r0) : chunk_rows);
+    su_stage_rows<R>(g, t, node, slot, lam, false, r0, nr, chunk);
+    __syncthreads();
+    for (int f = threadIdx.x; f < nr * cp; f += blockDim.x) {
+      const int rr = f / cp, col = f % cp;
+      const int jn = col / p, so = col % p;
+      cd acc = mk<double>(0, 0);
+      for (int x = 0; x < cp; ++x) fma_acc(acc, chunk[rr * cp + x], Z[x * cp + col]);
+      long long o = r0 + rr;
+      long long tidx = so, mul = p;
+      double wt = scale;
+      for (int j = d - 1; j >= 0; --j) {
+        int bj;
+        if (j == slot) {
+          bj = jn;
+        } else {
+          bj = (int)(o % chi);
+          o /= chi;
+          wt *= lam_mask(lam[(long long)g.adj_edge[base + j] * chi + bj], eps);
+        }
+        tidx += bj * mul;
+        mul *= chi;
+      }
+      t[tidx] = cvt<R>(wt * acc);
+    }
+    __syncthreads();
   }
 }
 
-template <typename R>
+// Workspace (complex doubles) of one gate: E (cp^2, later U), G (n^2, later
+// the core and W), V, Ra, Rb (n^2 each), Z (cp^2), chunk (rows x cp).
+__host__ __device__ inline long long su_ws_cd(int chi, int phys, int max_r, int chunk_rows) {
+  const long long n = (long long)chi * max_r, cp = (long long)chi * phys;
+  const long long en = cp * cp > n * n ? cp * cp : n * n;  // E doubles as U (n x n)
+  return en + 4 * n * n + cp * cp + chunk_rows * cp + 64;
+}
+
+constexpr int kSuChunk = 32;
+constexpr int kSuLdsN = 32;  // LDS variant: n = chi*r <= 32 and chi*phys <= 32
+
+template <typename R, bool kLds>
 __global__ void __launch_bounds__(256)
     k_su(DevGraph g, cx<R>* T, double* lam, const SuGate* gates, const cd* halves, double eps, int max_rank,
          double* err_out, double* ent_out, char* ws, long long wsb) {
+  __shared__ cd sm[kLds ? (6 * kSuLdsN * kSuLdsN + kSuChunk * kSuLdsN + 64) : 1];
   __shared__ double red[8];
+  __shared__ double rot[4 * 64 + 8];
+  __shared__ double sigma[128];
+  __shared__ int flag[2];
+  __shared__ int perm[128];
   __shared__ int sh_rank;
   const SuGate G = gates[blockIdx.x];
   const int e = G.edge, r = G.rank, chi = g.chi, p = g.phys;
+  const int n = chi * r, cp = chi * p;
   const int na = g.edge_node[2 * e + G.first], nb = g.edge_node[2 * e + 1 - G.first];
   const int ia = find_slot(g, na, e, G.first), ib = find_slot(g, nb, e, 1 - G.first);
-  const int da = g.deg[na], db = g.deg[nb];
-  const long long ma = ipow(chi, da - 1) * p, mb = ipow(chi, db - 1) * p;
-  const int n = chi * r;
-  const int ka = (int)(ma < n ? ma : n), kb = (int)(mb < n ? mb : n);
-  const int kk = ka < kb ? ka : kb;
-  const long long hstride = (long long)p * p * p * p;  // r <= p^2 rows of (p x p)
+  const long long hstride = (long long)p * p * p * p;
   const cd* h1 = halves + (long long)blockIdx.x * 2 * hstride;
   const cd* h2 = h1 + hstride;
-
-  // carve the workspace
-  char* w = ws + (long long)blockIdx.x * wsb;
-  cd* Aa = (cd*)w;
-  cd* Ab = Aa + ma * n;
-  cd* ba = Ab + mb * n;
-  cd* bb = ba + n;
-  const int kmax = ka > kb ? ka : kb;
-  cd* Gm = bb + n;
-  cd* U = Gm + (long long)kmax * kmax;
-  cd* V = U + (long long)kmax * kmax;
-  cd* Ya = V + (long long)kmax * kmax;
-  cd* Yb = Ya + ma * chi;
-  double* sigma = (double*)(Yb + mb * chi);
-  double* rot = sigma + kmax + 8;
-  int* flag = (int*)(rot + 2 * kmax + 16);
-  int* perm = flag + 2;
-
+  cd* E = kLds ? sm : (cd*)(ws + (long long)blockIdx.x * wsb);
+  cd* Gm = E + (cp * cp > n * n ? cp * cp : n * n);
+  cd* V = Gm + n * n;
+  cd* Ra = V + n * n;
+  cd* Rb = Ra + n * n;
+  cd* Z = Rb + n * n;
+  cd* chunk = Z + cp * cp;
+  cd* C = Gm;  // core (after both R factors)
+  cd* U = E;   // left singular vectors (after both environments)
+  cd* Vs = V;  // right singular vectors
   cx<R>* ta = T + g.toff[na];
   cx<R>* tb = T + g.toff[nb];
-  su_build<R>(g, ta, na, ia, lam, h1, r, Aa, ma, n, red);
-  su_build<R>(g, tb, nb, ib, lam, h2, r, Ab, mb, n, red);
-  householder_qr(Aa, (int)ma, (int)ma, n, ba, red);
-  householder_qr(Ab, (int)mb, (int)mb, n, bb, red);
-  // core = R_a diag(lambda_rep) R_b^T, stored column-major as the SVD input
+
+  su_environment<R>(g, ta, na, ia, lam, E, chunk, kSuChunk);
+  const double tra = su_gram(E, h1, chi, p, r, Gm, red);
+  jacobi_eigh(Gm, V, n, rot, flag);
+  su_rfactor(Gm, V, n, Ra);
+  su_environment<R>(g, tb, nb, ib, lam, E, chunk, kSuChunk);
+  const double trb = su_gram(E, h2, chi, p, r, Gm, red);
+  jacobi_eigh(Gm, V, n, rot, flag);
+  su_rfactor(Gm, V, n, Rb);
+  // core = R_a diag(lambda_rep) R_b^T, column-major
   const double* le = lam + (long long)e * chi;
-  const bool tall = ka >= kb;
-  for (int f = threadIdx.x; f < ka * kb; f += blockDim.x) {
-    const int i = f / kb, j = f % kb;
+  for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
+    const int j = f / n, i = f % n;
     cd acc = mk<double>(0, 0);
-    for (int c = (i > j ? i : j); c < n; ++c) {
-      const cd ra = c == i ? ba[i] : Aa[(long long)c * ma + i];
-      const cd rb = c == j ? bb[j] : Ab[(long long)c * mb + j];
-      fma_acc(acc, le[c / r] * ra, rb);
-    }
-    if (tall)
-      Gm[(long long)j * ka + i] = acc;
-    else
-      Gm[(long long)i * kb + j] = acc;
+    for (int c = 0; c < n; ++c) fma_acc(acc, le[c / r] * Ra[i * n + c], Rb[j * n + c]);
+    C[f] = acc;
   }
   __syncthreads();
-  if (tall)
-    jacobi_svd(Gm, ka, ka, kb, U, ka, V, sigma, rot, flag, perm);
-  else
-    jacobi_svd(Gm, kb, kb, ka, U, kb, V, sigma, rot, flag, perm);
-  // threshold, rank, entropy, error
+  jacobi_svd(C, n, n, n, U, n, Vs, sigma, rot, flag, perm);
   if (threadIdx.x == 0) {
     double nrm = 0.0;
-    for (int j = 0; j < kk; ++j) {
+    for (int j = 0; j < n; ++j) {
       if (!(sigma[j] > eps)) sigma[j] = 0.0;
       nrm += sigma[j] * sigma[j];
     }
     nrm = sqrt(nrm);
-    int rank = kk < max_rank ? kk : max_rank;
+    int rank = n < max_rank ? n : max_rank;
     if (rank > chi) rank = chi;
     double ent = 0.0, err = 0.0;
-    for (int j = 0; j < kk; ++j) {
+    for (int j = 0; j < n; ++j) {
       const double x = nrm > 0.0 ? (sigma[j] / nrm) * (sigma[j] / nrm) : 0.0;
       if (x > 0.0) ent -= x * log(x);
       if (j >= rank) err += x;
@@ -226,27 +298,39 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   const int rank = sh_rank;
-  // Y_a = [U_core[:, :rank]; 0], Y_b = [conj(V_core)[:, :rank]; 0]
-  for (long long f = threadIdx.x; f < ma * chi; f += blockDim.x) {
-    const int j = (int)(f / ma);
-    const long long i = f % ma;
-    cd v = mk<double>(0, 0);
-    if (i < ka && j < rank) v = tall ? U[(long long)j * ka + i] : conj(V[(long long)perm[j] * ka + i]);
-    Ya[f] = v;
+  // W_a = Lambda R_b^T V Sigma^-1 -> Z_a;  W_b = Lambda R_a^T conj(U) Sigma^-1 -> Z_b
+  for (int side = 0; side < 2; ++side) {
+    cd* W = Gm;  // n x chi (the core is no longer needed)
+    for (int f = threadIdx.x; f < n * chi; f += blockDim.x) {
+      const int x = f / chi, j = f % chi;
+      cd acc = mk<double>(0, 0);
+      if (j < rank && sigma[j] > 0.0) {
+        for (int i = 0; i < n; ++i) {
+          if (side == 0)
+            fma_acc(acc, Rb[i * n + x], Vs[(long long)perm[j] * n + i]);
+          else
+            fma_acc(acc, Ra[i * n + x], conj(U[(long long)j * n + i]));
+        }
+        acc = (le[x / r] / sigma[j]) * acc;
+      }
+      W[f] = acc;
+    }
+    __syncthreads();
+    const cd* h = side == 0 ? h1 : h2;
+    for (int f = threadIdx.x; f < cp * cp; f += blockDim.x) {
+      const int x = f / cp, y = f % cp;
+      const int b = x / p, si = x % p, jn = y / p, so = y % p;
+      cd acc = mk<double>(0, 0);
+      if (jn < chi)
+        for (int q = 0; q < r; ++q) fma_acc(acc, h[(q * p + so) * p + si], W[(b * r + q) * chi + jn]);
+      Z[f] = acc;
+    }
+    __syncthreads();
+    if (side == 0)
+      su_write_gamma<R>(g, ta, na, ia, lam, eps, Z, 1.0 / sqrt(tra), chunk, kSuChunk);
+    else
+      su_write_gamma<R>(g, tb, nb, ib, lam, eps, Z, 1.0 / sqrt(trb), chunk, kSuChunk);
   }
-  for (long long f = threadIdx.x; f < mb * chi; f += blockDim.x) {
-    const int j = (int)(f / mb);
-    const long long i = f % mb;
-    cd v = mk<double>(0, 0);
-    if (i < kb && j < rank) v = tall ? conj(V[(long long)perm[j] * kb + i]) : U[(long long)j * kb + i];
-    Yb[f] = v;
-  }
-  __syncthreads();
-  householder_apply_q(Aa, (int)ma, (int)ma, ka, Ya, (int)ma, rank);
-  householder_apply_q(Ab, (int)mb, (int)mb, kb, Yb, (int)mb, rank);
-  su_store<R>(g, ta, na, ia, lam, eps, Ya, ma, rank);
-  su_store<R>(g, tb, nb, ib, lam, eps, Yb, mb, rank);
-  __syncthreads();
   for (int j = threadIdx.x; j < chi; j += blockDim.x) lam[(long long)e * chi + j] = j < rank ? sigma[j] : 0.0;
 }
 
@@ -262,16 +346,32 @@ hipError_t launch_apply1(int rb, const DevGraph& g, void* T, const int* nodes, i
 }
 
 hipError_t launch_su(int rb, const DevGraph& g, void* T, double* lam, const SuGate* gates, int count,
-                     const cd* halves, double eps, int max_rank, double* err, double* entropy, void* ws, long long wsb,
-                     hipStream_t s) {
+                     const cd* halves, int max_gate_rank, double eps, int max_rank, double* err, double* entropy,
+                     void* ws, long long wsb, hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  if (rb == 4)
-    hipLaunchKernelGGL(k_su<float>, dim3(count), dim3(256), 0, s, g, (cx<float>*)T, lam, gates, halves, eps,
-                       max_rank, err, entropy, (char*)ws, wsb);
-  else
-    hipLaunchKernelGGL(k_su<double>, dim3(count), dim3(256), 0, s, g, (cx<double>*)T, lam, gates, halves, eps,
-                       max_rank, err, entropy, (char*)ws, wsb);
+  const bool lds = su_lds_fits(g.chi, g.phys, max_gate_rank);
+  if (rb == 4) {
+    if (lds)
+      hipLaunchKernelGGL((k_su<float, true>), dim3(count), dim3(256), 0, s, g, (cx<float>*)T, lam, gates, halves, eps,
+                         max_rank, err, entropy, (char*)ws, wsb);
+    else
+      hipLaunchKernelGGL((k_su<float, false>), dim3(count), dim3(256), 0, s, g, (cx<float>*)T, lam, gates, halves,
+                         eps, max_rank, err, entropy, (char*)ws, wsb);
+  } else {
+    if (lds)
+      hipLaunchKernelGGL((k_su<double, true>), dim3(count), dim3(256), 0, s, g, (cx<double>*)T, lam, gates, halves,
+                         eps, max_rank, err, entropy, (char*)ws, wsb);
+    else
+      hipLaunchKernelGGL((k_su<double, false>), dim3(count), dim3(256), 0, s, g, (cx<double>*)T, lam, gates, halves,
+                         eps, max_rank, err, entropy, (char*)ws, wsb);
+  }
   return hipGetLastError();
+}
+
+bool su_lds_fits(int chi, int phys, int max_r) { return chi * max_r <= kSuLdsN && chi * phys <= kSuLdsN; }
+
+long long su_ws_bytes(int chi, int phys, int max_r) {
+  return (su_ws_cd(chi, phys, max_r, kSuChunk) * (long long)sizeof(cd) + 255) / 256 * 256;
 }
 
 }  // namespace dq

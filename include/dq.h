@@ -116,13 +116,15 @@ int dq_apply1(dq_graph* g, int dtype, void* tensors, const int* nodes, int count
 /* Simple update of `count` two-site gates on pairwise node-disjoint edges
  * (device int gates[count][4] = {edge, side of the controlling node, gate rank
  * r, 0}; device complex128 halves[count][2][phys^4], each half (r, out, in)
- * from the operator-Schmidt split).  Updates tensors (Vidal Gammas) and
+ * from the operator-Schmidt split; max_gate_rank = an upper bound of the r,
+ * which selects the kernel variant).  Updates tensors (Vidal Gammas) and
  * lambdas in place; per-gate truncation error and entropy (device double).
  * Replaces TensorGraph.apply2_to_vidal_canonical (tensor_graph.py:294-375) =
  * simple_update (tensor_ops.py:318-389) with accuracy None and
  * max_rank = max_rank (capped at chi). */
 int dq_simple_update(dq_graph* g, int dtype, void* tensors, double* lam, const int* gates, int count,
-                     const void* halves, double eps, int max_rank, double* err, double* entropy, void* stream);
+                     const void* halves, int max_gate_rank, double eps, int max_rank, double* err, double* entropy,
+                     void* stream);
 
 /* One-site density matrices (device complex128 rho[count][phys][phys]) from
  * tensors and edge->node messages.  Replaces get_one_side_density_matrix
