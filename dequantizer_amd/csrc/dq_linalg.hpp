@@ -35,9 +35,9 @@ __device__ __forceinline__ void rr_pair(int r, int k, int n2, int& p, int& q) {
 // J = [[c, s], [-s e, c e]] with e = exp(-i arg apq) makes J^H B J diagonal.
 // Returns false when the block is already diagonal to working precision.
 __device__ __forceinline__ bool jacobi_params(double app, double aqq, cd apq, double& c, double& s, double& er,
-                                              double& ei) {
+                                              double& ei, double abs_tol) {
   const double mag = sqrt(norm2(apq));
-  if (!(mag > 1e-15 * sqrt(fabs(app * aqq))) || mag < 1e-300) return false;
+  if (!(mag > 1e-15 * sqrt(fabs(app * aqq))) || !(mag > abs_tol) || mag < 1e-300) return false;
   er = apq.re / mag;
   ei = -apq.im / mag;
   const double tau = (aqq - app) / (2.0 * mag);
@@ -48,12 +48,19 @@ __device__ __forceinline__ bool jacobi_params(double app, double aqq, cd apq, do
 }
 
 // A (n x n row-major Hermitian) -> eigenvalues on the diagonal; V columns =
-// eigenvectors.  rot: 4*ceil(n/2) doubles; flag: one int.
+// eigenvectors.  rot: 4*ceil(n/2) + 1 doubles; flag: one int.
 __device__ inline void jacobi_eigh(cd* A, cd* V, int n, double* rot, int* flag) {
   const int n2 = n + (n & 1), np = n2 / 2;
   if (V)
     for (int i = threadIdx.x; i < n * n; i += blockDim.x) V[i] = mk<double>((i / n) == (i % n) ? 1.0 : 0.0, 0.0);
+  // entries below ~eps_64 |A|_F are at the rounding level of A: no rotation
+  if (threadIdx.x == 0) {
+    double f = 0.0;
+    for (int i = 0; i < n * n; ++i) f += norm2(A[i]);
+    rot[4 * np] = 1e-16 * sqrt(f);
+  }
   __syncthreads();
+  const double abs_tol = rot[4 * np];
   for (int sweep = 0; sweep < 60; ++sweep) {
     if (threadIdx.x == 0) *flag = 0;
     __syncthreads();
@@ -62,7 +69,8 @@ __device__ inline void jacobi_eigh(cd* A, cd* V, int n, double* rot, int* flag) 
         int p, q;
         rr_pair(r, k, n2, p, q);
         double c = 1.0, s = 0.0, er = 1.0, ei = 0.0;
-        if (q < n && jacobi_params(A[p * n + p].re, A[q * n + q].re, A[p * n + q], c, s, er, ei)) *flag = 1;
+        if (q < n && jacobi_params(A[p * n + p].re, A[q * n + q].re, A[p * n + q], c, s, er, ei, abs_tol))
+          *flag = 1;
         rot[4 * k + 0] = c;
         rot[4 * k + 1] = s;
         rot[4 * k + 2] = er;
@@ -135,6 +143,14 @@ __device__ inline void jacobi_svd(cd* G, int ldg, int m, int n, cd* U, int ldu, 
   int tpp = 1;
   while (tpp * 2 <= 64 && tpp * 2 * np <= (int)blockDim.x) tpp *= 2;
   const int grp = threadIdx.x / tpp, lane = threadIdx.x % tpp;
+  if (threadIdx.x == 0) {
+    double f = 0.0;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < m; ++i) f += norm2(G[(long long)j * ldg + i]);
+    rot[0] = 1e-32 * f;  // |g_p^H g_q| below ~eps_64^2 |G|_F^2 is rounding noise
+  }
+  __syncthreads();
+  const double abs_tol = rot[0];
   __syncthreads();
   for (int sweep = 0; sweep < 60; ++sweep) {
     if (threadIdx.x == 0) *flag = 0;
@@ -165,7 +181,7 @@ __device__ inline void jacobi_svd(cd* G, int ldg, int m, int n, cd* U, int ldu, 
           gi += __shfl_xor(gi, o, 64);
         }
         double c, s, er, ei;
-        if (act && jacobi_params(al, be, mk<double>(gr, gi), c, s, er, ei)) {
+        if (act && jacobi_params(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
           if (lane == 0) *flag = 1;
           const cd e = mk<double>(er, ei);
           cd* gp = G + (long long)p * ldg;
@@ -224,6 +240,79 @@ __device__ inline void permute_cols(const cd* V, cd* out, int n, const int* perm
   for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
     const int j = t / n, i = t % n;
     out[(long long)j * n + i] = V[(long long)perm[j] * n + i];
+  }
+  __syncthreads();
+}
+
+// Factor R with R^H R = G for a Hermitian positive semi-definite G (n x n,
+// row-major, destroyed) by Cholesky with diagonal pivoting (Higham,
+// "Accuracy and Stability of Numerical Algorithms", 10.3): P^T G P = L L^H,
+// R = L^H P^T.  Pivots below n * eps_64 * max diag end the factorisation
+// (the remaining rows of R are zero).  R: n x n row-major.  piv: n ints;
+// scal: 2 doubles of scratch.
+__device__ inline void pivoted_cholesky(cd* G, int n, cd* R, int* piv, double* scal) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) piv[i] = i;
+  for (int i = threadIdx.x; i < n * n; i += blockDim.x) R[i] = mk<double>(0.0, 0.0);
+  if (threadIdx.x == 0) {
+    double mx = 0.0;
+    for (int i = 0; i < n; ++i) mx = fmax(mx, G[i * n + i].re);
+    scal[0] = n * 2.3e-16 * mx;
+  }
+  __syncthreads();
+  const double tol = scal[0];
+  int k = 0;
+  for (; k < n; ++k) {
+    if (threadIdx.x == 0) {
+      int j = k;
+      double best = G[k * n + k].re;
+      for (int i = k + 1; i < n; ++i)
+        if (G[i * n + i].re > best) {
+          best = G[i * n + i].re;
+          j = i;
+        }
+      scal[1] = best;
+      *(int*)(scal + 2) = j;
+    }
+    __syncthreads();
+    const double best = scal[1];
+    const int j = *(int*)(scal + 2);
+    if (!(best > tol)) break;
+    if (j != k) {
+      // symmetric swap of rows/columns k and j (columns first, then rows)
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const cd t = G[i * n + k];
+        G[i * n + k] = G[i * n + j];
+        G[i * n + j] = t;
+      }
+      __syncthreads();
+      for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        const cd t = G[k * n + c];
+        G[k * n + c] = G[j * n + c];
+        G[j * n + c] = t;
+      }
+      if (threadIdx.x == 0) {
+        const int t = piv[k];
+        piv[k] = piv[j];
+        piv[j] = t;
+      }
+      __syncthreads();
+    }
+    const double lkk = sqrt(best);
+    // column k of L below the diagonal, kept in G[i][k]
+    for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) G[i * n + k] = (1.0 / lkk) * G[i * n + k];
+    if (threadIdx.x == 0) G[k * n + k] = mk<double>(lkk, 0.0);
+    __syncthreads();
+    const int rem = n - k - 1;
+    for (int f = threadIdx.x; f < rem * rem; f += blockDim.x) {
+      const int i = k + 1 + f / rem, c = k + 1 + f % rem;
+      G[i * n + c] = G[i * n + c] - G[i * n + k] * conj(G[c * n + k]);
+    }
+    __syncthreads();
+  }
+  // R[i][piv[m]] = conj(L[m][i]) for m >= i, rows i < k
+  for (int f = threadIdx.x; f < k * n; f += blockDim.x) {
+    const int i = f / n, m = f % n;
+    if (m >= i) R[i * n + piv[m]] = conj(G[m * n + i]);
   }
   __syncthreads();
 }

@@ -56,7 +56,7 @@ __device__ inline int find_slot(const DevGraph& g, int node, int e, int side) {
 // A = (T * lambda_others) x gate-half (rows: other bonds + phys, columns:
 // bond * r + q) by QR and takes the SVD of core = R_a diag(lambda) R_b^T
 // (tensor_ops.py:265-389).  Only R up to a left unitary matters, so R is taken
-// from the Gram matrix G = A^H A = V S^2 V^H as R = S V^H; G is assembled from
+// from the Gram matrix G = A^H A by pivoted Cholesky (R^H R = G); G is assembled from
 // the environment E = X^H X of X = T * lambda_others (rows: other bonds,
 // columns: (bond, phys)) and the gate half.  With core = U Sigma V^H,
 //   Gamma_a = A_a Lambda R_b^T V Sigma^-1,   Gamma_b = A_b Lambda R_a^T conj(U) Sigma^-1
@@ -261,12 +261,10 @@ __global__ void __launch_bounds__(256)
 
   su_environment<R>(g, ta, na, ia, lam, E, chunk, kSuChunk);
   const double tra = su_gram(E, h1, chi, p, r, Gm, red);
-  jacobi_eigh(Gm, V, n, rot, flag);
-  su_rfactor(Gm, V, n, Ra);
+  pivoted_cholesky(Gm, n, Ra, perm, rot);
   su_environment<R>(g, tb, nb, ib, lam, E, chunk, kSuChunk);
   const double trb = su_gram(E, h2, chi, p, r, Gm, red);
-  jacobi_eigh(Gm, V, n, rot, flag);
-  su_rfactor(Gm, V, n, Rb);
+  pivoted_cholesky(Gm, n, Rb, perm, rot);
   // core = R_a diag(lambda_rep) R_b^T, column-major
   const double* le = lam + (long long)e * chi;
   for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
