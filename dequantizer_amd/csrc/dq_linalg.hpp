@@ -235,6 +235,110 @@ __device__ inline void jacobi_svd(cd* G, int ldg, int m, int n, cd* U, int ldu, 
   __syncthreads();
 }
 
+// Wave-synchronous one-sided Jacobi SVD: ONE wavefront (64 lanes, all active)
+// factors G (m x n column-major, leading dimension m, m >= n, n <= 32,
+// overwritten) with the round-robin pair order; lanes of a pair split the rows.
+// No block barrier: the columns live in LDS and every round is ordered by
+// program order within the wave.  Outputs as jacobi_svd: sigma (descending),
+// U (m x n column-major, zero columns for sigma == 0), unsorted V (n x n
+// column-major) with perm.  Needs 1 double of scratch in `scal`.
+__device__ inline void wave_jacobi_svd(cd* G, int m, int n, cd* U, cd* V, double* sigma, int* perm, double* scal) {
+  const int lane = threadIdx.x & 63;
+  const int n2 = n + (n & 1), np = n2 / 2;
+  int lpp = 1;
+  while (lpp * 2 * np <= 64 && lpp * 2 <= 16) lpp *= 2;
+  const int k = lane / lpp, sub = lane % lpp;
+  for (int i = lane; i < n * n; i += 64) V[i] = mk<double>((i / n) == (i % n) ? 1.0 : 0.0, 0.0);
+  double f = 0.0;
+  for (int i = lane; i < m * n; i += 64) f += norm2(G[i]);
+  f = wave_sum(f);
+  const double abs_tol = 1e-32 * f;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    int rotated = 0;
+    for (int r = 0; r < n2 - 1; ++r) {
+      int p = 0, q = 0;
+      const bool live = k < np;
+      if (live) rr_pair(r, k, n2, p, q);
+      const bool act = live && q < n;
+      double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+      if (act) {
+        const cd* gp = G + p * m;
+        const cd* gq = G + q * m;
+        for (int i = sub; i < m; i += lpp) {
+          const cd x = gp[i], y = gq[i];
+          al += norm2(x);
+          be += norm2(y);
+          gr += x.re * y.re + x.im * y.im;
+          gi += x.re * y.im - x.im * y.re;
+        }
+      }
+      for (int o = lpp / 2; o > 0; o >>= 1) {
+        al += __shfl_xor(al, o, 64);
+        be += __shfl_xor(be, o, 64);
+        gr += __shfl_xor(gr, o, 64);
+        gi += __shfl_xor(gi, o, 64);
+      }
+      double c, s, er, ei;
+      if (act && jacobi_params(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
+        rotated = 1;
+        const cd e = mk<double>(er, ei);
+        cd* gp = G + p * m;
+        cd* gq = G + q * m;
+        for (int i = sub; i < m; i += lpp) {
+          const cd x = gp[i], ey = e * gq[i];
+          gp[i] = mk<double>(c * x.re - s * ey.re, c * x.im - s * ey.im);
+          gq[i] = mk<double>(s * x.re + c * ey.re, s * x.im + c * ey.im);
+        }
+        cd* vp = V + p * n;
+        cd* vq = V + q * n;
+        for (int i = sub; i < n; i += lpp) {
+          const cd x = vp[i], ey = e * vq[i];
+          vp[i] = mk<double>(c * x.re - s * ey.re, c * x.im - s * ey.im);
+          vq[i] = mk<double>(s * x.re + c * ey.re, s * x.im + c * ey.im);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (!__any(rotated)) break;
+  }
+  // column norms, rank sort (descending, ties by index), normalised left vectors
+  double nrm = 0.0;
+  if (lane < n) {
+    for (int i = 0; i < m; ++i) nrm += norm2(G[lane * m + i]);
+    nrm = sqrt(nrm);
+    sigma[lane] = nrm;  // temporary: unsorted
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (lane < n) {
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+      const double o = sigma[j];
+      rank += (o > nrm || (o == nrm && j < lane)) ? 1 : 0;
+    }
+    perm[rank] = lane;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (lane < n) scal[lane] = sigma[perm[lane]];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (lane < n) sigma[lane] = scal[lane];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (int t = lane; t < m * n; t += 64) {
+    const int j = t / m, i = t % m;
+    const double sg = sigma[j];
+    const cd g = G[perm[j] * m + i];
+    U[j * m + i] = sg > 0.0 ? mk<double>(g.re / sg, g.im / sg) : mk<double>(0.0, 0.0);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // V columns permuted by perm: out[:, j] = V[:, perm[j]] (separate buffer).
 __device__ inline void permute_cols(const cd* V, cd* out, int n, const int* perm) {
   for (int t = threadIdx.x; t < n * n; t += blockDim.x) {

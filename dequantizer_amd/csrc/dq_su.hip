@@ -76,23 +76,24 @@ __device__ __forceinline__ double lam_mask(double l, double eps) {
 // bonds in order, last one fastest.
 template <typename R>
 __device__ void su_stage_rows(const DevGraph& g, const cx<R>* t, int node, int slot, const double* lam, bool weighted,
-                              long long r0, int nr, cd* buf) {
+                              int r0, int nr, cd* buf) {
   const int d = g.deg[node], chi = g.chi, p = g.phys, base = g.adj_ptr[node];
   const int cp = chi * p;
   for (int f = threadIdx.x; f < nr * cp; f += blockDim.x) {
-    const int rr = f / cp, col = f % cp;
-    const int b = col / p, s = col % p;
-    long long o = r0 + rr;
-    long long tidx = s, mul = p;
+    const int rr = f / cp, col = f - rr * cp;
+    const int b = col / p, s = col - b * p;
+    int o = r0 + rr;
+    int tidx = s, mul = p;
     double wt = 1.0;
     for (int j = d - 1; j >= 0; --j) {
       int bj;
       if (j == slot) {
         bj = b;
       } else {
-        bj = (int)(o % chi);
-        o /= chi;
-        if (weighted) wt *= lam[(long long)g.adj_edge[base + j] * chi + bj];
+        const int q = o / chi;
+        bj = o - q * chi;
+        o = q;
+        if (weighted) wt *= lam[g.adj_edge[base + j] * chi + bj];
       }
       tidx += bj * mul;
       mul *= chi;
@@ -101,32 +102,36 @@ __device__ void su_stage_rows(const DevGraph& g, const cx<R>* t, int node, int s
   }
 }
 
-// E[(b,s),(b',s')] = sum_o conj(X[o][(b,s)]) X[o][(b',s')]  (cp x cp).
+// E[(b,s),(b',s')] = sum_o conj(X[o][(b,s)]) X[o][(b',s')]  (cp x cp, cp <= 64).
 template <typename R>
 __device__ void su_environment(const DevGraph& g, const cx<R>* t, int node, int slot, const double* lam, cd* E,
                                cd* chunk, int chunk_rows) {
   const int d = g.deg[node], chi = g.chi, p = g.phys;
   const int cp = chi * p;
-  const long long rows = ipow(chi, d - 1);
-  const int per = (cp * cp + blockDim.x - 1) / blockDim.x;
-  cd acc[16];
-  for (int i = 0; i < per && i < 16; ++i) acc[i] = mk<double>(0, 0);
-  for (long long r0 = 0; r0 < rows; r0 += chunk_rows) {
-    const int nr = (int)((rows - r0) < chunk_rows ? (rows - r0) : chunk_rows);
+  const int rows = (int)ipow(chi, d - 1);
+  constexpr int kMaxPer = 16;  // (cp^2) / 256 for cp <= 64
+  cd acc[kMaxPer];
+#pragma unroll
+  for (int i = 0; i < kMaxPer; ++i) acc[i] = mk<double>(0, 0);
+  for (int r0 = 0; r0 < rows; r0 += chunk_rows) {
+    const int nr = (rows - r0) < chunk_rows ? (rows - r0) : chunk_rows;
     su_stage_rows<R>(g, t, node, slot, lam, true, r0, nr, chunk);
     __syncthreads();
-    for (int i = 0; i < per && i < 16; ++i) {
-      const int q = threadIdx.x + i * blockDim.x;
-      if (q >= cp * cp) break;
-      const int x = q / cp, y = q % cp;
-      cd a = acc[i];
-      for (int r = 0; r < nr; ++r) fma_cacc(a, chunk[r * cp + x], chunk[r * cp + y]);
-      acc[i] = a;
+#pragma unroll
+    for (int i = 0; i < kMaxPer; ++i) {
+      const int q = threadIdx.x + i * 256;
+      if (q < cp * cp) {
+        const int x = q / cp, y = q - x * cp;
+        cd a = acc[i];
+        for (int r = 0; r < nr; ++r) fma_cacc(a, chunk[r * cp + x], chunk[r * cp + y]);
+        acc[i] = a;
+      }
     }
     __syncthreads();
   }
-  for (int i = 0; i < per && i < 16; ++i) {
-    const int q = threadIdx.x + i * blockDim.x;
+#pragma unroll
+  for (int i = 0; i < kMaxPer; ++i) {
+    const int q = threadIdx.x + i * 256;
     if (q < cp * cp) E[q] = acc[i];
   }
   __syncthreads();
@@ -185,27 +190,28 @@ __device__ void su_write_gamma(const DevGraph& g, cx<R>* t, int node, int slot, 
                                const cd* Z, double scale, cd* chunk, int chunk_rows) {
   const int d = g.deg[node], chi = g.chi, p = g.phys, base = g.adj_ptr[node];
   const int cp = chi * p;
-  const long long rows = ipow(chi, d - 1);
-  for (long long r0 = 0; r0 < rows; r0 += chunk_rows) {
-    const int nr = (int)((rows - r0) < chunk_rows ? (rows - r0) : chunk_rows);
+  const int rows = (int)ipow(chi, d - 1);
+  for (int r0 = 0; r0 < rows; r0 += chunk_rows) {
+    const int nr = (rows - r0) < chunk_rows ? (rows - r0) : chunk_rows;
     su_stage_rows<R>(g, t, node, slot, lam, false, r0, nr, chunk);
     __syncthreads();
     for (int f = threadIdx.x; f < nr * cp; f += blockDim.x) {
-      const int rr = f / cp, col = f % cp;
-      const int jn = col / p, so = col % p;
+      const int rr = f / cp, col = f - rr * cp;
+      const int jn = col / p, so = col - jn * p;
       cd acc = mk<double>(0, 0);
       for (int x = 0; x < cp; ++x) fma_acc(acc, chunk[rr * cp + x], Z[x * cp + col]);
-      long long o = r0 + rr;
-      long long tidx = so, mul = p;
+      int o = r0 + rr;
+      int tidx = so, mul = p;
       double wt = scale;
       for (int j = d - 1; j >= 0; --j) {
         int bj;
         if (j == slot) {
           bj = jn;
         } else {
-          bj = (int)(o % chi);
-          o /= chi;
-          wt *= lam_mask(lam[(long long)g.adj_edge[base + j] * chi + bj], eps);
+          const int q = o / chi;
+          bj = o - q * chi;
+          o = q;
+          wt *= lam_mask(lam[g.adj_edge[base + j] * chi + bj], eps);
         }
         tidx += bj * mul;
         mul *= chi;
@@ -274,7 +280,12 @@ __global__ void __launch_bounds__(256)
     C[f] = acc;
   }
   __syncthreads();
-  jacobi_svd(C, n, n, n, U, n, Vs, sigma, rot, flag, perm);
+  if (n <= 32) {
+    if (threadIdx.x < 64) wave_jacobi_svd(C, n, n, U, Vs, sigma, perm, rot);
+    __syncthreads();
+  } else {
+    jacobi_svd(C, n, n, n, U, n, Vs, sigma, rot, flag, perm);
+  }
   if (threadIdx.x == 0) {
     double nrm = 0.0;
     for (int j = 0; j < n; ++j) {
