@@ -52,6 +52,8 @@ def main():
     ap.add_argument("--fixed-sweeps", type=int, default=10,
                     help="BP sweeps per regauge with the convergence test off (fixed work); 0 = converge to 1e-5")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="run the oracle CPU leg (1) or not (0)")
+    ap.add_argument("--gate-order", default="colored", choices=["colored", "pairs"],
+                    help="coupling-gate order: pair list sorted by edge colour, or the generator's pair order")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: every box in this setup has one GPU; "
@@ -60,11 +62,13 @@ def main():
 
     import torch
 
-    from dequantizer_amd.annealing import Annealer, get_scheduler, linear_schedule, random_regular
+    from dequantizer_amd.annealing import Annealer, edge_colored, get_scheduler, linear_schedule, random_regular
     from dequantizer_amd import PRNGKey
 
     torch.cuda.init()
     ef = random_regular(args.nodes, 3, seed=args.seed, couplings="pm1", field_std=0.1)
+    if args.gate_order == "colored":
+        ef = edge_colored(ef)
     schedule = get_scheduler(args.dt, linear_schedule, args.schedule_steps)
     if args.warmup + args.steps > len(schedule):
         raise SystemExit("warmup + steps exceeds the schedule length")
@@ -129,7 +133,7 @@ def main():
                                                args.warmup, args.warmup + args.steps - 1),
             "nodes": args.nodes, "edges": ef.edges_number, "chi": chi, "phys": p,
             "bp_schedule": ("sync" if args.sync else "async-" + args.traversal),
-            "bp_accuracy": 1e-5, "regauge": "every layer",
+            "bp_accuracy": 1e-5, "regauge": "every layer", "gate_order": args.gate_order,
             "sweeps_per_regauge": (args.fixed_sweeps or "to convergence"),
         },
         "su_gates_per_s": n_gates / elapsed,

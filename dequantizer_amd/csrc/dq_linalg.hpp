@@ -37,7 +37,9 @@ __device__ __forceinline__ void rr_pair(int r, int k, int n2, int& p, int& q) {
 __device__ __forceinline__ bool jacobi_params(double app, double aqq, cd apq, double& c, double& s, double& er,
                                               double& ei, double abs_tol) {
   const double mag = sqrt(norm2(apq));
-  if (!(mag > 1e-15 * sqrt(fabs(app * aqq))) || !(mag > abs_tol) || mag < 1e-300) return false;
+  // 1e-13 sits above the orthogonality FP64 can reach for n <= 64 (~n eps),
+  // so converged pairs stop rotating; convergence is quadratic past it
+  if (!(mag > 1e-13 * sqrt(fabs(app * aqq))) || !(mag > abs_tol) || mag < 1e-300) return false;
   er = apq.re / mag;
   ei = -apq.im / mag;
   const double tau = (aqq - app) / (2.0 * mag);
@@ -339,6 +341,193 @@ __device__ inline void wave_jacobi_svd(cd* G, int m, int n, cd* U, cd* V, double
   __builtin_amdgcn_wave_barrier();
 }
 
+// Fixed-size variant of wave_jacobi_svd for square N x N (N = 16 or 32):
+// loop bounds are compile-time constants, so each lane loads its rows of a
+// column pair in one burst, rotates them in registers and stores them back.
+template <int N>
+__device__ inline void wave_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, int* perm, double* scal) {
+  constexpr int NP = N / 2;
+  constexpr int LPP = 64 / NP;    // lanes per pair
+  constexpr int RPL = N / LPP;    // rows per lane
+  const int lane = threadIdx.x & 63;
+  const int k = lane / LPP, sub = lane % LPP;
+  for (int i = lane; i < N * N; i += 64) V[i] = mk<double>((i / N) == (i % N) ? 1.0 : 0.0, 0.0);
+  double f = 0.0;
+  for (int i = lane; i < N * N; i += 64) f += norm2(G[i]);
+  f = wave_sum(f);
+  const double abs_tol = 1e-32 * f;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    int rotated = 0;
+    for (int r = 0; r < N - 1; ++r) {
+      int p, q;
+      rr_pair(r, k, N, p, q);
+      cd x[RPL], y[RPL];
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        x[t] = G[p * N + sub + t * LPP];
+        y[t] = G[q * N + sub + t * LPP];
+      }
+      double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        al += norm2(x[t]);
+        be += norm2(y[t]);
+        gr += x[t].re * y[t].re + x[t].im * y[t].im;
+        gi += x[t].re * y[t].im - x[t].im * y[t].re;
+      }
+#pragma unroll
+      for (int o = LPP / 2; o > 0; o >>= 1) {
+        al += __shfl_xor(al, o, 64);
+        be += __shfl_xor(be, o, 64);
+        gr += __shfl_xor(gr, o, 64);
+        gi += __shfl_xor(gi, o, 64);
+      }
+      double c, s, er, ei;
+      if (jacobi_params(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
+        rotated = 1;
+        const cd e = mk<double>(er, ei);
+        cd vx[RPL], vy[RPL];
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          vx[t] = V[p * N + sub + t * LPP];
+          vy[t] = V[q * N + sub + t * LPP];
+        }
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          const cd ey = e * y[t];
+          G[p * N + sub + t * LPP] = mk<double>(c * x[t].re - s * ey.re, c * x[t].im - s * ey.im);
+          G[q * N + sub + t * LPP] = mk<double>(s * x[t].re + c * ey.re, s * x[t].im + c * ey.im);
+          const cd evy = e * vy[t];
+          V[p * N + sub + t * LPP] = mk<double>(c * vx[t].re - s * evy.re, c * vx[t].im - s * evy.im);
+          V[q * N + sub + t * LPP] = mk<double>(s * vx[t].re + c * evy.re, s * vx[t].im + c * evy.im);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (!__any(rotated)) break;
+  }
+  double nrm = 0.0;
+  if (lane < N) {
+#pragma unroll 8
+    for (int i = 0; i < N; ++i) nrm += norm2(G[lane * N + i]);
+    nrm = sqrt(nrm);
+    sigma[lane] = nrm;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (lane < N) {
+    int rank = 0;
+    for (int j = 0; j < N; ++j) {
+      const double o = sigma[j];
+      rank += (o > nrm || (o == nrm && j < lane)) ? 1 : 0;
+    }
+    perm[rank] = lane;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (lane < N) scal[lane] = sigma[perm[lane]];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (lane < N) sigma[lane] = scal[lane];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (int t = lane; t < N * N; t += 64) {
+    const int j = t / N, i = t % N;
+    const double sg = sigma[j];
+    const cd g = G[perm[j] * N + i];
+    U[j * N + i] = sg > 0.0 ? mk<double>(g.re / sg, g.im / sg) : mk<double>(0.0, 0.0);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Block-parallel fixed-size one-sided Jacobi SVD (N = 32, 256 threads): the
+// 16 pairs of a round run on 16-lane groups (4 per wave), 2 rows per lane, one
+// block barrier per round.  Same outputs as jacobi_svd.  scal: N doubles.
+template <int N>
+__device__ inline void block_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, int* perm, double* scal) {
+  constexpr int NP = N / 2;
+  constexpr int LPP = 256 / NP;  // 16 lanes per pair for N = 32
+  constexpr int RPL = N / LPP;   // rows per lane
+  static_assert(LPP <= 64 && RPL >= 1, "block SVD sizing");
+  const int k = threadIdx.x / LPP, sub = threadIdx.x % LPP;
+  for (int i = threadIdx.x; i < N * N; i += blockDim.x) V[i] = mk<double>((i / N) == (i % N) ? 1.0 : 0.0, 0.0);
+  double f = 0.0;
+  for (int i = threadIdx.x; i < N * N; i += blockDim.x) f += norm2(G[i]);
+  f = block_sum(f, scal);
+  const double abs_tol = 1e-32 * f;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    int rotated = 0;
+    for (int r = 0; r < N - 1; ++r) {
+      int p, q;
+      rr_pair(r, k, N, p, q);
+      cd x[RPL], y[RPL], vx[RPL], vy[RPL];
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        x[t] = G[p * N + sub + t * LPP];
+        y[t] = G[q * N + sub + t * LPP];
+        vx[t] = V[p * N + sub + t * LPP];
+        vy[t] = V[q * N + sub + t * LPP];
+      }
+      double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        al += norm2(x[t]);
+        be += norm2(y[t]);
+        gr += x[t].re * y[t].re + x[t].im * y[t].im;
+        gi += x[t].re * y[t].im - x[t].im * y[t].re;
+      }
+#pragma unroll
+      for (int o = LPP / 2; o > 0; o >>= 1) {
+        al += __shfl_xor(al, o, 64);
+        be += __shfl_xor(be, o, 64);
+        gr += __shfl_xor(gr, o, 64);
+        gi += __shfl_xor(gi, o, 64);
+      }
+      double c, s, er, ei;
+      if (jacobi_params(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
+        rotated = 1;
+        const cd e = mk<double>(er, ei);
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          const cd ey = e * y[t];
+          G[p * N + sub + t * LPP] = mk<double>(c * x[t].re - s * ey.re, c * x[t].im - s * ey.im);
+          G[q * N + sub + t * LPP] = mk<double>(s * x[t].re + c * ey.re, s * x[t].im + c * ey.im);
+          const cd evy = e * vy[t];
+          V[p * N + sub + t * LPP] = mk<double>(c * vx[t].re - s * evy.re, c * vx[t].im - s * evy.im);
+          V[q * N + sub + t * LPP] = mk<double>(s * vx[t].re + c * evy.re, s * vx[t].im + c * evy.im);
+        }
+      }
+      __syncthreads();
+    }
+    if (!__syncthreads_or(rotated)) break;
+  }
+  if (threadIdx.x < N) {
+    double nrm = 0.0;
+    for (int i = 0; i < N; ++i) nrm += norm2(G[threadIdx.x * N + i]);
+    scal[threadIdx.x] = sqrt(nrm);
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    const double me = scal[threadIdx.x];
+    int rank = 0;
+    for (int j = 0; j < N; ++j) rank += (scal[j] > me || (scal[j] == me && j < (int)threadIdx.x)) ? 1 : 0;
+    perm[rank] = threadIdx.x;
+    sigma[rank] = me;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < N * N; t += blockDim.x) {
+    const int j = t / N, i = t % N;
+    const double sg = sigma[j];
+    const cd g = G[perm[j] * N + i];
+    U[j * N + i] = sg > 0.0 ? mk<double>(g.re / sg, g.im / sg) : mk<double>(0.0, 0.0);
+  }
+  __syncthreads();
+}
+
 // V columns permuted by perm: out[:, j] = V[:, perm[j]] (separate buffer).
 __device__ inline void permute_cols(const cd* V, cd* out, int n, const int* perm) {
   for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
@@ -366,16 +555,29 @@ __device__ inline void pivoted_cholesky(cd* G, int n, cd* R, int* piv, double* s
   const double tol = scal[0];
   int k = 0;
   for (; k < n; ++k) {
-    if (threadIdx.x == 0) {
-      int j = k;
-      double best = G[k * n + k].re;
-      for (int i = k + 1; i < n; ++i)
-        if (G[i * n + i].re > best) {
-          best = G[i * n + i].re;
+    if (threadIdx.x < 64) {
+      // wave-0 argmax of the remaining diagonal (ties -> smallest index)
+      double best = -1.0;
+      int j = n;
+      for (int i = k + (int)threadIdx.x; i < n; i += 64) {
+        const double v = G[i * n + i].re;
+        if (v > best) {
+          best = v;
           j = i;
         }
-      scal[1] = best;
-      *(int*)(scal + 2) = j;
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o, 64);
+        const int oj = __shfl_xor(j, o, 64);
+        if (ob > best || (ob == best && oj < j)) {
+          best = ob;
+          j = oj;
+        }
+      }
+      if (threadIdx.x == 0) {
+        scal[1] = best;
+        *(int*)(scal + 2) = j;
+      }
     }
     __syncthreads();
     const double best = scal[1];

@@ -75,9 +75,9 @@ __device__ __forceinline__ double lam_mask(double l, double eps) {
 // into buf (nr x cp, row-major, complex double).  Rows enumerate the other
 // bonds in order, last one fastest.
 template <typename R>
-__device__ void su_stage_rows(const DevGraph& g, const cx<R>* t, int node, int slot, const double* lam, bool weighted,
+__device__ void su_stage_rows(const DevGraph& g, const cx<R>* t, int node, int slot, const double* wl, bool weighted,
                               int r0, int nr, cd* buf) {
-  const int d = g.deg[node], chi = g.chi, p = g.phys, base = g.adj_ptr[node];
+  const int d = g.deg[node], chi = g.chi, p = g.phys;
   const int cp = chi * p;
   for (int f = threadIdx.x; f < nr * cp; f += blockDim.x) {
     const int rr = f / cp, col = f - rr * cp;
@@ -93,7 +93,7 @@ __device__ void su_stage_rows(const DevGraph& g, const cx<R>* t, int node, int s
         const int q = o / chi;
         bj = o - q * chi;
         o = q;
-        if (weighted) wt *= lam[g.adj_edge[base + j] * chi + bj];
+        if (weighted) wt *= wl[j * chi + bj];
       }
       tidx += bj * mul;
       mul *= chi;
@@ -104,18 +104,45 @@ __device__ void su_stage_rows(const DevGraph& g, const cx<R>* t, int node, int s
 
 // E[(b,s),(b',s')] = sum_o conj(X[o][(b,s)]) X[o][(b',s')]  (cp x cp, cp <= 64).
 template <typename R>
-__device__ void su_environment(const DevGraph& g, const cx<R>* t, int node, int slot, const double* lam, cd* E,
+__device__ void su_environment(const DevGraph& g, const cx<R>* t, int node, int slot, const double* wl, cd* E,
                                cd* chunk, int chunk_rows) {
   const int d = g.deg[node], chi = g.chi, p = g.phys;
   const int cp = chi * p;
   const int rows = (int)ipow(chi, d - 1);
+  if (cp == 32 && blockDim.x == 256) {
+    // 2x2 register tiles: thread -> rows x0, x0+1 and columns y0, y0+1 of E
+    const int tx = threadIdx.x >> 4, ty = threadIdx.x & 15;
+    const int x0 = 2 * tx, y0 = 2 * ty;
+    cd a00 = mk<double>(0, 0), a01 = a00, a10 = a00, a11 = a00;
+    for (int r0 = 0; r0 < rows; r0 += chunk_rows) {
+      const int nr = (rows - r0) < chunk_rows ? (rows - r0) : chunk_rows;
+      su_stage_rows<R>(g, t, node, slot, wl, true, r0, nr, chunk);
+      __syncthreads();
+#pragma unroll 4
+      for (int r = 0; r < nr; ++r) {
+        const cd u0 = chunk[r * 32 + x0], u1 = chunk[r * 32 + x0 + 1];
+        const cd v0 = chunk[r * 32 + y0], v1 = chunk[r * 32 + y0 + 1];
+        fma_cacc(a00, u0, v0);
+        fma_cacc(a01, u0, v1);
+        fma_cacc(a10, u1, v0);
+        fma_cacc(a11, u1, v1);
+      }
+      __syncthreads();
+    }
+    E[x0 * 32 + y0] = a00;
+    E[x0 * 32 + y0 + 1] = a01;
+    E[(x0 + 1) * 32 + y0] = a10;
+    E[(x0 + 1) * 32 + y0 + 1] = a11;
+    __syncthreads();
+    return;
+  }
   constexpr int kMaxPer = 16;  // (cp^2) / 256 for cp <= 64
   cd acc[kMaxPer];
 #pragma unroll
   for (int i = 0; i < kMaxPer; ++i) acc[i] = mk<double>(0, 0);
   for (int r0 = 0; r0 < rows; r0 += chunk_rows) {
     const int nr = (rows - r0) < chunk_rows ? (rows - r0) : chunk_rows;
-    su_stage_rows<R>(g, t, node, slot, lam, true, r0, nr, chunk);
+    su_stage_rows<R>(g, t, node, slot, wl, true, r0, nr, chunk);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kMaxPer; ++i) {
@@ -186,20 +213,64 @@ __device__ void su_rfactor(const cd* G, const cd* V, int n, cd* R) {
 // Z[(b,s),(j,s')] (cp x cp, complex double) holds the contraction with the
 // gate half and W; scale = 1 / sqrt(trace G).
 template <typename R>
-__device__ void su_write_gamma(const DevGraph& g, cx<R>* t, int node, int slot, const double* lam, double eps,
+__device__ void su_write_gamma(const DevGraph& g, cx<R>* t, int node, int slot, const double* wl, double eps,
                                const cd* Z, double scale, cd* chunk, int chunk_rows) {
-  const int d = g.deg[node], chi = g.chi, p = g.phys, base = g.adj_ptr[node];
+  const int d = g.deg[node], chi = g.chi, p = g.phys;
   const int cp = chi * p;
   const int rows = (int)ipow(chi, d - 1);
   for (int r0 = 0; r0 < rows; r0 += chunk_rows) {
     const int nr = (rows - r0) < chunk_rows ? (rows - r0) : chunk_rows;
-    su_stage_rows<R>(g, t, node, slot, lam, false, r0, nr, chunk);
+    su_stage_rows<R>(g, t, node, slot, wl, false, r0, nr, chunk);
     __syncthreads();
+    if (cp == 32 && nr == 32 && blockDim.x == 256) {
+      // 2 x 2 register tiles: rows rr0, rr0+1 (of 32) x columns c0, c0+1
+      const int rr0 = 2 * (threadIdx.x >> 4), c0 = 2 * (threadIdx.x & 15);
+      cd a00 = mk<double>(0, 0), a01 = a00, a10 = a00, a11 = a00;
+#pragma unroll 8
+      for (int x = 0; x < 32; ++x) {
+        const cd u0 = chunk[rr0 * 32 + x], u1 = chunk[(rr0 + 1) * 32 + x];
+        const cd z0 = Z[x * 32 + c0], z1 = Z[x * 32 + c0 + 1];
+        fma_acc(a00, u0, z0);
+        fma_acc(a01, u0, z1);
+        fma_acc(a10, u1, z0);
+        fma_acc(a11, u1, z1);
+      }
+      const cd av[4] = {a00, a01, a10, a11};
+#pragma unroll
+      for (int q4 = 0; q4 < 4; ++q4) {
+        const int rr = rr0 + (q4 >> 1), col = c0 + (q4 & 1);
+        const int jn = col / p, so = col - jn * p;
+        int o = r0 + rr;
+        int tidx = so, mul = p;
+        double wt = scale;
+        for (int j = d - 1; j >= 0; --j) {
+          int bj;
+          if (j == slot) {
+            bj = jn;
+          } else {
+            const int qq = o / chi;
+            bj = o - qq * chi;
+            o = qq;
+            wt *= lam_mask(wl[j * chi + bj], eps);
+          }
+          tidx += bj * mul;
+          mul *= chi;
+        }
+        t[tidx] = cvt<R>(wt * av[q4]);
+      }
+      __syncthreads();
+      continue;
+    }
     for (int f = threadIdx.x; f < nr * cp; f += blockDim.x) {
       const int rr = f / cp, col = f - rr * cp;
       const int jn = col / p, so = col - jn * p;
       cd acc = mk<double>(0, 0);
-      for (int x = 0; x < cp; ++x) fma_acc(acc, chunk[rr * cp + x], Z[x * cp + col]);
+      if (cp == 32) {
+#pragma unroll 8
+        for (int x = 0; x < 32; ++x) fma_acc(acc, chunk[rr * 32 + x], Z[x * 32 + col]);
+      } else {
+        for (int x = 0; x < cp; ++x) fma_acc(acc, chunk[rr * cp + x], Z[x * cp + col]);
+      }
       int o = r0 + rr;
       int tidx = so, mul = p;
       double wt = scale;
@@ -211,7 +282,7 @@ __device__ void su_write_gamma(const DevGraph& g, cx<R>* t, int node, int slot, 
           const int q = o / chi;
           bj = o - q * chi;
           o = q;
-          wt *= lam_mask(lam[g.adj_edge[base + j] * chi + bj], eps);
+          wt *= lam_mask(wl[j * chi + bj], eps);
         }
         tidx += bj * mul;
         mul *= chi;
@@ -264,15 +335,27 @@ __global__ void __launch_bounds__(256)
   cd* Vs = V;  // right singular vectors
   cx<R>* ta = T + g.toff[na];
   cx<R>* tb = T + g.toff[nb];
+  // lambdas of the bonds of both nodes (read before this gate writes lambda_e)
+  __shared__ double wla[kMaxDeg * 64], wlb[kMaxDeg * 64];
+  {
+    const int da = g.deg[na], db = g.deg[nb], ba = g.adj_ptr[na], bb = g.adj_ptr[nb];
+    for (int f = threadIdx.x; f < da * chi; f += blockDim.x)
+      wla[f] = lam[(long long)g.adj_edge[ba + f / chi] * chi + f % chi];
+    for (int f = threadIdx.x; f < db * chi; f += blockDim.x)
+      wlb[f] = lam[(long long)g.adj_edge[bb + f / chi] * chi + f % chi];
+  }
+  __syncthreads();
 
-  su_environment<R>(g, ta, na, ia, lam, E, chunk, kSuChunk);
+  su_environment<R>(g, ta, na, ia, wla, E, chunk, kSuChunk);
   const double tra = su_gram(E, h1, chi, p, r, Gm, red);
   pivoted_cholesky(Gm, n, Ra, perm, rot);
-  su_environment<R>(g, tb, nb, ib, lam, E, chunk, kSuChunk);
+  su_environment<R>(g, tb, nb, ib, wlb, E, chunk, kSuChunk);
   const double trb = su_gram(E, h2, chi, p, r, Gm, red);
   pivoted_cholesky(Gm, n, Rb, perm, rot);
   // core = R_a diag(lambda_rep) R_b^T, column-major
-  const double* le = lam + (long long)e * chi;
+  __shared__ double le[128];
+  for (int j = threadIdx.x; j < chi; j += blockDim.x) le[j] = lam[(long long)e * chi + j];
+  __syncthreads();
   for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
     const int j = f / n, i = f % n;
     cd acc = mk<double>(0, 0);
@@ -280,7 +363,12 @@ __global__ void __launch_bounds__(256)
     C[f] = acc;
   }
   __syncthreads();
-  if (n <= 32) {
+  if (n == 32 && blockDim.x == 256) {
+    block_jacobi_svd_n<32>(C, U, Vs, sigma, perm, rot);
+  } else if (n == 16) {
+    if (threadIdx.x < 64) wave_jacobi_svd_n<16>(C, U, Vs, sigma, perm, rot);
+    __syncthreads();
+  } else if (n < 32) {
     if (threadIdx.x < 64) wave_jacobi_svd(C, n, n, U, Vs, sigma, perm, rot);
     __syncthreads();
   } else {
@@ -336,9 +424,9 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
     if (side == 0)
-      su_write_gamma<R>(g, ta, na, ia, lam, eps, Z, 1.0 / sqrt(tra), chunk, kSuChunk);
+      su_write_gamma<R>(g, ta, na, ia, wla, eps, Z, 1.0 / sqrt(tra), chunk, kSuChunk);
     else
-      su_write_gamma<R>(g, tb, nb, ib, lam, eps, Z, 1.0 / sqrt(trb), chunk, kSuChunk);
+      su_write_gamma<R>(g, tb, nb, ib, wlb, eps, Z, 1.0 / sqrt(trb), chunk, kSuChunk);
   }
   for (int j = threadIdx.x; j < chi; j += blockDim.x) lam[(long long)e * chi + j] = j < rank ? sigma[j] : 0.0;
 }

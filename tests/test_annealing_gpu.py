@@ -13,8 +13,8 @@ from tests.conftest import gpu_available
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
 
 from dequantizer_amd import PRNGKey  # noqa: E402
-from dequantizer_amd.annealing import (Annealer, bp_energy, get_scheduler, linear_schedule,  # noqa: E402
-                                       random_regular)
+from dequantizer_amd.annealing import (Annealer, bp_energy, edge_colored, get_scheduler,  # noqa: E402
+                                       linear_schedule, random_regular)
 from oracle import annealing as oa  # noqa: E402
 from oracle import graph as og  # noqa: E402
 
@@ -70,3 +70,16 @@ def test_batched_annealing_matches_oracle(sync, traversal, fixed, dtype, tol):
         if dtype == torch.complex128:
             assert res[k].bp_iterations == emu_o.bp_iterations
             np.testing.assert_allclose(res[k].truncation_errors, emu_o.truncation_errors, atol=1e-9)
+
+
+def test_edge_colored_order_matches_oracle():
+    """The colour-sorted pair list is an ordinary pair order for the oracle."""
+    efs = [edge_colored(random_regular(14, 3, seed=5))]
+    keys = [PRNGKey(9)]
+    ann = Annealer(efs, keys, 4, 1, 1e-5, 100, True, "bfs", dtype=torch.complex128, chi=4, fixed_sweeps=4)
+    for mix, cpl in get_scheduler(0.1, linear_schedule, 5):
+        ann.step(mix, cpl)
+    res = ann.finish(two_site=True)[0]
+    _, r1, r2 = _oracle_run(efs[0], keys[0], 5, 4, True, "bfs", 4)
+    assert np.abs(res.density_matrices - r1).max() < 1e-9
+    assert len(ann.plan.launches) <= 2 * 3 - 1 + 1  # greedy colours (<= 2d-1) + the mixing level
