@@ -48,13 +48,15 @@ def main():
     ap.add_argument("--dt", type=float, default=0.1)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--traversal", default="bfs")
-    ap.add_argument("--sync", action="store_true", help="synchronous BP sweeps")
+    ap.add_argument("--bp-schedule", default="sync", choices=["sync", "async"],
+                    help="synchronous sweeps, or asynchronous sweeps in --traversal order")
     ap.add_argument("--fixed-sweeps", type=int, default=10,
                     help="BP sweeps per regauge with the convergence test off (fixed work); 0 = converge to 1e-5")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="run the oracle CPU leg (1) or not (0)")
     ap.add_argument("--gate-order", default="colored", choices=["colored", "pairs"],
                     help="coupling-gate order: pair list sorted by edge colour, or the generator's pair order")
     args = ap.parse_args()
+    args.sync = args.bp_schedule == "sync"
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: every box in this setup has one GPU; "
               "multi-GPU sharding is described in DESIGN.md, not run", file=sys.stderr)

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libdq.so")
 
 DQ_C64 = 0
 DQ_C128 = 1
+DQ_TENSOR_WS = 1024
 
 _lock = threading.Lock()
 _lib = None
@@ -44,6 +45,9 @@ SIGNATURES = {
     "dq_symmetric_gauge": (c_int, [vp, c_int, vp, vp, c_dbl, vp, vp]),
     "dq_apply1": (c_int, [vp, c_int, vp, vp, c_int, vp, vp]),
     "dq_simple_update": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, c_int, c_dbl, c_int, vp, vp, vp]),
+    "dq_decompose_gates": (c_int, [vp, c_int, c_int, c_dbl, vp, vp, c_int, vp]),
+    "dq_tensor_apply1": (c_int, [c_int, vp, c_ll, c_int, vp, vp, vp, vp]),
+    "dq_tensor_apply_half": (c_int, [c_int, vp, c_ll, c_int, c_ll, c_int, vp, c_int, vp, vp, vp]),
     "dq_rdm1": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, vp]),
     "dq_rdm2": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, vp]),
     "dq_update_active": (c_int, [vp, c_dbl, vp, c_int, vp, c_int, vp]),

@@ -18,7 +18,7 @@ from typing import Callable, Iterable, List, Optional, Sequence, Tuple, Union
 import numpy as np
 import torch
 
-from .emulator import BatchedBPEmulator, decompose_gates
+from .emulator import BatchedBPEmulator
 from .graph import TensorGraph
 
 HADAMARD = np.sqrt(0.5) * np.array([[1, 1], [1, -1]], dtype=np.complex128)
@@ -208,10 +208,8 @@ class Annealer:
             due = [i for i in range(len(self.efs)) if self.emu.gates_since[i] >= self.emu.per_regauge[i]]
             self.emu._regauge(due)
             gates = np.concatenate([coupling_gates(ef, coupling_time) for ef in self.efs])
-            halves, ranks = decompose_gates(gates)
-            mg = mixing_gate(mixing_time)
-            n1 = sum(ef.nodes_number for ef in self.efs)
-            self.emu.run_plan(self.plan, halves, ranks, np.broadcast_to(mg, (n1, 2, 2)))
+            # coupling gates are diagonal: operator-Schmidt rank <= phys = 2
+            self.emu.run_plan(self.plan, gates, mixing_gate(mixing_time), rank_bound=2)
         else:
             layer = []
             for ef in self.efs:

@@ -130,12 +130,7 @@ __global__ void k_update_active(const double* dist, double acc, int* iters, int 
 }
 
 // ------------------------------------------------------------- launchers
-static inline int grid_for(long long total, int block) {
-  long long gsz = (total + block - 1) / block;
-  if (gsz > 65536) gsz = 65536;
-  if (gsz < 1) gsz = 1;
-  return (int)gsz;
-}
+
 
 hipError_t launch_bp_nodes(int rb, const DevGraph& g, const int* nodes, int count, const void* T, const void* n2e_old,
                            const void* e2n_old, void* n2e_new, const uint8_t* node_src, void* ws, long long wsb,

@@ -49,6 +49,9 @@ struct dq_graph {
   int* d_level_nodes = nullptr;
   uint8_t *d_node_src = nullptr, *d_edge_src = nullptr;
   int max_level = 0;
+  // every node once with all sources = incoming e2n (one synchronous level)
+  int* d_all_nodes = nullptr;
+  uint8_t* d_zero_src = nullptr;
   // owned scratch
   void* ws = nullptr;
   size_t ws_bytes = 0;
@@ -195,6 +198,8 @@ int dq_graph_create(dq_graph** out, int n_nodes, int n_edges, int chi, int phys,
   for (int n = 0; n < n_nodes; ++n) ln[n] = n;
   std::vector<unsigned char> ns(nadj, 0), es(2 * (size_t)n_edges, 1);
   rc = dq_bp_set_schedule(g, 1, lp.data(), ln.data(), ns.data(), es.data());
+  if (rc == DQ_OK) rc = upload(&g->d_all_nodes, ln.data(), (size_t)n_nodes);
+  if (rc == DQ_OK) rc = upload(&g->d_zero_src, ns.data(), (size_t)nadj);
   if (rc != DQ_OK) {
     dq_graph_destroy(g);
     return rc;
@@ -207,7 +212,8 @@ int dq_graph_destroy(dq_graph* g) {
   if (!g) return DQ_OK;
   void* ptrs[] = {g->d_deg,       g->d_adj_ptr,       g->d_adj_edge,      g->d_adj_side,   g->d_toff,
                   g->d_edge_node, g->d_node_inst,     g->d_edge_inst,     g->d_inst_node_ptr,
-                  g->d_inst_edge_ptr, g->d_level_nodes, g->d_node_src, g->d_edge_src, g->ws, g->X, g->node_dist};
+                  g->d_inst_edge_ptr, g->d_level_nodes, g->d_node_src, g->d_edge_src, g->ws, g->X, g->node_dist,
+                  g->d_all_nodes, g->d_zero_src};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete g;
@@ -320,6 +326,12 @@ int dq_vidal_distance(dq_graph* g, int dtype, const void* gammas, const double* 
   int rb, rc;
   if (!g || !out) return fail(DQ_ERR_INVALID, "NULL argument");
   if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  if (bp_fast_applicable(rb, g->dev, g->min_deg, g->max_deg)) {
+    if ((rc = ws_ensure(g, (size_t)vidal_fast_ws_bytes(g->dev))) != DQ_OK) return rc;
+    RET_LAUNCH(launch_vidal_distance_fast(g->dev, gammas, lam, eps, g->d_all_nodes, g->d_zero_src,
+                                          g->d_inst_edge_ptr, g->n_inst, out, g->ws, inst_active, S(stream)));
+    return DQ_OK;
+  }
   const long long cc = (long long)g->dev.chi * g->dev.chi;
   const long long wsv =
       ((3 * cc * (long long)sizeof(cd) + ((long long)g->max_deg * g->dev.chi + 4 * g->dev.chi + 16) * 8 + 16) + 255) /
@@ -344,6 +356,36 @@ int dq_apply1(dq_graph* g, int dtype, void* T, const int* nodes, int count, cons
   if (!g || count < 0) return fail(DQ_ERR_INVALID, "bad argument");
   if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
   RET_LAUNCH(launch_apply1(rb, g->dev, T, nodes, count, (const cd*)gates, S(stream)));
+  return DQ_OK;
+}
+
+int dq_decompose_gates(const void* gates, int count, int phys, double eps, void* halves, int* ranks, int rank_stride,
+                       void* stream) {
+  if (count < 0 || rank_stride < 1) return fail(DQ_ERR_INVALID, "bad argument");
+  if (phys < 1 || phys > 4) return fail(DQ_ERR_UNSUPPORTED, "gate split supports phys 1..4 (phys=%d)", phys);
+  if (count > 0 && (!gates || !halves || !ranks)) return fail(DQ_ERR_INVALID, "null pointer");
+  RET_LAUNCH(launch_decompose_gates((const cd*)gates, count, phys, eps, (cd*)halves, ranks, rank_stride, S(stream)));
+  return DQ_OK;
+}
+
+int dq_tensor_apply1(int dtype, const void* t, long long rows, int phys, const void* gate, void* out, double* ws,
+                     void* stream) {
+  int rb, rc;
+  if (rows < 0 || phys < 1 || (rows > 0 && (!t || !gate || !out || !ws))) return fail(DQ_ERR_INVALID, "bad argument");
+  if (t == out) return fail(DQ_ERR_INVALID, "dq_tensor_apply1: out must not alias the input");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  RET_LAUNCH(launch_tensor_gate(rb, t, rows, 1, 1, phys, (const cd*)gate, 1, out, ws, S(stream)));
+  return DQ_OK;
+}
+
+int dq_tensor_apply_half(int dtype, const void* t, long long lead, int bond, long long trail, int phys,
+                         const void* half, int rank, void* out, double* ws, void* stream) {
+  int rb, rc;
+  if (lead < 0 || bond < 0 || trail < 0 || phys < 1 || rank < 1) return fail(DQ_ERR_INVALID, "bad argument");
+  if (lead * bond * trail > 0 && (!t || !half || !out || !ws)) return fail(DQ_ERR_INVALID, "null pointer");
+  if (t == out) return fail(DQ_ERR_INVALID, "dq_tensor_apply_half: out must not alias the input");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  RET_LAUNCH(launch_tensor_gate(rb, t, lead, bond, trail, phys, (const cd*)half, rank, out, ws, S(stream)));
   return DQ_OK;
 }
 

@@ -293,6 +293,101 @@ hipError_t launch_vidal_distance(int rb, const DevGraph& g, const void* gam, con
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Vidal distance for the matrix-core graphs (degree 3, chi 16, phys 2,
+// complex64).  The trace-normalised R of vidal_dist (tensor_ops.py:143-175),
+// R = Gram over the other bonds of conj(Gamma) and Gamma * prod lambda_i^2,
+// is the BP node update (tensor_ops.py:36-60) with the diagonal messages
+// diag(lambda^2) on every bond: the MFMA node kernel computes all of them,
+// one per (edge, side).  One wave then forms R - ideal and its nuclear norm.
+
+// e2n[e][s] = diag(lambda_e^2).
+__global__ void __launch_bounds__(256) k_lam2_diag(DevGraph g, const double* lam, cx<float>* e2n) {
+  const long long total = 2LL * g.n_edges * 256;
+  for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < total;
+       f += (long long)gridDim.x * blockDim.x) {
+    const int e = (int)(f >> 9), k = (int)((f >> 4) & 15), l = (int)(f & 15);
+    const double v = lam[(long long)e * 16 + k];
+    e2n[f] = mk<float>(k == l ? (float)(v * v) : 0.0f, 0.0f);
+  }
+}
+
+// Per (edge, side): |R/tr(R) - ideal|_* with ideal = diag(R_kk/tr > eps)/count,
+// R Hermitised (it is a Gram matrix up to rounding).  4 waves per block.
+__global__ void __launch_bounds__(256)
+    k_vidal_nuclear16(DevGraph g, const cx<float>* R, double eps, double* es_dist, const int* inst_active) {
+  __shared__ cd A[4][256];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long es = (long long)blockIdx.x * 4 + w;
+  if (es >= 2LL * g.n_edges) return;  // whole wave: no block barrier below
+  const int e = (int)(es >> 1);
+  if (inst_active && !inst_active[g.node_inst[g.edge_node[2 * e]]]) return;
+  cd* G = A[w];
+  const cx<float>* r = R + es * 256;
+  for (int i = lane; i < 256; i += 64) G[i] = cvt<double>(r[i]);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  cd tr = mk<double>(0.0, 0.0);
+  for (int k = 0; k < 16; ++k) tr = tr + G[k * 17];
+  int cnt = 0;
+  for (int k = 0; k < 16; ++k) cnt += complex_gt(cdiv(G[k * 17], tr), eps) ? 1 : 0;
+  cd h[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int i = lane + 64 * t, k = i >> 4, l = i & 15;
+    const cd a = cdiv(G[i], tr), at = cdiv(G[l * 16 + k], tr);
+    h[t] = mk<double>(0.5 * (a.re + at.re), 0.5 * (a.im - at.im));
+    if (k == l && complex_gt(a, eps)) h[t].re -= 1.0 / cnt;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int t = 0; t < 4; ++t) G[lane + 64 * t] = h[t];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const double nuc = wave_nuclear_norm_n<16>(G);
+  if (lane == 0) es_dist[es] = nuc;
+}
+
+// out[i] = sum over the (edge, side) terms of instance i / (2 |E_i|).
+__global__ void __launch_bounds__(256)
+    k_vidal_reduce_es(const double* es_dist, const int* inst_edge_ptr, double* out, const int* inst_active) {
+  __shared__ double red[8];
+  const int inst = blockIdx.x;
+  if (inst_active && !inst_active[inst]) return;
+  const int e0 = inst_edge_ptr[inst], e1 = inst_edge_ptr[inst + 1];
+  double acc = 0.0;
+  for (int q = 2 * e0 + threadIdx.x; q < 2 * e1; q += blockDim.x) acc += es_dist[q];
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) out[inst] = s / (2.0 * (e1 - e0));
+}
+
+long long vidal_fast_ws_bytes(const DevGraph& g) { return 2LL * g.n_edges * (2 * 256 * 8 + 8) + 256; }
+
+hipError_t launch_vidal_distance_fast(const DevGraph& g, const void* gam, const double* lam, double eps,
+                                      const int* all_nodes, const uint8_t* zero_src, const int* inst_edge_ptr,
+                                      int n_inst, double* inst_dist, void* ws, const int* inst_active,
+                                      hipStream_t s) {
+  if (g.n_edges <= 0) return hipSuccess;
+  cx<float>* msg = (cx<float>*)ws;
+  cx<float>* R = msg + 2LL * g.n_edges * 256;
+  double* es_dist = (double*)(R + 2LL * g.n_edges * 256);
+  const long long total = 2LL * g.n_edges * 256;
+  hipLaunchKernelGGL(k_lam2_diag, dim3(grid_for(total, 256)), dim3(256), 0, s, g, lam, msg);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  err = launch_bp_nodes_fast(g, all_nodes, g.n_nodes, gam, msg, msg, R, zero_src, inst_active, s);
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(k_vidal_nuclear16, dim3((unsigned)((2LL * g.n_edges + 3) / 4)), dim3(256), 0, s, g, R, eps,
+                     es_dist, inst_active);
+  err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  if (n_inst > 0)
+    hipLaunchKernelGGL(k_vidal_reduce_es, dim3(n_inst), dim3(256), 0, s, es_dist, inst_edge_ptr, inst_dist,
+                       inst_active);
+  return hipGetLastError();
+}
+
 hipError_t launch_symmetric_gauge(int rb, const DevGraph& g, const void* gam, const double* lam, double eps, void* out,
                                   hipStream_t s) {
   if (g.n_nodes <= 0) return hipSuccess;

@@ -9,6 +9,14 @@
 
 namespace dq {
 
+// Grid size of a grid-stride loop over `total` items (capped at 65536 blocks).
+inline int grid_for(long long total, int block) {
+  long long gsz = (total + block - 1) / block;
+  if (gsz > 65536) gsz = 65536;
+  if (gsz < 1) gsz = 1;
+  return (int)gsz;
+}
+
 // Per-level BP node update.  node_src codes per adjacency slot:
 // 0 = e2n_old[e][side], 1 = n2e_new[e][1-side], 2 = n2e_old[e][1-side].
 hipError_t launch_bp_nodes(int real_bytes, const DevGraph& g, const int* nodes, int count, const void* T,
@@ -54,6 +62,15 @@ hipError_t launch_vidal_distance(int real_bytes, const DevGraph& g, const void* 
                                  double* inst_dist, void* ws, long long ws_bytes_per_block, const int* inst_active,
                                  hipStream_t s);
 
+// Vidal distance on the matrix-core graphs (bp_fast_applicable): diag(lam^2)
+// messages through the MFMA node kernel, then one wave per (edge, side).
+// ws: vidal_fast_ws_bytes(g); all_nodes = 0..n-1, zero_src = zeros(sum deg).
+long long vidal_fast_ws_bytes(const DevGraph& g);
+hipError_t launch_vidal_distance_fast(const DevGraph& g, const void* gam, const double* lam, double eps,
+                                      const int* all_nodes, const uint8_t* zero_src, const int* inst_edge_ptr,
+                                      int n_inst, double* inst_dist, void* ws, const int* inst_active,
+                                      hipStream_t s);
+
 // Symmetric gauge: out = Gamma * prod_i sqrt(lam_i > eps) normalised.
 hipError_t launch_symmetric_gauge(int real_bytes, const DevGraph& g, const void* gammas, const double* lam, double eps,
                                   void* out, hipStream_t s);
@@ -90,6 +107,18 @@ hipError_t launch_rdm2(int real_bytes, const DevGraph& g, const void* T, const v
 // Re-pad node tensors from chi_old to the graph's chi (zeros in new slots).
 hipError_t launch_repad(int real_bytes, const DevGraph& g_new, const long long* toff_old, int chi_old,
                         const void* T_old, void* T_new, hipStream_t s);
+
+// Gate half (r, out, in) on one unpadded tensor viewed as (lead, bond, trail,
+// phys), fused bond b*r+q, then Frobenius normalisation (dq_tensor.hip).
+// `partial` holds kTensorParts doubles.
+constexpr int kTensorParts = 1024;
+hipError_t launch_tensor_gate(int real_bytes, const void* t, long long lead, int bond, long long trail, int p,
+                              const cd* half, int r, void* out, double* partial, hipStream_t s);
+
+// Operator-Schmidt split of two-site gates (dq_gates.hip): halves
+// [count][2][p^4], ranks[i * rank_stride].  phys 1..4.
+hipError_t launch_decompose_gates(const cd* gates, int count, int p, double eps, cd* halves, int* ranks,
+                                  int rank_stride, hipStream_t s);
 
 // Per-instance convergence bookkeeping of the BP-to-Vidal loop.
 hipError_t launch_update_active(const double* dist, double acc, int* iters, int max_iters, int* active, int n_inst,

@@ -444,6 +444,74 @@ __device__ inline void wave_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, int
   __builtin_amdgcn_wave_barrier();
 }
 
+// Sum of the singular values (nuclear norm) of a square N x N matrix G
+// (column-major, overwritten) by the same wave-synchronous one-sided Jacobi
+// iteration, without accumulating singular vectors.  ONE wavefront; every
+// lane gets the result.
+template <int N>
+__device__ inline double wave_nuclear_norm_n(cd* G) {
+  constexpr int NP = N / 2;
+  constexpr int LPP = 64 / NP;
+  constexpr int RPL = N / LPP;
+  const int lane = threadIdx.x & 63;
+  const int k = lane / LPP, sub = lane % LPP;
+  double f = 0.0;
+  for (int i = lane; i < N * N; i += 64) f += norm2(G[i]);
+  f = wave_sum(f);
+  const double abs_tol = 1e-32 * f;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    int rotated = 0;
+    for (int r = 0; r < N - 1; ++r) {
+      int p, q;
+      rr_pair(r, k, N, p, q);
+      cd x[RPL], y[RPL];
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        x[t] = G[p * N + sub + t * LPP];
+        y[t] = G[q * N + sub + t * LPP];
+      }
+      double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        al += norm2(x[t]);
+        be += norm2(y[t]);
+        gr += x[t].re * y[t].re + x[t].im * y[t].im;
+        gi += x[t].re * y[t].im - x[t].im * y[t].re;
+      }
+#pragma unroll
+      for (int o = LPP / 2; o > 0; o >>= 1) {
+        al += __shfl_xor(al, o, 64);
+        be += __shfl_xor(be, o, 64);
+        gr += __shfl_xor(gr, o, 64);
+        gi += __shfl_xor(gi, o, 64);
+      }
+      double c, s, er, ei;
+      if (jacobi_params(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
+        rotated = 1;
+        const cd e = mk<double>(er, ei);
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          const cd ey = e * y[t];
+          G[p * N + sub + t * LPP] = mk<double>(c * x[t].re - s * ey.re, c * x[t].im - s * ey.im);
+          G[q * N + sub + t * LPP] = mk<double>(s * x[t].re + c * ey.re, s * x[t].im + c * ey.im);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (!__any(rotated)) break;
+  }
+  double nrm = 0.0;
+  if (lane < N) {
+#pragma unroll 8
+    for (int i = 0; i < N; ++i) nrm += norm2(G[lane * N + i]);
+    nrm = sqrt(nrm);
+  }
+  return wave_sum(nrm);
+}
+
 // Block-parallel fixed-size one-sided Jacobi SVD (N = 32, 256 threads): the
 // 16 pairs of a round run on 16-lane groups (4 per wave), 2 rows per lane, one
 // block barrier per round.  Same outputs as jacobi_svd.  scal: N doubles.

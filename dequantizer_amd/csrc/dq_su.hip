@@ -306,8 +306,8 @@ constexpr int kSuLdsN = 32;  // LDS variant: n = chi*r <= 32 and chi*phys <= 32
 
 template <typename R, bool kLds>
 __global__ void __launch_bounds__(256)
-    k_su(DevGraph g, cx<R>* T, double* lam, const SuGate* gates, const cd* halves, double eps, int max_rank,
-         double* err_out, double* ent_out, char* ws, long long wsb) {
+    k_su(DevGraph g, cx<R>* T, double* lam, const SuGate* gates, const cd* halves, int max_r, double eps,
+         int max_rank, double* err_out, double* ent_out, char* ws, long long wsb) {
   __shared__ cd sm[kLds ? (6 * kSuLdsN * kSuLdsN + kSuChunk * kSuLdsN + 64) : 1];
   __shared__ double red[8];
   __shared__ double rot[4 * 64 + 8];
@@ -316,6 +316,15 @@ __global__ void __launch_bounds__(256)
   __shared__ int perm[128];
   __shared__ int sh_rank;
   const SuGate G = gates[blockIdx.x];
+  if (G.rank < 1 || G.rank > max_r) {
+    // rank outside what this launch's buffers were sized for: the gate is
+    // skipped and flagged with NaN error and entropy (uniform per block)
+    if (threadIdx.x == 0) {
+      err_out[blockIdx.x] = __builtin_nan("");
+      ent_out[blockIdx.x] = __builtin_nan("");
+    }
+    return;
+  }
   const int e = G.edge, r = G.rank, chi = g.chi, p = g.phys;
   const int n = chi * r, cp = chi * p;
   const int na = g.edge_node[2 * e + G.first], nb = g.edge_node[2 * e + 1 - G.first];
@@ -449,18 +458,18 @@ hipError_t launch_su(int rb, const DevGraph& g, void* T, double* lam, const SuGa
   const bool lds = su_lds_fits(g.chi, g.phys, max_gate_rank);
   if (rb == 4) {
     if (lds)
-      hipLaunchKernelGGL((k_su<float, true>), dim3(count), dim3(256), 0, s, g, (cx<float>*)T, lam, gates, halves, eps,
-                         max_rank, err, entropy, (char*)ws, wsb);
+      hipLaunchKernelGGL((k_su<float, true>), dim3(count), dim3(256), 0, s, g, (cx<float>*)T, lam, gates, halves,
+                         max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb);
     else
       hipLaunchKernelGGL((k_su<float, false>), dim3(count), dim3(256), 0, s, g, (cx<float>*)T, lam, gates, halves,
-                         eps, max_rank, err, entropy, (char*)ws, wsb);
+                         max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb);
   } else {
     if (lds)
       hipLaunchKernelGGL((k_su<double, true>), dim3(count), dim3(256), 0, s, g, (cx<double>*)T, lam, gates, halves,
-                         eps, max_rank, err, entropy, (char*)ws, wsb);
+                         max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb);
     else
       hipLaunchKernelGGL((k_su<double, false>), dim3(count), dim3(256), 0, s, g, (cx<double>*)T, lam, gates, halves,
-                         eps, max_rank, err, entropy, (char*)ws, wsb);
+                         max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb);
   }
   return hipGetLastError();
 }

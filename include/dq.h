@@ -8,7 +8,7 @@
  *     description.  Nothing is synchronised unless stated.
  *   - dtype: DQ_C64 (complex64 storage, float arithmetic for the bulk
  *     contractions) or DQ_C128 (complex128).  Eigen/singular decompositions,
- *     QR, lambdas and residuals are always FP64.
+ *     Cholesky factors, lambdas and residuals are always FP64.
  *   - A graph handle fixes the topology, a padded bond dimension chi and a
  *     uniform physical dimension phys.  Node tensors live in one buffer at
  *     offsets from dq_graph_tensor_offsets, row-major (bond_0..bond_{d-1},
@@ -113,11 +113,24 @@ int dq_symmetric_gauge(dq_graph* g, int dtype, const void* gammas, const double*
  * (dequantizer/src/graph/tensor_graph.py:268-275). */
 int dq_apply1(dq_graph* g, int dtype, void* tensors, const int* nodes, int count, const void* gates, void* stream);
 
+/* Operator-Schmidt split of `count` two-site gates (device complex128
+ * gates[count][phys][phys][phys][phys], axes (out1, out2, in1, in2)): SVD of
+ * the (out1 in1) x (out2 in2) matrix in FP64, the r singular values above
+ * eps kept in descending order.  halves (device complex128
+ * [count][2][phys^4]): half 1 = (r, out1, in1), half 2 = (r, out2, in2), each
+ * scaled by sqrt(s), zero beyond r*phys^2.  ranks[i * rank_stride] = r
+ * (rank_stride 4 fills the rank field of dq_simple_update's gate records).
+ * Replaces _decompose_gate (dequantizer/src/tensor_ops.py:202-218).
+ * phys 1..4. */
+int dq_decompose_gates(const void* gates, int count, int phys, double eps, void* halves, int* ranks, int rank_stride,
+                       void* stream);
+
 /* Simple update of `count` two-site gates on pairwise node-disjoint edges
  * (device int gates[count][4] = {edge, side of the controlling node, gate rank
  * r, 0}; device complex128 halves[count][2][phys^4], each half (r, out, in)
  * from the operator-Schmidt split; max_gate_rank = an upper bound of the r,
- * which selects the kernel variant).  Updates tensors (Vidal Gammas) and
+ * which selects the kernel variant; a gate whose rank is above it is skipped
+ * and gets NaN error and entropy).  Updates tensors (Vidal Gammas) and
  * lambdas in place; per-gate truncation error and entropy (device double).
  * Replaces TensorGraph.apply2_to_vidal_canonical (tensor_graph.py:294-375) =
  * simple_update (tensor_ops.py:318-389) with accuracy None and
@@ -125,6 +138,26 @@ int dq_apply1(dq_graph* g, int dtype, void* tensors, const int* nodes, int count
 int dq_simple_update(dq_graph* g, int dtype, void* tensors, double* lam, const int* gates, int count,
                      const void* halves, int max_gate_rank, double eps, int max_rank, double* err, double* entropy,
                      void* stream);
+
+/* Workspace of the single-tensor calls below: device double[DQ_TENSOR_WS]. */
+#define DQ_TENSOR_WS 1024
+
+/* One-site gate on one unpadded tensor t[rows][phys] (any bond shape,
+ * flattened): out[x][o] = sum_i gate[o][i] t[x][i], then out /= |out|_F.
+ * gate: device complex128 [phys][phys]; out must not alias t.  Replaces
+ * TensorGraph.apply1 (dequantizer/src/graph/tensor_graph.py:268-275) of the
+ * dictionary API. */
+int dq_tensor_apply1(int dtype, const void* t, long long rows, int phys, const void* gate, void* out, double* ws,
+                     void* stream);
+
+/* Gate half on one unpadded tensor viewed as t[lead][bond][trail][phys]:
+ * out[l][b*rank+q][tr][o] = sum_i half[q][o][i] t[l][b][tr][i], then
+ * out /= |out|_F.  half: device complex128 [rank][phys][phys]; out has
+ * lead*bond*rank*trail*phys elements and must not alias t.  Replaces
+ * _apply_half_gate (dequantizer/src/tensor_ops.py:221-237), used by
+ * TensorGraph.apply2 (tensor_graph.py:387-446). */
+int dq_tensor_apply_half(int dtype, const void* t, long long lead, int bond, long long trail, int phys,
+                         const void* half, int rank, void* out, double* ws, void* stream);
 
 /* One-site density matrices (device complex128 rho[count][phys][phys]) from
  * tensors and edge->node messages.  Replaces get_one_side_density_matrix

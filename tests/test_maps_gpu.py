@@ -192,3 +192,34 @@ def test_bp_matrix_core_path_chi16(sync, order):
                  {k: v.astype(np.complex64) for k, v in messages.items()})
     worst = max(_rel(host(out_p[dq.MessageID(k.src, k.dst)]), v) for k, v in out_o.items())
     assert worst < 2e-5, worst
+
+
+@pytest.mark.parametrize("zero_tail", [False, True])
+def test_vidal_distance_matrix_core_path_chi16(zero_tail):
+    """At chi=16, degree 3, complex64 the Vidal distance runs through the
+    matrix-core node kernel (diag(lambda^2) messages) and a wave-level nuclear
+    norm; compared with the complex128 oracle.  zero_tail: the last four
+    lambdas of every other edge are zero, which changes the ideal's count."""
+    gp, go = regular_pair(24, 16, 5)
+    rng = np.random.default_rng(8)
+    tensors = random_tensors(gp, rng)
+    lambdas = {}
+    for k, eid in enumerate(go.edges):
+        lm = np.sort(rng.uniform(0.05, 1.0, 16))[::-1]
+        if zero_tail and k % 2:
+            lm[12:] = 0.0
+        lambdas[eid] = lm
+    trav_p = list(gp.get_traversal_iterator())
+    trav_o = go.traversal()
+    ref = om.vidal_gauge_distance_map(trav_o, 1e-10)(tensors, lambdas)
+    got = float(dq.get_vidal_gauge_distance_map(trav_p, 1e-10)(
+        {k: v.astype(np.complex64) for k, v in tensors.items()}, lambdas))
+    assert abs(got - ref) < 2e-5 * max(1.0, abs(ref)), (got, ref)
+    # a canonical state has distance ~0: Vidal gauge of converged messages
+    bp = dq.get_belief_propagation_map(trav_p, True)
+    t64 = {k: v.astype(np.complex64) for k, v in tensors.items()}
+    msgs = {k: v.astype(np.complex64) for k, v in random_messages(gp, rng).items()}
+    for _ in range(60):
+        msgs = bp(t64, msgs)
+    gam, lam = dq.get_vidal_gauge_fixing_map(trav_p, 1e-10)(t64, msgs)
+    assert float(dq.get_vidal_gauge_distance_map(trav_p, 1e-10)(gam, lam)) < 1e-4

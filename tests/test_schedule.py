@@ -106,3 +106,40 @@ def test_level_counts():
         topo = Topology([list(g.get_traversal_iterator(ordering=order))])
         assert topo.schedule(True)[0] == 1
         assert lo <= topo.schedule(False)[0] <= hi, (order, topo.schedule(False)[0])
+
+
+def test_gate_rank_bounds():
+    """The structural bound used before the device split: at least the SVD
+    rank, and equal to it for diagonal and controlled gates."""
+    from dequantizer_amd.emulator import gate_rank_bounds
+    rng = np.random.default_rng(4)
+    cnot = np.eye(4)[[0, 1, 3, 2]].reshape(2, 2, 2, 2)
+    diag = np.diag(np.exp(1j * rng.uniform(0, 6, 4))).reshape(2, 2, 2, 2)
+    ident = np.eye(4).reshape(2, 2, 2, 2)
+    dense = (rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4))).reshape(2, 2, 2, 2)
+    gates = np.stack([cnot, diag, ident, dense])
+    bounds = gate_rank_bounds(gates)
+    mats = np.transpose(gates, (0, 1, 3, 2, 4)).reshape(4, 4, 4)
+    ranks = (np.linalg.svd(mats, compute_uv=False) > 1e-10).sum(axis=1)
+    assert np.all(bounds >= ranks)
+    assert list(bounds) == [2, 2, 2, 4]
+    assert list(ranks) == [2, 2, 1, 4]
+
+
+def test_dims_bookkeeping_matches_shapes():
+    """apply_dims: the new bond is min(max_chi, m_a, m_b, chi_e r), checked on
+    a 3-node chain against the reference's shape rule worked by hand."""
+    from dequantizer_amd.emulator import apply_dims, dim_levels
+    # chain 0 -e0- 1 -e1- 2, phys 2; adjacency padded with index 2 (= n_edges)
+    adjpad = np.array([[0, 2], [0, 1], [1, 2]])
+    dims = np.array([1, 1])
+    levels = dim_levels(adjpad, 2, [0, 1], [0, 1], [1, 2], [0, 1])
+    apply_dims(dims, levels, np.array([2, 2]), 2, 16)
+    # gate on e0: m_0 = 2, m_1 = 2*dims[e1] = 2 -> min(2, 2, 1*2) = 2;
+    # gate on e1: m_1 = 2*dims[e0] = 4, m_2 = 2 -> min(4, 2, 1*2) = 2
+    assert list(dims) == [2, 2]
+    apply_dims(dims, levels, np.array([2, 2]), 2, 16)
+    # e0: m_0 = 2 -> 2; e1: m_2 = 2 -> 2 (leaves cap the bonds at phys)
+    assert list(dims) == [2, 2]
+    apply_dims(dims, levels, np.array([1, 1]), 2, 1)
+    assert list(dims) == [1, 1]
