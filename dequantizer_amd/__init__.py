@@ -34,6 +34,8 @@ from .mappings import (
     lambdas2messages,
     messages_frob_distance,
 )
+from ._lib import DQError
+from .emulator import BatchedBPEmulator, BPQuantumEmulator
 from .random import PRNGKey, split
 
 __all__ = [
@@ -44,5 +46,5 @@ __all__ = [
     "get_heavy_hex_ibm_eagle_lattice_infinite", "get_belief_propagation_map", "get_vidal_gauge_fixing_map",
     "get_vidal_gauge_distance_map", "get_symmetric_gauge_fixing_map", "lambdas2messages",
     "messages_frob_distance", "get_one_side_density_matrix", "get_two_side_density_matrix",
-    "PRNGKey", "split",
+    "PRNGKey", "split", "BPQuantumEmulator", "BatchedBPEmulator", "DQError",
 ]

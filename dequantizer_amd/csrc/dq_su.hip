@@ -164,6 +164,139 @@ __device__ void su_environment(const DevGraph& g, const cx<R>* t, int node, int 
   __syncthreads();
 }
 
+// ------------------------------------------------ FP64 matrix-core paths
+// For cp = chi * phys = 32 and chi^(d-1) rows a multiple of 16, the two
+// GEMM-shaped steps of a side run on v_mfma_f64_16x16x4_f64 (exact FP64
+// products and sums, like the FMA paths).  Lane l of a 16x16x4 step holds
+// A[l % 16][l / 16] and B[l / 16][l % 16]; the result D[l / 16 + 4 v][l % 16],
+// v = 0..3, stays in its four accumulator registers (rows interleaved by 4,
+// unlike the FP32 16x16x4 variant).  Operands come straight
+// from the node tensor (L2-resident) with the lambda weights applied in FP64.
+typedef double su_d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ su_d4 su_mfma(double a, double b, su_d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Row o of the other bonds -> element offset (bond `slot` and phys at 0) and
+// the product of fn(lambda) over those bonds.
+template <bool kMask>
+__device__ __forceinline__ long long su_row(const DevGraph& g, int d, int chi, int slot, const int* str,
+                                            const double* wl, double eps, int o, double& wt) {
+  long long off = 0;
+  wt = 1.0;
+  for (int j = d - 1; j >= 0; --j) {
+    if (j == slot) continue;
+    const int q = o / chi, bj = o - q * chi;
+    o = q;
+    off += (long long)bj * str[j];
+    const double l = wl[j * chi + bj];
+    wt *= kMask ? lam_mask(l, eps) : l;
+  }
+  return off;
+}
+
+// E (32 x 32) = X^H X, X[o][(b,s)] = t[...] * prod_{j != slot} lambda_j[b_j].
+// Wave w computes the 16x16 tile (w / 2, w % 2); 4 MFMAs per 4 rows.
+template <typename R>
+__device__ void su_environment_mfma(const DevGraph& g, const cx<R>* t, int node, int slot, const double* wl,
+                                    cd* E) {
+  const int d = g.deg[node], chi = g.chi, p = g.phys;
+  const int rows = (int)ipow(chi, d - 1);
+  int str[kMaxDeg];
+  {
+    int m = p;
+    for (int j = d - 1; j >= 0; --j) {
+      str[j] = m;
+      m *= chi;
+    }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int x0 = (w >> 1) * 16, y0 = (w & 1) * 16;
+  const int i = lane & 15, kk = lane >> 4;
+  const long long offx = (long long)((x0 + i) / p) * str[slot] + (x0 + i) % p;
+  const long long offy = (long long)((y0 + i) / p) * str[slot] + (y0 + i) % p;
+  su_d4 are = {0.0, 0.0, 0.0, 0.0}, aim = {0.0, 0.0, 0.0, 0.0};
+  for (int o0 = 0; o0 < rows; o0 += 16) {
+    cx<R> va[4], vb[4];
+    double wt[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long off = su_row<false>(g, d, chi, slot, str, wl, 0.0, o0 + 4 * u + kk, wt[u]);
+      va[u] = t[off + offx];
+      vb[u] = t[off + offy];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double ar = wt[u] * (double)va[u].re, ai = wt[u] * (double)va[u].im;
+      const double br = wt[u] * (double)vb[u].re, bi = wt[u] * (double)vb[u].im;
+      are = su_mfma(ar, br, are);
+      are = su_mfma(ai, bi, are);
+      aim = su_mfma(ar, bi, aim);
+      aim = su_mfma(-ai, br, aim);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) E[(x0 + kk + 4 * v) * 32 + y0 + i] = mk<double>(are[v], aim[v]);
+  __syncthreads();
+}
+
+// Gamma rows o (other bonds) x (j, s') = scale * mask(o) * sum_x X[o][x] Z[x][(j,s')],
+// X unweighted, in place.  Wave w owns the row tiles w, w+4, ... (both column
+// tiles), so every element it overwrites was read by the same wave first.
+template <typename R>
+__device__ void su_write_gamma_mfma(const DevGraph& g, cx<R>* t, int node, int slot, const double* wl, double eps,
+                                    const cd* Z, double scale) {
+  const int d = g.deg[node], chi = g.chi, p = g.phys;
+  const int rows = (int)ipow(chi, d - 1);
+  int str[kMaxDeg];
+  {
+    int m = p;
+    for (int j = d - 1; j >= 0; --j) {
+      str[j] = m;
+      m *= chi;
+    }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = lane & 15, kk = lane >> 4;
+  for (int rt = w; rt * 16 < rows; rt += 4) {
+    double unused;
+    const long long offa = su_row<false>(g, d, chi, slot, str, wl, 0.0, rt * 16 + i, unused);
+    su_d4 r0 = {0.0, 0.0, 0.0, 0.0}, i0 = r0, r1 = r0, i1 = r0;
+    cx<R> va[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int x = 4 * u + kk;
+      va[u] = t[offa + (long long)(x / p) * str[slot] + x % p];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int x = 4 * u + kk;
+      const double ar = (double)va[u].re, ai = (double)va[u].im;
+      const cd z0 = Z[x * 32 + i], z1 = Z[x * 32 + 16 + i];
+      r0 = su_mfma(ar, z0.re, r0);
+      r0 = su_mfma(-ai, z0.im, r0);
+      i0 = su_mfma(ar, z0.im, i0);
+      i0 = su_mfma(ai, z0.re, i0);
+      r1 = su_mfma(ar, z1.re, r1);
+      r1 = su_mfma(-ai, z1.im, r1);
+      i1 = su_mfma(ar, z1.im, i1);
+      i1 = su_mfma(ai, z1.re, i1);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double wt;
+      const int o = rt * 16 + kk + 4 * v;
+      const long long off = su_row<true>(g, d, chi, slot, str, wl, eps, o, wt);
+      const double f = scale * wt;
+      const int c0 = i, c1 = 16 + i;
+      t[off + (long long)(c0 / p) * str[slot] + c0 % p] = cvt<R>(f * mk<double>(r0[v], i0[v]));
+      t[off + (long long)(c1 / p) * str[slot] + c1 % p] = cvt<R>(f * mk<double>(r1[v], i1[v]));
+    }
+  }
+  __syncthreads();
+}
+
 // G = E (x) K combined with the gate half h (r, p, p): G[(b,q),(b',q')] =
 // sum_{s,s'} E[(b,s),(b',s')] sum_o conj(h[q,o,s]) h[q',o,s'];  G /= trace(G).
 // Returns trace(G) before normalisation (= |A|^2).
@@ -355,10 +488,18 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
 
-  su_environment<R>(g, ta, na, ia, wla, E, chunk, kSuChunk);
+  const bool mfma_a = cp == 32 && blockDim.x == 256 && ipow(chi, g.deg[na] - 1) % 16 == 0;
+  const bool mfma_b = cp == 32 && blockDim.x == 256 && ipow(chi, g.deg[nb] - 1) % 16 == 0;
+  if (mfma_a)
+    su_environment_mfma<R>(g, ta, na, ia, wla, E);
+  else
+    su_environment<R>(g, ta, na, ia, wla, E, chunk, kSuChunk);
   const double tra = su_gram(E, h1, chi, p, r, Gm, red);
   pivoted_cholesky(Gm, n, Ra, perm, rot);
-  su_environment<R>(g, tb, nb, ib, wlb, E, chunk, kSuChunk);
+  if (mfma_b)
+    su_environment_mfma<R>(g, tb, nb, ib, wlb, E);
+  else
+    su_environment<R>(g, tb, nb, ib, wlb, E, chunk, kSuChunk);
   const double trb = su_gram(E, h2, chi, p, r, Gm, red);
   pivoted_cholesky(Gm, n, Rb, perm, rot);
   // core = R_a diag(lambda_rep) R_b^T, column-major
@@ -432,10 +573,17 @@ __global__ void __launch_bounds__(256)
       Z[f] = acc;
     }
     __syncthreads();
-    if (side == 0)
-      su_write_gamma<R>(g, ta, na, ia, wla, eps, Z, 1.0 / sqrt(tra), chunk, kSuChunk);
-    else
-      su_write_gamma<R>(g, tb, nb, ib, wlb, eps, Z, 1.0 / sqrt(trb), chunk, kSuChunk);
+    if (side == 0) {
+      if (mfma_a)
+        su_write_gamma_mfma<R>(g, ta, na, ia, wla, eps, Z, 1.0 / sqrt(tra));
+      else
+        su_write_gamma<R>(g, ta, na, ia, wla, eps, Z, 1.0 / sqrt(tra), chunk, kSuChunk);
+    } else {
+      if (mfma_b)
+        su_write_gamma_mfma<R>(g, tb, nb, ib, wlb, eps, Z, 1.0 / sqrt(trb));
+      else
+        su_write_gamma<R>(g, tb, nb, ib, wlb, eps, Z, 1.0 / sqrt(trb), chunk, kSuChunk);
+    }
   }
   for (int j = threadIdx.x; j < chi; j += blockDim.x) lam[(long long)e * chi + j] = j < rank ? sigma[j] : 0.0;
 }

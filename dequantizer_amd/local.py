@@ -191,7 +191,13 @@ def simple_update(graph, edge, n1, n2, tensors, lambdas, gate, threshold, accura
     m1 = p * int(np.prod([x for k, x in enumerate(dims1) if k != i1], dtype=np.int64))
     m2 = p * int(np.prod([x for k, x in enumerate(dims2) if k != i2], dtype=np.int64))
     n_s = min(m1, m2, de * r)  # length of the reference's singular spectrum
-    chi = max(dims1 + dims2 + [n_s])
+    if accuracy is None:
+        rank = n_s if max_rank is None else min(int(max_rank), n_s)
+        # the kernel computes all chi * r singular values and keeps the top
+        # min(max_rank, chi): padding to the kept rank is enough
+        chi = max(dims1 + dims2 + [rank])
+    else:
+        chi = max(dims1 + dims2 + [n_s])
     h = _local_handle(len(dims1), len(dims2), i1, i2, chi, p)
     T = torch.zeros(h.tensor_elems, dtype=dtype, device=DEVICE)
     h.node_view(T, 0)[tuple(slice(0, x) for x in dims1)] = t1
@@ -201,7 +207,6 @@ def simple_update(graph, edge, n1, n2, tensors, lambdas, gate, threshold, accura
         for lm, e in zip(node_lams, slots):
             lam[e, : lm.shape[0]] = lm
     if accuracy is None:
-        rank = n_s if max_rank is None else min(int(max_rank), n_s)
         kernel_rank = max(1, rank)
     else:
         kernel_rank = chi

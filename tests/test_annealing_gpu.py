@@ -83,3 +83,23 @@ def test_edge_colored_order_matches_oracle():
     _, r1, r2 = _oracle_run(efs[0], keys[0], 5, 4, True, "bfs", 4)
     assert np.abs(res.density_matrices - r1).max() < 1e-9
     assert len(ann.plan.launches) <= 2 * 3 - 1 + 1  # greedy colours (<= 2d-1) + the mixing level
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.complex128, 1e-9), (torch.complex64, 2e-5)])
+def test_chi16_matrix_core_paths_match_oracle(dtype, tol):
+    """Padded chi=16 on a degree-3 graph: the simple update runs its FP64
+    matrix-core environment / Gamma products (chi * phys = 32) and, in
+    complex64, BP and the Vidal distance run on the MFMA node kernel."""
+    ef = random_regular(10, 3, seed=4)
+    key = PRNGKey(31)
+    steps = 7
+    ann = Annealer([ef], [key], 16, 1, 1e-5, 100, True, "bfs", dtype=dtype, chi=16, fixed_sweeps=6)
+    for mix, cpl in get_scheduler(0.1, linear_schedule, steps):
+        ann.step(mix, cpl)
+    res = ann.finish(two_site=True)[0]
+    emu_o, r1, r2 = _oracle_run(ef, key, steps, 16, True, "bfs", 6)
+    assert np.abs(res.density_matrices - r1).max() < tol
+    assert np.abs(res.two_site_density_matrices - r2).max() < tol
+    np.testing.assert_allclose(res.truncation_errors, emu_o.truncation_errors, atol=max(tol, 1e-9))
+    vd_o = np.asarray(emu_o.vidal_distances_after_regauging, dtype=np.float64)
+    assert np.abs(np.asarray(res.vidal_distances_after_regauging, dtype=np.float64) - vd_o).max() < max(tol, 1e-9) * 10

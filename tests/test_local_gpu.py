@@ -150,8 +150,9 @@ def _graph_for(dims1, dims2, i1, i2):
     ((4, 3), (2, 4, 3), 0, 1),
     ((2, 2), (2, 2), 0, 0),
     ((1, 3), (2, 3), 1, 1),
+    ((16, 16, 16), (16, 16, 16), 2, 0),  # cp = 32: FP64 matrix-core environment / Gamma path
 ])
-@pytest.mark.parametrize("accuracy,max_rank", [(None, None), (None, 3), (1e-1, None)])
+@pytest.mark.parametrize("accuracy,max_rank", [(None, None), (None, 3), (1e-1, None), (None, 16)])
 @pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-9), (np.complex64, 1e-4)])
 def test_simple_update_matches_oracle(dims1, dims2, i1, i2, accuracy, max_rank, dtype, tol):
     rng = np.random.default_rng(5)
@@ -165,6 +166,12 @@ def test_simple_update_matches_oracle(dims1, dims2, i1, i2, accuracy, max_rank, 
             lambdas[(a, b) if k == shared else leaves[(node, k)]] = lm
     tensors = {a: t1.astype(dtype), b: t2.astype(dtype)}
     gate = _random_gate(rng)
+    if dims1 == (16, 16, 16) and (accuracy is not None or max_rank is None):
+        # the full spectrum (16 * 4 values) needs chi = 64 padding: outside
+        # the kernel's chi * phys <= 64, chi * phys^2 <= 128
+        with pytest.raises(dq.DQError, match="supports chi"):
+            g.apply2_to_vidal_canonical(a, b, tensors, lambdas, gate, 1e-10, accuracy, max_rank)
+        return
     tensors, lambdas, err, ent = g.apply2_to_vidal_canonical(a, b, tensors, lambdas, gate, 1e-10, accuracy, max_rank)
     h1, h2 = ot.decompose_gate(gate, 1e-10)
     o1, o2, ol, oerr, oent = ot.simple_update(t1, t2, lam1, lam2, h1, h2, i1, i2, 1e-10, accuracy, max_rank)
