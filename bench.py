@@ -53,6 +53,7 @@ def main():
     ap.add_argument("--fixed-sweeps", type=int, default=10,
                     help="BP sweeps per regauge with the convergence test off (fixed work); 0 = converge to 1e-5")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="run the oracle CPU leg (1) or not (0)")
+    ap.add_argument("--plain-gpu", type=int, default=1, help="run the torch-eager GPU comparator (1) or not (0)")
     ap.add_argument("--gate-order", default="colored", choices=["colored", "pairs"],
                     help="coupling-gate order: pair list sorted by edge colour, or the generator's pair order")
     args = ap.parse_args()
@@ -157,9 +158,206 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if args.plain_gpu:
+        result["plain_gpu"] = plain_gpu_baseline(ann, ef, schedule[args.warmup + args.steps], args)
+        result["vs_plain_gpu"] = result["value"] / result["plain_gpu"]["value"]
+        print(f"bench: plain GPU comparator done: {result['plain_gpu']['ms_per_step']:.0f} ms/step", file=sys.stderr,
+              flush=True)
     if args.cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ann, ef, schedule[args.warmup + args.steps], args)
+        result["vs_cpu_baseline"] = result["value"] / result["cpu_baseline"]["value"]
     print(json.dumps(result))
+
+
+def plain_gpu_baseline(ann, ef, step_times, args, sample_sweeps=2, sample_gates=150):
+    """The same Trotter step written as plain torch eager ops on the GPU: one
+    node, edge or gate at a time (tensordot for contractions in complex64,
+    torch.linalg eigh / svd / qr in complex128), no batching, no fused kernels.
+    Started from the engine's state after the timed steps.  A bounded sample is
+    timed (the first Vidal gauge and distance, ``sample_sweeps`` BP sweeps, the
+    final gauge and distance, ``sample_gates`` simple updates, all one-site
+    gates) and the step time is extrapolated to the full work (fixed sweeps and
+    every gate).  Reports EU/s = EU per step / step time."""
+    import torch
+    from dequantizer_amd.annealing import coupling_gates, mixing_gate
+    emu = ann.emu
+    emu._flush()
+    topo, chi, eps = emu.topo, emu.chi, 1e-10
+    c64, c128 = torch.complex64, torch.complex128
+    dims = emu.dims.copy()
+    lam_all = emu.lam.detach().view(-1, chi)
+    lams = [lam_all[j, : int(dims[j])].clone() for j in range(topo.n_edges)]
+    adj = [[(int(topo.adj_edge[k]), int(topo.adj_side[k])) for k in range(topo.adj_ptr[i], topo.adj_ptr[i + 1])]
+           for i in range(topo.n_nodes)]
+    gam = []
+    for i in range(topo.n_nodes):
+        deg, off = int(topo.deg[i]), int(emu.h.offsets[i])
+        full = emu.gam.detach()[off: off + chi ** deg * 2].view(*((chi,) * deg), 2)
+        gam.append(full[tuple(slice(0, int(dims[e])) for e, _ in adj[i])].clone())
+
+    def bcast(v, axis, nd):
+        shape = [1] * nd
+        shape[axis] = v.shape[0]
+        return v.reshape(shape)
+
+    def sym(t, ls):
+        for a, lm in enumerate(ls):
+            t = t * bcast(torch.sqrt(lm * (lm > eps)).to(c64), a, t.ndim)
+        return t / torch.linalg.vector_norm(t)
+
+    def on_axis(t, m, a):
+        return torch.movedim(torch.tensordot(m, t, dims=([1], [a])), 0, a)
+
+    def node_update(t, msgs):
+        out = []
+        for idx in range(t.ndim - 1):
+            u = t
+            for a, m in enumerate(msgs):
+                if a != idx:
+                    u = on_axis(u, m, a)
+            others = [a for a in range(t.ndim) if a != idx]
+            g = torch.tensordot(t.conj(), u, dims=(others, others))
+            out.append(g / torch.trace(g))
+        return out
+
+    def edge_svd(mf, mb):
+        lf, uf = torch.linalg.eigh(mf.to(c128))
+        lb, ub = torch.linalg.eigh(mb.to(c128))
+        inv = lambda x: 1.0 / ((x < eps) * 3.4028234663852886e38 + x)
+        fn = lambda u, v: (u * v.to(c128)) @ u.conj().T
+        isf, isb = fn(uf, torch.sqrt(inv(lf))), fn(ub, torch.sqrt(inv(lb)))
+        sf, sb = fn(uf, torch.sqrt(lf * (lf > eps))), fn(ub, torch.sqrt(lb * (lb > eps)))
+        u, lm, vh = torch.linalg.svd(sf @ sb.T, full_matrices=False)
+        lm = lm / torch.linalg.vector_norm(lm)
+        lm = lm * (lm > eps)
+        mask = (lm > eps).to(c128)
+        u = u * mask
+        u = u / torch.linalg.vector_norm(u)
+        vh = vh * mask[:, None]
+        vh = vh / torch.linalg.vector_norm(vh)
+        return (isf @ u).to(c64), (isb @ vh.T).to(c64), lm
+
+    def absorb(t, xs):
+        for a, x in enumerate(xs):
+            t = on_axis(t, x.T, a)
+        return t / torch.linalg.vector_norm(t)
+
+    def vdist(t, ls):
+        total = 0.0
+        for idx in range(t.ndim - 1):
+            u = t
+            for a, lm in enumerate(ls):
+                if a != idx:
+                    u = u * bcast((lm ** 2).to(c64), a, t.ndim)
+            others = [a for a in range(t.ndim) if a != idx]
+            r = torch.tensordot(t.conj(), u, dims=(others, others)).to(c128)
+            r = r / torch.trace(r)
+            ideal = torch.diag((torch.diagonal(r).real > eps).to(c128))
+            total = total + torch.linalg.svdvals(r - ideal / torch.trace(ideal)).sum()
+        return total
+
+    def regauge_vg(tensors, msgs):
+        xs, new_l = {}, {}
+        for j in range(topo.n_edges):
+            xf, xb, lm = edge_svd(msgs[(j, 0)], msgs[(j, 1)])
+            xs[(j, 0)], xs[(j, 1)], new_l[j] = xf, xb, lm
+        new_t = [absorb(tensors[i], [xs[(e, s)] for e, s in adj[i]]) for i in range(topo.n_nodes)]
+        dist = sum(vdist(new_t[i], [new_l[e] for e, _ in adj[i]]) for i in range(topo.n_nodes))
+        return new_t, new_l, dist
+
+    def sweep(tensors, e2n):
+        n2e = {}
+        for i in range(topo.n_nodes):
+            outs = node_update(tensors[i], [e2n[(e, s)] for e, s in adj[i]])
+            for (e, s), m in zip(adj[i], outs):
+                n2e[(e, s)] = m
+        return {(e, s): n2e[(e, 1 - s)] for (e, s) in n2e}
+
+    def su(i_a, i_b, j, g):
+        mat = g.permute(0, 2, 1, 3).reshape(4, 4)
+        uu, ss, vvh = torch.linalg.svd(mat)
+        r = int((ss > eps).sum())
+        root = torch.sqrt(ss[:r])
+        h1 = (uu[:, :r] * root).reshape(2, 2, r).permute(2, 0, 1).to(c64)
+        h2 = (vvh[:r] * root[:, None]).reshape(r, 2, 2).to(c64)
+        sides = []
+        for node, h in ((i_a, h1), (i_b, h2)):
+            t = gam[node]
+            idx = [k for k, (e, _) in enumerate(adj[node]) if e == j][0]
+            for a, (e, _) in enumerate(adj[node]):
+                if a != idx:
+                    t = t * bcast(lams[e].to(c64), a, t.ndim)
+            th = torch.einsum("...i,qoi->...qo", t, h)
+            th = torch.movedim(th, -2, idx + 1)
+            shape = list(t.shape)
+            shape[idx] *= r
+            th = th.reshape(shape)
+            th = torch.movedim(th / torch.linalg.vector_norm(th), idx, -1)
+            q, rr = torch.linalg.qr(th.reshape(-1, th.shape[-1]).to(c128))
+            q = q.reshape(*th.shape[:-1], -1)
+            for a, (e, _) in enumerate([x for k, x in enumerate(adj[node]) if k != idx]):
+                lm = lams[e]
+                q = q * bcast((1.0 / ((lm < eps) * 3.4028234663852886e38 + lm)).to(c128), a, q.ndim)
+            sides.append((q, rr, idx))
+        (qa, ra, ia), (qb, rb, ib) = sides
+        lam_rep = lams[j].repeat_interleave(r)
+        u, sv, vh = torch.linalg.svd((ra * lam_rep.to(c128)) @ rb.T, full_matrices=False)
+        sv = sv * (sv > eps)
+        rank = min(args.chi, sv.shape[0])
+        gam[i_a] = torch.movedim(torch.tensordot(qa, u[:, :rank], dims=([qa.ndim - 1], [0])), -1, ia).to(c64)
+        gam[i_b] = torch.movedim(torch.tensordot(qb, vh[:rank], dims=([qb.ndim - 1], [1])), -1, ib).to(c64)
+        lams[j] = sv[:rank].to(torch.float64)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tensors = [sym(gam[i], [lams[e] for e, _ in adj[i]]) for i in range(topo.n_nodes)]
+    msgs = {(j, s): torch.diag(lams[j].to(c64)) for j in range(topo.n_edges) for s in range(2)}
+    regauge_vg(tensors, msgs)
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(sample_sweeps):
+        msgs = sweep(tensors, msgs)
+    torch.cuda.synchronize()
+    t_sweep = (time.perf_counter() - t0) / sample_sweeps
+    t0 = time.perf_counter()
+    new_t, new_l, _ = regauge_vg(tensors, {k: v for k, v in msgs.items()})
+    torch.cuda.synchronize()
+    t_last = time.perf_counter() - t0
+    gam[:] = new_t
+    for j in range(topo.n_edges):
+        lams[j] = new_l[j]
+    mix, cpl = step_times
+    gates = torch.from_numpy(coupling_gates(ef, cpl)).to("cuda")
+    edge_of = {}
+    for j in range(topo.n_edges):
+        a, b = (int(x) for x in topo.edge_node[j])
+        edge_of[(a, b)] = edge_of[(b, a)] = j
+    nidx = topo.node_index[0]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k, (a, b) in enumerate(ef.coupled_spin_pairs[:sample_gates]):
+        ia, ib = nidx[int(a)], nidx[int(b)]
+        su(ia, ib, edge_of[(ia, ib)], gates[k])
+    torch.cuda.synchronize()
+    t_gate = (time.perf_counter() - t0) / min(sample_gates, ef.edges_number)
+    mg = torch.from_numpy(mixing_gate(mix)).to("cuda", c64)
+    t0 = time.perf_counter()
+    for i in range(topo.n_nodes):
+        t = torch.tensordot(gam[i], mg, dims=([gam[i].ndim - 1], [1]))
+        gam[i] = t / torch.linalg.vector_norm(t)
+    torch.cuda.synchronize()
+    t_apply1 = time.perf_counter() - t0
+    sweeps = args.fixed_sweeps or 10
+    t_step = t_first + sweeps * t_sweep + t_last + ef.edges_number * t_gate + t_apply1
+    eu = sweeps * 2 * ef.edges_number
+    return {"value": eu / t_step, "unit": "edge-updates/s", "kind": "torch eager, one element at a time",
+            "ms_per_step": 1e3 * t_step, "gates_per_s": 1.0 / t_gate,
+            "sample": "timed: first Vidal gauge + distance %.2f s, %d BP sweeps at %.3f s each, final gauge + "
+                      "distance %.2f s, %d simple updates at %.2f ms each, %d one-site gates %.3f s; step "
+                      "extrapolated to %d sweeps and %d gates" % (
+                          t_first, sample_sweeps, t_sweep, t_last, min(sample_gates, ef.edges_number),
+                          1e3 * t_gate, topo.n_nodes, t_apply1, sweeps, ef.edges_number)}
 
 
 def cpu_baseline(ann, ef, step_times, args):
