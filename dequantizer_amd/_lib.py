@@ -48,6 +48,9 @@ SIGNATURES = {
     "dq_decompose_gates": (c_int, [vp, c_int, c_int, c_dbl, vp, vp, c_int, vp]),
     "dq_tensor_apply1": (c_int, [c_int, vp, c_ll, c_int, vp, vp, vp, vp]),
     "dq_tensor_apply_half": (c_int, [c_int, vp, c_ll, c_int, c_ll, c_int, vp, c_int, vp, vp, vp]),
+    "dq_su_tape_bytes": (c_ll, [vp, c_int]),
+    "dq_simple_update_tape": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, c_int, c_dbl, c_int, vp, vp, vp, vp]),
+    "dq_simple_update_vjp": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, c_int, c_dbl, vp, vp, vp, vp, vp]),
     "dq_rdm1": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, vp]),
     "dq_rdm2": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, vp]),
     "dq_update_active": (c_int, [vp, c_dbl, vp, c_int, vp, c_int, vp]),

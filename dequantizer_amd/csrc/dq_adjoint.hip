@@ -1,0 +1,617 @@
+// Hand-written adjoints (vector-Jacobian products) of the forward kernels,
+// complex128.  Gradient convention: for a complex value z the gradient is
+// zbar = dL/dRe z + i dL/dIm z, so dL = Re sum conj(zbar) dz; real values
+// carry dL/dx.
+//
+// Simple update (forward: dq_su.hip, reference tensor_ops.py:318-389).  With
+// Atil = apply_half(Gamma) (rows = other bonds x phys out, columns = (b, q)),
+// D = prod of the other lambdas per row, nu^2 = |D Atil|^2,
+// G = (D Atil)^H (D Atil) / nu^2 = R^H R (pivoted Cholesky), the core
+// C = R_a Lambda R_b^T = U S V^H, and K_a = Lambda R_b^T V_k S_k^-1,
+// K_b = Lambda R_a^T conj(U_k) S_k^-1, the outputs are
+//   Gamma_a' = mask_a (Atil_a / nu_a) K_a,  Gamma_b' = mask_b (Atil_b / nu_b) K_b,
+//   lambda' = S_k.
+// This equals the reference's QR + SVD formulation (Gamma_a' = Q_a U_k /
+// lambda_others); the adjoint below differentiates this closed form, reading
+// R_a, R_b, U, V, S and the pivots from the forward's tape so that it sees
+// the forward's singular-vector phases.
+#include <float.h>
+
+#include "dq_launch.hpp"
+#include "dq_linalg.hpp"
+
+namespace dq {
+
+namespace adj {
+
+// op codes of the small dense helpers
+constexpr int N = 0, T = 1, H = 2, C = 3;
+
+__device__ __forceinline__ cd opget(const cd* A, int lda, int op, int i, int j) {
+  switch (op) {
+    case T:
+      return A[(long long)j * lda + i];
+    case H:
+      return conj(A[(long long)j * lda + i]);
+    case C:
+      return conj(A[(long long)i * lda + j]);
+    default:
+      return A[(long long)i * lda + j];
+  }
+}
+
+// Cm (m x n, row-major, ldc) (+)= op(A) (m x k) op(B) (k x n).  Block-wide;
+// ends with a barrier.  Cm must not alias A or B.
+__device__ void bmm(cd* Cm, int ldc, const cd* A, int lda, int opA, const cd* B, int ldb, int opB, int m, int n, int k,
+                    bool accumulate) {
+  for (int f = threadIdx.x; f < m * n; f += blockDim.x) {
+    const int i = f / n, j = f % n;
+    cd acc = accumulate ? Cm[(long long)i * ldc + j] : mk<double>(0.0, 0.0);
+    for (int l = 0; l < k; ++l) fma_acc(acc, opget(A, lda, opA, i, l), opget(B, ldb, opB, l, j));
+    Cm[(long long)i * ldc + j] = acc;
+  }
+  __syncthreads();
+}
+
+// X (k x ncols, ldx) <- R^-1 X for upper-triangular R (k x k, ldr).  One
+// thread per column (back substitution).
+__device__ void solve_upper_left(const cd* R, int ldr, int k, cd* X, int ldx, int ncols) {
+  for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
+    for (int i = k - 1; i >= 0; --i) {
+      cd acc = X[(long long)i * ldx + c];
+      for (int m = i + 1; m < k; ++m) acc = acc - R[(long long)i * ldr + m] * X[(long long)m * ldx + c];
+      X[(long long)i * ldx + c] = cdiv(acc, R[(long long)i * ldr + i]);
+    }
+  }
+  __syncthreads();
+}
+
+// X (nrows x k, ldx) <- X R^-H for upper-triangular R: solve Y R^H = X row by
+// row, (Y R^H)_c = sum_{m >= c} Y_m conj(R[c][m]).
+__device__ void solve_upperH_right(const cd* R, int ldr, int k, cd* X, int ldx, int nrows) {
+  for (int r = threadIdx.x; r < nrows; r += blockDim.x) {
+    cd* x = X + (long long)r * ldx;
+    for (int c = k - 1; c >= 0; --c) {
+      cd acc = x[c];
+      for (int m = c + 1; m < k; ++m) acc = acc - x[m] * conj(R[(long long)c * ldr + m]);
+      x[c] = cdiv(acc, conj(R[(long long)c * ldr + c]));
+    }
+  }
+  __syncthreads();
+}
+
+// SVD adjoint of a square n x n C = U diag(s) V^H (U, V row-major, all n
+// columns; s descending, zeros allowed).  Given Ubar, Vbar (n x n) and sbar,
+// writes Cbar = U Abar V^H with
+//   Abar_ij = F_ij (s_j (J - J^H)_ij + s_i (K - K^H)_ij),  i != j,
+//   Abar_ii = sbar_i + i (Im J_ii - Im K_ii) / (2 s_i),
+// J = U^H Ubar, K = V^H Vbar, F_ij = 1 / (s_j^2 - s_i^2) (Lorentzian-broadened
+// at degeneracies, zero where both values vanish), plus the null-space term
+// for the zero columns U carries where s = 0.  w1..w3: n x n scratch.
+__device__ void svd_vjp(const cd* U, const cd* V, const double* s, const cd* Ubar, const cd* Vbar,
+                        const double* sbar, int n, cd* Cbar, cd* w1, cd* w2, cd* w3) {
+  bmm(w1, n, U, n, H, Ubar, n, N, n, n, n, false);  // J
+  bmm(w2, n, V, n, H, Vbar, n, N, n, n, n, false);  // K
+  double smax = 0.0;
+  for (int i = 0; i < n; ++i) smax = fmax(smax, s[i]);
+  const double broad = 1e-24 * smax * smax * smax * smax + 1e-300;
+  for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
+    const int i = f / n, j = f % n;
+    cd a = mk<double>(0.0, 0.0);
+    if (i != j) {
+      const double d = s[j] * s[j] - s[i] * s[i];
+      if (s[i] > 0.0 || s[j] > 0.0) {
+        const double F = d / (d * d + broad);
+        const cd jj = w1[f] - conj(w1[(long long)j * n + i]);
+        const cd kk = w2[f] - conj(w2[(long long)j * n + i]);
+        a = F * (s[j] * jj + s[i] * kk);
+      }
+    } else {
+      a = mk<double>(sbar[i], 0.0);
+      if (s[i] > 0.0) a.im = (w1[f].im - w2[f].im) / (2.0 * s[i]);
+    }
+    w3[f] = a;
+  }
+  __syncthreads();
+  bmm(w1, n, U, n, N, w3, n, N, n, n, n, false);       // U Abar
+  bmm(Cbar, n, w1, n, N, V, n, H, n, n, n, false);     // (U Abar) V^H
+  // U holds zero columns where s = 0 (no orthonormal completion), so the
+  // null-space part of the kept left vectors' adjoint is added explicitly:
+  // Cbar += (I - U U^H) Ubar S^-1 V^H  (V is a complete basis).
+  for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
+    const int j = f % n;
+    w1[f] = s[j] > 0.0 ? (1.0 / s[j]) * Ubar[f] : mk<double>(0.0, 0.0);
+  }
+  __syncthreads();
+  bmm(w2, n, U, n, H, w1, n, N, n, n, n, false);       // U^H X
+  bmm(w3, n, U, n, N, w2, n, N, n, n, n, false);       // U U^H X
+  for (int f = threadIdx.x; f < n * n; f += blockDim.x) w1[f] = w1[f] - w3[f];
+  __syncthreads();
+  bmm(Cbar, n, w1, n, N, V, n, H, n, n, n, true);
+}
+
+// Adjoint of the pivoted Cholesky factor R (n x n row-major, R = L^H P^T,
+// rows >= kc zero) of a Hermitian positive semi-definite G of rank kc:
+// Gbar (Hermitian) from Rbar.  In pivoted coordinates Rt = R P = [R11 R12; 0 0]
+// with G11 = R11^H R11, G12 = R11^H R12:
+//   M11' = triu(M11 - R12 M12^H R11^-H),  B = strict_upper(M11' R11^H) + diag(Re)/2,
+//   G11bar = R11^-1 herm(B) R11^-H,  G12bar = R11^-1 M12 (split over both blocks).
+// w1..w4: n x n scratch (w1 receives Rt, w2 the permuted Rbar).
+__device__ void chol_vjp(const cd* R, const cd* Rbar, const int* piv, int n, cd* Gbar, cd* w1, cd* w2, cd* w3,
+                         cd* w4, int* kc_sh) {
+  for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
+    const int i = f / n, m = f % n;
+    w1[f] = R[(long long)i * n + piv[m]];
+    w2[f] = m >= i ? Rbar[(long long)i * n + piv[m]] : mk<double>(0.0, 0.0);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int kc = 0;
+    while (kc < n && w1[(long long)kc * n + kc].re > 0.0) ++kc;
+    *kc_sh = kc;
+  }
+  __syncthreads();
+  const int kc = *kc_sh;
+  const int nr = n - kc;
+  // w3 = M11 (kc x kc, ld n), then M11' = M11 - (R12 M12^H) R11^-H
+  for (int f = threadIdx.x; f < kc * kc; f += blockDim.x) {
+    const int i = f / kc, m = f % kc;
+    w3[(long long)i * n + m] = w2[(long long)i * n + m];
+  }
+  __syncthreads();
+  if (nr > 0) {
+    bmm(w4, n, w1 + kc, n, N, w2 + kc, n, H, kc, kc, nr, false);  // R12 M12^H
+    solve_upperH_right(w1, n, kc, w4, n, kc);                      // (R12 M12^H) R11^-H
+    for (int f = threadIdx.x; f < kc * kc; f += blockDim.x) {
+      const int i = f / kc, m = f % kc;
+      w3[(long long)i * n + m] = w3[(long long)i * n + m] - w4[(long long)i * n + m];
+    }
+    __syncthreads();
+  }
+  for (int f = threadIdx.x; f < kc * kc; f += blockDim.x) {  // triu
+    const int i = f / kc, m = f % kc;
+    if (m < i) w3[(long long)i * n + m] = mk<double>(0.0, 0.0);
+  }
+  __syncthreads();
+  bmm(w4, n, w3, n, N, w1, n, H, kc, kc, kc, false);  // A = M11' R11^H
+  // herm(strict_upper(A) + diag(Re A)/2) into w3
+  for (int f = threadIdx.x; f < kc * kc; f += blockDim.x) {
+    const int i = f / kc, m = f % kc;
+    cd v;
+    if (i == m)
+      v = mk<double>(0.5 * w4[(long long)i * n + i].re, 0.0);
+    else if (m > i)
+      v = 0.5 * w4[(long long)i * n + m];
+    else
+      v = 0.5 * conj(w4[(long long)m * n + i]);
+    w3[(long long)i * n + m] = v;
+  }
+  __syncthreads();
+  solve_upper_left(w1, n, kc, w3, n, kc);     // R11^-1 herm
+  solve_upperH_right(w1, n, kc, w3, n, kc);   // ... R11^-H
+  // G12bar = R11^-1 M12 -> w2 columns kc.. (rows < kc)
+  if (nr > 0) solve_upper_left(w1, n, kc, w2 + kc, n, nr);
+  for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
+    const int i = f / n, m = f % n;
+    cd v = mk<double>(0.0, 0.0);
+    if (i < kc && m < kc)
+      v = w3[(long long)i * n + m];
+    else if (i < kc && m >= kc)
+      v = 0.5 * w2[(long long)i * n + m];
+    else if (i >= kc && m < kc)
+      v = 0.5 * conj(w2[(long long)m * n + i]);
+    Gbar[(long long)piv[i] * n + piv[m]] = v;
+  }
+  __syncthreads();
+}
+
+}  // namespace adj
+
+// Per-gate workspace layout (complex doubles unless noted).
+struct SuVjpWs {
+  cd* base;
+  int m_max, n, chi;
+  __device__ cd* Aa() const { return base; }
+  __device__ cd* Ab() const { return Aa() + (long long)m_max * n; }
+  __device__ cd* Ba() const { return Ab() + (long long)m_max * n; }  // Atil gradients
+  __device__ cd* Bb() const { return Ba() + (long long)m_max * n; }
+  __device__ cd* AE() const { return Bb() + (long long)m_max * n; }  // Atil Ebar
+  __device__ cd* Ya() const { return AE() + (long long)m_max * n; }
+  __device__ cd* Yb() const { return Ya() + (long long)m_max * chi; }
+  __device__ cd* core(int k) const { return Yb() + (long long)m_max * chi + (long long)k * n * n; }  // 16 slots
+  __device__ double* dbl() const { return (double*)core(16); }
+};
+
+__host__ __device__ inline long long su_vjp_ws_bytes(int chi, int phys, int max_r, int max_deg) {
+  const long long n = (long long)chi * max_r;
+  const long long m = ipow(chi, max_deg - 1) * phys;
+  const long long cds = 5 * m * n + 2 * m * chi + 16 * n * n;
+  const long long dbls = 8 * n + 4 * ipow(chi, max_deg - 1) + 64;
+  return (cds * 16 + dbls * 8 + 255) / 256 * 256;
+}
+
+__device__ __forceinline__ long long elem_off(const int* str, int d, int slot, int chi, int row, int b, int s) {
+  long long off = (long long)b * str[slot] + s;
+  for (int j = d - 1; j >= 0; --j) {
+    if (j == slot) continue;
+    const int q = row / chi, bj = row - q * chi;
+    row = q;
+    off += (long long)bj * str[j];
+  }
+  return off;
+}
+
+// Other-bond weight of a row: product of lambda (or of lam_mask when mask)
+// over the bonds j != slot, optionally leaving out bond `skip`.
+__device__ __forceinline__ double row_weight(const double* wl, int d, int slot, int chi, int row, int skip,
+                                             bool mask, double eps) {
+  double w = 1.0;
+  for (int j = d - 1; j >= 0; --j) {
+    if (j == slot) continue;
+    const int q = row / chi, bj = row - q * chi;
+    row = q;
+    if (j == skip) continue;
+    const double l = wl[j * chi + bj];
+    w *= mask ? (l < eps ? 0.0 : 1.0) : l;
+  }
+  return w;
+}
+
+__device__ __forceinline__ int row_bond(int d, int slot, int chi, int row, int j) {
+  for (int jj = d - 1; jj >= 0; --jj) {
+    if (jj == slot) continue;
+    const int q = row / chi, bj = row - q * chi;
+    if (jj == j) return bj;
+    row = q;
+  }
+  return 0;
+}
+
+__global__ void __launch_bounds__(256)
+    k_su_vjp(DevGraph g, const cd* Tpre, const double* lam, const SuGate* gates, const cd* halves, int max_r,
+             double eps, const char* tape, long long tape_bytes, cd* Tbar, double* lambar, cd* hbar, char* ws,
+             long long wsb) {
+  __shared__ double wl[2][kMaxDeg * 64];
+  __shared__ double sbar[128], lamb[128], le[64];
+  __shared__ double red[8];
+  __shared__ double scal[8];
+  __shared__ int kc_sh;
+  const SuGate G = gates[blockIdx.x];
+  if (G.rank < 1 || G.rank > max_r) return;
+  const int e = G.edge, rg = G.rank, chi = g.chi, p = g.phys;
+  const int n = chi * rg;
+  const int nd[2] = {g.edge_node[2 * e + G.first], g.edge_node[2 * e + 1 - G.first]};
+  const int sl[2] = {find_slot(g, nd[0], e, G.first), find_slot(g, nd[1], e, 1 - G.first)};
+  const long long hstride = (long long)p * p * p * p;
+  const cd* hh[2] = {halves + (long long)blockIdx.x * 2 * hstride, halves + (long long)blockIdx.x * 2 * hstride + hstride};
+  const SuTape tp((char*)tape + (long long)blockIdx.x * tape_bytes, chi * max_r);
+  const int kr = tp.meta()[0];
+  const double* sg = tp.sigma();
+  const cd* Ra = tp.Ra();
+  const cd* Rb = tp.Rb();
+  const cd* U = tp.U();
+  const cd* V = tp.V();
+  const double nu[2] = {sqrt(tp.tr()[0]), sqrt(tp.tr()[1])};
+  int maxdeg = g.deg[nd[0]] > g.deg[nd[1]] ? g.deg[nd[0]] : g.deg[nd[1]];
+  SuVjpWs W{(cd*)(ws + (long long)blockIdx.x * wsb), (int)(ipow(chi, maxdeg - 1) * p), n, chi};
+  for (int side = 0; side < 2; ++side) {
+    const int d = g.deg[nd[side]], base = g.adj_ptr[nd[side]];
+    for (int f = threadIdx.x; f < d * chi; f += blockDim.x)
+      wl[side][f] = lam[(long long)g.adj_edge[base + f / chi] * chi + f % chi];
+  }
+  for (int j = threadIdx.x; j < chi; j += blockDim.x) le[j] = lam[(long long)e * chi + j];
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    sbar[j] = (j < kr && sg[j] > 0.0) ? lambar[(long long)e * chi + j] : 0.0;
+    lamb[j] = 0.0;
+  }
+  __syncthreads();
+
+  cd* Ka = W.core(0);
+  cd* Kb = W.core(1);
+  cd* Kba = W.core(2);
+  cd* Kbb = W.core(3);
+  cd* Rab = W.core(4);
+  cd* Rbb = W.core(5);
+  cd* Ub = W.core(6);
+  cd* Vb = W.core(7);
+  cd* Cb = W.core(8);
+  cd* w1 = W.core(9);
+  cd* w2 = W.core(10);
+  cd* w3 = W.core(11);
+  cd* w4 = W.core(12);
+  cd* Gba = W.core(13);
+  cd* Gbb = W.core(14);
+  // K_a = Lambda R_b^T V_k S^-1,  K_b = Lambda R_a^T conj(U_k) S^-1  (n x chi, columns >= kr zero)
+  for (int f = threadIdx.x; f < n * chi; f += blockDim.x) {
+    const int c = f / chi, j = f % chi;
+    cd a = mk<double>(0.0, 0.0), b = a;
+    if (j < kr && sg[j] > 0.0) {
+      for (int i = 0; i < n; ++i) {
+        fma_acc(a, Rb[(long long)i * n + c], V[(long long)i * n + j]);
+        fma_acc(b, Ra[(long long)i * n + c], conj(U[(long long)i * n + j]));
+      }
+      const double f2 = le[c / rg] / sg[j];
+      a = f2 * a;
+      b = f2 * b;
+    }
+    Ka[f] = a;
+    Kb[f] = b;
+  }
+  __syncthreads();
+
+  // ---------------- per side: Atil, masked output gradient, K-bar, write-path Atil gradient
+  double nubar[2];
+  for (int side = 0; side < 2; ++side) {
+    const int nod = nd[side], slot = sl[side], d = g.deg[nod];
+    const int rows = (int)ipow(chi, d - 1), m = rows * p;
+    int str[kMaxDeg];
+    {
+      int mm = p;
+      for (int j = d - 1; j >= 0; --j) {
+        str[j] = mm;
+        mm *= chi;
+      }
+    }
+    const cd* t = Tpre + g.toff[nod];
+    const cd* tb = Tbar + g.toff[nod];
+    const cd* h = hh[side];
+    cd* A = side == 0 ? W.Aa() : W.Ab();
+    cd* Y = side == 0 ? W.Ya() : W.Yb();
+    cd* B = side == 0 ? W.Ba() : W.Bb();
+    const cd* K = side == 0 ? Ka : Kb;
+    cd* Kbar = side == 0 ? Kba : Kbb;
+    const double* w = wl[side];
+    for (int f = threadIdx.x; f < m * n; f += blockDim.x) {
+      const int R = f / n, c = f % n;
+      const int r = R / p, o = R % p, b = c / rg, q = c % rg;
+      cd acc = mk<double>(0.0, 0.0);
+      for (int s = 0; s < p; ++s) fma_acc(acc, h[((long long)q * p + o) * p + s], t[elem_off(str, d, slot, chi, r, b, s)]);
+      A[f] = acc;
+    }
+    for (int f = threadIdx.x; f < m * chi; f += blockDim.x) {
+      const int R = f / chi, j = f % chi;
+      const int r = R / p, o = R % p;
+      const double mk_r = row_weight(w, d, slot, chi, r, -1, true, eps);
+      Y[f] = j < kr ? mk_r * tb[elem_off(str, d, slot, chi, r, j, o)] : mk<double>(0.0, 0.0);
+    }
+    __syncthreads();
+    // Kbar = Atil^H Y / nu  (n x chi)
+    adj::bmm(Kbar, chi, A, n, adj::H, Y, chi, adj::N, n, chi, m, false);
+    for (int f = threadIdx.x; f < n * chi; f += blockDim.x) Kbar[f] = (1.0 / nu[side]) * Kbar[f];
+    __syncthreads();
+    // nubar = -Re<Kbar, K> / nu
+    double acc = 0.0;
+    for (int f = threadIdx.x; f < n * chi; f += blockDim.x)
+      acc += Kbar[f].re * K[f].re + Kbar[f].im * K[f].im;
+    acc = block_sum(acc, red);
+    nubar[side] = -acc / nu[side];
+    // write-path gradient of Atil: B = Y K^H / nu
+    adj::bmm(B, n, Y, chi, adj::N, K, chi, adj::H, m, n, chi, false);
+    for (int f = threadIdx.x; f < m * n; f += blockDim.x) B[f] = (1.0 / nu[side]) * B[f];
+    __syncthreads();
+  }
+
+  // ---------------- core adjoint
+  for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
+    Rab[f] = mk<double>(0.0, 0.0);
+    Rbb[f] = Rab[f];
+    Ub[f] = Rab[f];
+    Vb[f] = Rab[f];
+  }
+  __syncthreads();
+  for (int side = 0; side < 2; ++side) {
+    // side 0: K_a = Lambda R_b^T Wk, Wk = V_k S^-1 ; side 1: K_b = Lambda R_a^T Wk, Wk = conj(U_k) S^-1
+    const cd* Rx = side == 0 ? Rb : Ra;
+    const cd* Kbar = side == 0 ? Kba : Kbb;
+    cd* Rxbar = side == 0 ? Rbb : Rab;
+    // w1 = Wk (n x chi)
+    for (int f = threadIdx.x; f < n * chi; f += blockDim.x) {
+      const int i = f / chi, j = f % chi;
+      cd v = mk<double>(0.0, 0.0);
+      if (j < kr && sg[j] > 0.0) {
+        v = side == 0 ? V[(long long)i * n + j] : conj(U[(long long)i * n + j]);
+        v = (1.0 / sg[j]) * v;
+      }
+      w1[f] = v;
+    }
+    __syncthreads();
+    adj::bmm(w2, chi, Rx, n, adj::T, w1, chi, adj::N, n, chi, n, false);  // Rx^T Wk
+    // lambda-bar (per fused index c), R_x-bar, Wk-bar
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+      double a = 0.0;
+      for (int j = 0; j < chi; ++j) {
+        const cd kb = Kbar[(long long)c * chi + j], t2 = w2[(long long)c * chi + j];
+        a += kb.re * t2.re + kb.im * t2.im;
+      }
+      lamb[c] += a;
+    }
+    for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
+      const int i = f / n, c = f % n;
+      cd a = mk<double>(0.0, 0.0);
+      for (int j = 0; j < chi; ++j) fma_acc(a, Kbar[(long long)c * chi + j], conj(w1[(long long)i * chi + j]));
+      Rxbar[f] = Rxbar[f] + le[c / rg] * a;
+    }
+    __syncthreads();
+    // w3 = Wk-bar = conj(Rx) Lambda Kbar  (n x chi)
+    for (int f = threadIdx.x; f < n * chi; f += blockDim.x) {
+      const int i = f / chi, j = f % chi;
+      cd a = mk<double>(0.0, 0.0);
+      for (int c = 0; c < n; ++c) fma_acc(a, conj(Rx[(long long)i * n + c]), le[c / rg] * Kbar[(long long)c * chi + j]);
+      w3[f] = a;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < chi; j += blockDim.x) {
+      if (j < kr && sg[j] > 0.0) {
+        double a = 0.0;
+        for (int i = 0; i < n; ++i) {
+          const cd wb = w3[(long long)i * chi + j];
+          if (side == 0) {
+            const cd v = V[(long long)i * n + j];
+            a += v.re * wb.re + v.im * wb.im;  // Re conj(v) wb
+          } else {
+            const cd u = U[(long long)i * n + j];
+            a += u.re * wb.re - u.im * wb.im;  // Re u wb
+          }
+        }
+        sbar[j] += -a / (sg[j] * sg[j]);
+      }
+    }
+    for (int f = threadIdx.x; f < n * chi; f += blockDim.x) {
+      const int i = f / chi, j = f % chi;
+      if (j < kr && sg[j] > 0.0) {
+        const cd wb = (1.0 / sg[j]) * w3[f];
+        if (side == 0)
+          Vb[(long long)i * n + j] = Vb[(long long)i * n + j] + wb;
+        else
+          Ub[(long long)i * n + j] = Ub[(long long)i * n + j] + conj(wb);
+      }
+    }
+    __syncthreads();
+  }
+  // C-bar from the SVD adjoint
+  adj::svd_vjp(U, V, sg, Ub, Vb, sbar, n, Cb, w1, w2, w3);
+  // C = R_a Lambda R_b^T:  X = Cbar conj(R_b), Y = Cbar^T conj(R_a)
+  adj::bmm(w1, n, Cb, n, adj::N, Rb, n, adj::C, n, n, n, false);
+  adj::bmm(w2, n, Cb, n, adj::T, Ra, n, adj::C, n, n, n, false);
+  for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
+    const int c = f % n;
+    Rab[f] = Rab[f] + le[c / rg] * w1[f];
+    Rbb[f] = Rbb[f] + le[c / rg] * w2[f];
+  }
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    double a = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const cd x = w1[(long long)i * n + c], r = Ra[(long long)i * n + c];
+      a += x.re * r.re + x.im * r.im;
+    }
+    lamb[c] += a;
+  }
+  __syncthreads();
+  // Cholesky adjoints -> G-bar (Hermitian)
+  adj::chol_vjp(Ra, Rab, tp.piv_a(), n, Gba, w1, w2, w3, w4, &kc_sh);
+  adj::chol_vjp(Rb, Rbb, tp.piv_b(), n, Gbb, w1, w2, w3, w4, &kc_sh);
+
+  // ---------------- per side: environment Gram adjoint, half gate, lambda of other bonds
+  for (int side = 0; side < 2; ++side) {
+    const int nod = nd[side], slot = sl[side], d = g.deg[nod];
+    const int rows = (int)ipow(chi, d - 1), m = rows * p;
+    int str[kMaxDeg];
+    {
+      int mm = p;
+      for (int j = d - 1; j >= 0; --j) {
+        str[j] = mm;
+        mm *= chi;
+      }
+    }
+    const cd* R = side == 0 ? Ra : Rb;
+    cd* Gb = side == 0 ? Gba : Gbb;
+    const cd* A = side == 0 ? W.Aa() : W.Ab();
+    cd* B = side == 0 ? W.Ba() : W.Bb();
+    const double* w = wl[side];
+    const double tr = nu[side] * nu[side];
+    // G = R^H R (normalised Gram);  Ebar = (Gbar - Re<G, Gbar> I) / tr + nubar / (2 nu) I
+    adj::bmm(w1, n, R, n, adj::H, R, n, adj::N, n, n, n, false);
+    double acc = 0.0;
+    for (int f = threadIdx.x; f < n * n; f += blockDim.x) acc += w1[f].re * Gb[f].re + w1[f].im * Gb[f].im;
+    acc = block_sum(acc, red);
+    const double diag = -acc / tr + nubar[side] / (2.0 * nu[side]);
+    for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
+      const int i = f / n, j = f % n;
+      cd v = (1.0 / tr) * Gb[f];
+      if (i == j) v.re += diag;
+      w2[f] = v;
+    }
+    __syncthreads();
+    // AE = Atil Ebar;  B += 2 D^2 AE (Gram adjoint, D the row weight)
+    cd* AE = W.AE();
+    adj::bmm(AE, n, A, n, adj::N, w2, n, adj::N, m, n, n, false);
+    for (int f = threadIdx.x; f < m * n; f += blockDim.x) {
+      const double D = row_weight(w, d, slot, chi, (f / n) / p, -1, false, eps);
+      B[f] = B[f] + (2.0 * D * D) * AE[f];
+    }
+    __syncthreads();
+    // row weight gradient: rho_r = 2 D_r Re sum_{o,c} conj(AE[(r,o),c]) Atil[(r,o),c];
+    // lambda_j[b_j(r)] += rho_r * prod of the other weights of row r
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+      double rho = 0.0;
+      for (int o = 0; o < p; ++o) {
+        const long long Rr = (long long)r * p + o;
+        for (int c = 0; c < n; ++c) {
+          const cd ae = AE[Rr * n + c], a = A[Rr * n + c];
+          rho += ae.re * a.re + ae.im * a.im;
+        }
+      }
+      const double D = row_weight(w, d, slot, chi, r, -1, false, eps);
+      rho *= 2.0 * D;
+      if (rho != 0.0) {
+        for (int j = 0; j < d; ++j) {
+          if (j == slot) continue;
+          const int bj = row_bond(d, slot, chi, r, j);
+          const double wj = row_weight(w, d, slot, chi, r, j, false, eps);
+          const int ej = g.adj_edge[g.adj_ptr[nod] + j];
+          atomicAdd(&lambar[(long long)ej * chi + bj], rho * wj);
+        }
+      }
+    }
+    __syncthreads();
+    // Gamma-bar and h-bar from B = gradient of Atil
+    const cd* t = Tpre + g.toff[nod];
+    cd* tbw = Tbar + g.toff[nod];
+    const cd* h = hh[side];
+    const long long S = ipow(chi, d) * p;
+    for (long long f = threadIdx.x; f < S; f += blockDim.x) {
+      // decode f -> (row r, bond b at slot, phys s)
+      const int s = (int)(f % p);
+      long long rest = f / p;
+      int b = 0, r = 0, mul = 1;
+      for (int j = d - 1; j >= 0; --j) {
+        const int bj = (int)(rest % chi);
+        rest /= chi;
+        if (j == slot) {
+          b = bj;
+        } else {
+          r += bj * mul;
+          mul *= chi;
+        }
+      }
+      cd acc2 = mk<double>(0.0, 0.0);
+      for (int o = 0; o < p; ++o)
+        for (int q = 0; q < rg; ++q)
+          fma_acc(acc2, conj(h[((long long)q * p + o) * p + s]), B[(long long)(r * p + o) * n + b * rg + q]);
+      tbw[f] = acc2;
+    }
+    cd* hb = hbar + (long long)blockIdx.x * 2 * hstride + side * hstride;
+    for (int f = threadIdx.x; f < hstride; f += blockDim.x) {
+      const int q = f / (p * p), o = (f / p) % p, s = f % p;
+      cd acc2 = mk<double>(0.0, 0.0);
+      if (q < rg)
+        for (int r = 0; r < rows; ++r)
+          for (int b = 0; b < chi; ++b)
+            fma_acc(acc2, B[(long long)(r * p + o) * n + b * rg + q], conj(t[elem_off(str, d, slot, chi, r, b, s)]));
+      hb[f] = acc2;
+    }
+    __syncthreads();
+  }
+  // shared-edge lambda gradient (replaces the consumed output gradient)
+  for (int b = threadIdx.x; b < chi; b += blockDim.x) {
+    double a = 0.0;
+    for (int q = 0; q < rg; ++q) a += lamb[b * rg + q];
+    lambar[(long long)e * chi + b] = a;
+  }
+}
+
+hipError_t launch_su_vjp(const DevGraph& g, const void* Tpre, const double* lam, const SuGate* gates, int count,
+                         const cd* halves, int max_gate_rank, double eps, const void* tape, void* Tbar,
+                         double* lambar, void* hbar, void* ws, long long wsb, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_su_vjp, dim3(count), dim3(256), 0, s, g, (const cd*)Tpre, lam, gates, halves, max_gate_rank,
+                     eps, (const char*)tape, su_tape_bytes(g.chi, max_gate_rank), (cd*)Tbar, lambar, (cd*)hbar,
+                     (char*)ws, wsb);
+  return hipGetLastError();
+}
+
+long long su_vjp_ws_bytes_host(int chi, int phys, int max_r, int max_deg) {
+  return su_vjp_ws_bytes(chi, phys, max_r, max_deg);
+}
+
+}  // namespace dq

@@ -359,6 +359,28 @@ int dq_apply1(dq_graph* g, int dtype, void* T, const int* nodes, int count, cons
   return DQ_OK;
 }
 
+int dq_simple_update_vjp(dq_graph* g, int dtype, const void* T_pre, const double* lam_pre, const int* gates,
+                         int count, const void* halves, int max_gate_rank, double eps, const void* tape, void* T_bar,
+                         double* lam_bar, void* halves_bar, void* stream) {
+  if (!g || count < 0) return fail(DQ_ERR_INVALID, "bad argument");
+  if (dtype != DQ_C128) return fail(DQ_ERR_UNSUPPORTED, "the adjoint runs in complex128 only");
+  if (max_gate_rank < 1 || max_gate_rank > g->dev.phys * g->dev.phys)
+    return fail(DQ_ERR_INVALID, "max_gate_rank %d outside [1, phys^2]", max_gate_rank);
+  if (count == 0) return DQ_OK;
+  if (!T_pre || !lam_pre || !gates || !halves || !tape || !T_bar || !lam_bar || !halves_bar)
+    return fail(DQ_ERR_INVALID, "NULL argument");
+  const int chi = g->dev.chi, p = g->dev.phys;
+  if (chi * p > 64 || chi * p * p > 128)
+    return fail(DQ_ERR_UNSUPPORTED, "simple update supports chi*phys <= 64 and chi*phys^2 <= 128 (chi=%d, phys=%d)",
+                chi, p);
+  const long long wsb = su_vjp_ws_bytes_host(chi, p, max_gate_rank, g->max_deg);
+  int rc;
+  if ((rc = ws_ensure(g, (size_t)wsb * count)) != DQ_OK) return rc;
+  RET_LAUNCH(launch_su_vjp(g->dev, T_pre, lam_pre, (const SuGate*)gates, count, (const cd*)halves, max_gate_rank, eps,
+                           tape, T_bar, lam_bar, halves_bar, g->ws, wsb, S(stream)));
+  return DQ_OK;
+}
+
 int dq_decompose_gates(const void* gates, int count, int phys, double eps, void* halves, int* ranks, int rank_stride,
                        void* stream) {
   if (count < 0 || rank_stride < 1) return fail(DQ_ERR_INVALID, "bad argument");
@@ -389,8 +411,9 @@ int dq_tensor_apply_half(int dtype, const void* t, long long lead, int bond, lon
   return DQ_OK;
 }
 
-int dq_simple_update(dq_graph* g, int dtype, void* T, double* lam, const int* gates, int count, const void* halves,
-                     int max_gate_rank, double eps, int max_rank, double* err, double* entropy, void* stream) {
+static int simple_update_impl(dq_graph* g, int dtype, void* T, double* lam, const int* gates, int count,
+                              const void* halves, int max_gate_rank, double eps, int max_rank, double* err,
+                              double* entropy, void* tape, void* stream) {
   int rb, rc;
   if (!g || count < 0 || max_rank < 1) return fail(DQ_ERR_INVALID, "bad argument");
   if (max_gate_rank < 1 || max_gate_rank > g->dev.phys * g->dev.phys)
@@ -405,8 +428,26 @@ int dq_simple_update(dq_graph* g, int dtype, void* T, double* lam, const int* ga
   const long long wsb = lds ? 256 : su_ws_bytes(chi, p, max_gate_rank);
   if ((rc = ws_ensure(g, (size_t)wsb * count)) != DQ_OK) return rc;
   RET_LAUNCH(launch_su(rb, g->dev, T, lam, (const SuGate*)gates, count, (const cd*)halves, max_gate_rank, eps,
-                       max_rank, err, entropy, g->ws, wsb, S(stream)));
+                       max_rank, err, entropy, g->ws, wsb, tape, S(stream)));
   return DQ_OK;
+}
+
+int dq_simple_update(dq_graph* g, int dtype, void* T, double* lam, const int* gates, int count, const void* halves,
+                     int max_gate_rank, double eps, int max_rank, double* err, double* entropy, void* stream) {
+  return simple_update_impl(g, dtype, T, lam, gates, count, halves, max_gate_rank, eps, max_rank, err, entropy,
+                            nullptr, stream);
+}
+
+long long dq_su_tape_bytes(const dq_graph* g, int max_gate_rank) {
+  return g ? su_tape_bytes(g->dev.chi, max_gate_rank) : -1;
+}
+
+int dq_simple_update_tape(dq_graph* g, int dtype, void* T, double* lam, const int* gates, int count,
+                          const void* halves, int max_gate_rank, double eps, int max_rank, double* err,
+                          double* entropy, void* tape, void* stream) {
+  if (!tape && count > 0) return fail(DQ_ERR_INVALID, "dq_simple_update_tape: NULL tape");
+  return simple_update_impl(g, dtype, T, lam, gates, count, halves, max_gate_rank, eps, max_rank, err, entropy, tape,
+                            stream);
 }
 
 int dq_rdm1(dq_graph* g, int dtype, const void* T, const void* e2n, const int* nodes, int count, void* rho,

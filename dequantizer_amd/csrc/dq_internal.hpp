@@ -111,6 +111,14 @@ __device__ inline double block_sum(double v, double* red) {
   return s;
 }
 
+// Adjacency slot of edge e (node on side `side`) in `node`'s bond order.
+__device__ inline int find_slot(const DevGraph& g, int node, int e, int side) {
+  const int b = g.adj_ptr[node], d = g.deg[node];
+  for (int k = 0; k < d; ++k)
+    if (g.adj_edge[b + k] == e && g.adj_side[b + k] == side) return k;
+  return -1;
+}
+
 // ----------------------------------------------------- generic tensor helpers
 // dst = src with matrix M applied on axis j of a tensor with `d` bonds of
 // size chi and a trailing phys axis: dst[o, a, in] = sum_b M[a*chi+b] src[o, b, in]

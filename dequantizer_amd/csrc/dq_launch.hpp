@@ -86,10 +86,31 @@ struct SuGate {
   int rank;   // operator-Schmidt rank r of the gate
   int pad;
 };
+// Per-gate record of the simple update for the adjoint (tape != NULL):
+// R_a, R_b, U, V (n x n row-major, n = chi * r, slots sized for
+// nmax = chi * max_gate_rank), singular values, traces of both
+// environments, Cholesky pivots, kept rank.
+struct SuTape {
+  char* base;
+  int nmax;
+  __host__ __device__ SuTape(char* b, int nm) : base(b), nmax(nm) {}
+  __host__ __device__ cd* Ra() const { return (cd*)base; }
+  __host__ __device__ cd* Rb() const { return Ra() + (long long)nmax * nmax; }
+  __host__ __device__ cd* U() const { return Rb() + (long long)nmax * nmax; }
+  __host__ __device__ cd* V() const { return U() + (long long)nmax * nmax; }
+  __host__ __device__ double* sigma() const { return (double*)(V() + (long long)nmax * nmax); }
+  __host__ __device__ double* tr() const { return sigma() + nmax; }
+  __host__ __device__ int* piv_a() const { return (int*)(tr() + 2); }
+  __host__ __device__ int* piv_b() const { return piv_a() + nmax; }
+  __host__ __device__ int* meta() const { return piv_b() + nmax; }  // kept rank, n
+};
+__host__ __device__ inline long long su_tape_bytes(int chi, int max_r) {
+  const long long nm = (long long)chi * max_r;
+  return (4 * nm * nm * 16 + nm * 8 + 16 + 2 * nm * 4 + 16 + 255) / 256 * 256;
+}
 hipError_t launch_su(int real_bytes, const DevGraph& g, void* T, double* lam, const SuGate* gates, int count,
-                     const cd* halves /* per gate: 2 halves of phys^4 (r <= phys^2, each (r, phys, phys)) */,
-                     int max_gate_rank, double eps, int max_rank, double* err,
-                     double* entropy, void* ws, long long ws_bytes_per_block, hipStream_t s);
+                     const cd* halves, int max_gate_rank, double eps, int max_rank, double* err, double* entropy,
+                     void* ws, long long wsb, void* tape, hipStream_t s);
 
 // Per-block workspace of the simple update (bytes) and whether the small
 // FP64 matrices fit the LDS variant.
@@ -119,6 +140,15 @@ hipError_t launch_tensor_gate(int real_bytes, const void* t, long long lead, int
 // [count][2][p^4], ranks[i * rank_stride].  phys 1..4.
 hipError_t launch_decompose_gates(const cd* gates, int count, int p, double eps, cd* halves, int* ranks,
                                   int rank_stride, hipStream_t s);
+
+// Adjoint of one simple-update launch (complex128, dq_adjoint.hip).  Tpre,
+// lam: state before the launch; tape: the forward's record; Tbar, lambar:
+// gradients of the state after the launch, turned in place into those of
+// the state before it; hbar: per-gate gradients of the two halves.
+hipError_t launch_su_vjp(const DevGraph& g, const void* Tpre, const double* lam, const SuGate* gates, int count,
+                         const cd* halves, int max_gate_rank, double eps, const void* tape, void* Tbar,
+                         double* lambar, void* hbar, void* ws, long long wsb, hipStream_t s);
+long long su_vjp_ws_bytes_host(int chi, int phys, int max_r, int max_deg);
 
 // Per-instance convergence bookkeeping of the BP-to-Vidal loop.
 hipError_t launch_update_active(const double* dist, double acc, int* iters, int max_iters, int* active, int n_inst,

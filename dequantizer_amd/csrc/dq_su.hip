@@ -44,12 +44,6 @@ __device__ __forceinline__ double su_safe_inverse(double l, double eps) {
 }
 
 // Slot of edge e (at side `side`) in the adjacency of `node`.
-__device__ inline int find_slot(const DevGraph& g, int node, int e, int side) {
-  const int b = g.adj_ptr[node], d = g.deg[node];
-  for (int k = 0; k < d; ++k)
-    if (g.adj_edge[b + k] == e && g.adj_side[b + k] == side) return k;
-  return -1;
-}
 
 // ---------------------------------------------------------------------------
 // Simple update, one block per gate.  The reference factorises each side
@@ -440,7 +434,7 @@ constexpr int kSuLdsN = 32;  // LDS variant: n = chi*r <= 32 and chi*phys <= 32
 template <typename R, bool kLds>
 __global__ void __launch_bounds__(256)
     k_su(DevGraph g, cx<R>* T, double* lam, const SuGate* gates, const cd* halves, int max_r, double eps,
-         int max_rank, double* err_out, double* ent_out, char* ws, long long wsb) {
+         int max_rank, double* err_out, double* ent_out, char* ws, long long wsb, char* tape, long long tape_bytes) {
   __shared__ cd sm[kLds ? (6 * kSuLdsN * kSuLdsN + kSuChunk * kSuLdsN + 64) : 1];
   __shared__ double red[8];
   __shared__ double rot[4 * 64 + 8];
@@ -496,12 +490,21 @@ __global__ void __launch_bounds__(256)
     su_environment<R>(g, ta, na, ia, wla, E, chunk, kSuChunk);
   const double tra = su_gram(E, h1, chi, p, r, Gm, red);
   pivoted_cholesky(Gm, n, Ra, perm, rot);
+  SuTape tp(tape ? tape + (long long)blockIdx.x * tape_bytes : nullptr, chi * max_r);
+  if (tape) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tp.piv_a()[i] = perm[i];
+    __syncthreads();
+  }
   if (mfma_b)
     su_environment_mfma<R>(g, tb, nb, ib, wlb, E);
   else
     su_environment<R>(g, tb, nb, ib, wlb, E, chunk, kSuChunk);
   const double trb = su_gram(E, h2, chi, p, r, Gm, red);
   pivoted_cholesky(Gm, n, Rb, perm, rot);
+  if (tape) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tp.piv_b()[i] = perm[i];
+    __syncthreads();
+  }
   // core = R_a diag(lambda_rep) R_b^T, column-major
   __shared__ double le[128];
   for (int j = threadIdx.x; j < chi; j += blockDim.x) le[j] = lam[(long long)e * chi + j];
@@ -545,6 +548,24 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   const int rank = sh_rank;
+  if (tape) {
+    // factors the adjoint needs: R_a, R_b (row-major), sorted U and V
+    // (row-major), thresholded singular values, traces, kept rank
+    for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
+      const int i = f / n, j = f % n;
+      tp.Ra()[f] = Ra[f];
+      tp.Rb()[f] = Rb[f];
+      tp.U()[f] = U[(long long)j * n + i];
+      tp.V()[f] = Vs[(long long)perm[j] * n + i];
+    }
+    for (int j = threadIdx.x; j < n; j += blockDim.x) tp.sigma()[j] = sigma[j];
+    if (threadIdx.x == 0) {
+      tp.tr()[0] = tra;
+      tp.tr()[1] = trb;
+      tp.meta()[0] = rank;
+      tp.meta()[1] = n;
+    }
+  }
   // W_a = Lambda R_b^T V Sigma^-1 -> Z_a;  W_b = Lambda R_a^T conj(U) Sigma^-1 -> Z_b
   for (int side = 0; side < 2; ++side) {
     cd* W = Gm;  // n x chi (the core is no longer needed)
@@ -601,23 +622,25 @@ hipError_t launch_apply1(int rb, const DevGraph& g, void* T, const int* nodes, i
 
 hipError_t launch_su(int rb, const DevGraph& g, void* T, double* lam, const SuGate* gates, int count,
                      const cd* halves, int max_gate_rank, double eps, int max_rank, double* err, double* entropy,
-                     void* ws, long long wsb, hipStream_t s) {
+                     void* ws, long long wsb, void* tape, hipStream_t s) {
   if (count <= 0) return hipSuccess;
   const bool lds = su_lds_fits(g.chi, g.phys, max_gate_rank);
+  const long long tb = su_tape_bytes(g.chi, max_gate_rank);
+  char* tp = (char*)tape;
   if (rb == 4) {
     if (lds)
       hipLaunchKernelGGL((k_su<float, true>), dim3(count), dim3(256), 0, s, g, (cx<float>*)T, lam, gates, halves,
-                         max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb);
+                         max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb, tp, tb);
     else
       hipLaunchKernelGGL((k_su<float, false>), dim3(count), dim3(256), 0, s, g, (cx<float>*)T, lam, gates, halves,
-                         max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb);
+                         max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb, tp, tb);
   } else {
     if (lds)
       hipLaunchKernelGGL((k_su<double, true>), dim3(count), dim3(256), 0, s, g, (cx<double>*)T, lam, gates, halves,
-                         max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb);
+                         max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb, tp, tb);
     else
       hipLaunchKernelGGL((k_su<double, false>), dim3(count), dim3(256), 0, s, g, (cx<double>*)T, lam, gates, halves,
-                         max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb);
+                         max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb, tp, tb);
   }
   return hipGetLastError();
 }

@@ -159,6 +159,30 @@ int dq_tensor_apply1(int dtype, const void* t, long long rows, int phys, const v
 int dq_tensor_apply_half(int dtype, const void* t, long long lead, int bond, long long trail, int phys,
                          const void* half, int rank, void* out, double* ws, void* stream);
 
+/* dq_simple_update with a record for the adjoint: tape is a device buffer
+ * of count * dq_su_tape_bytes(g, max_gate_rank) bytes that receives, per
+ * gate, the Cholesky factors and pivots of both environments, the core's
+ * singular vectors and values, the traces and the kept rank.  Same results
+ * as dq_simple_update.  No reference counterpart (the reference has no
+ * gradients). */
+long long dq_su_tape_bytes(const dq_graph* g, int max_gate_rank);
+int dq_simple_update_tape(dq_graph* g, int dtype, void* tensors, double* lam, const int* gates, int count,
+                          const void* halves, int max_gate_rank, double eps, int max_rank, double* err,
+                          double* entropy, void* tape, void* stream);
+
+/* Adjoint of one dq_simple_update_tape launch (DQ_C128 only).  T_pre, lam_pre:
+ * tensors and lambdas before the launch; tape: that launch's record; T_bar,
+ * lam_bar (device, same layouts as tensors / lambdas): gradients with
+ * respect to the state after the launch on entry, with respect to the state
+ * before it on return (only the gates' nodes and bonds change); halves_bar
+ * (device complex128 [count][2][phys^4]): gradients of the gate halves.
+ * Gradients follow dL = Re sum conj(zbar) dz.  The adjoint of the
+ * reference's simple_update (dequantizer/src/tensor_ops.py:318-389), which
+ * has none. */
+int dq_simple_update_vjp(dq_graph* g, int dtype, const void* T_pre, const double* lam_pre, const int* gates,
+                         int count, const void* halves, int max_gate_rank, double eps, const void* tape, void* T_bar,
+                         double* lam_bar, void* halves_bar, void* stream);
+
 /* One-site density matrices (device complex128 rho[count][phys][phys]) from
  * tensors and edge->node messages.  Replaces get_one_side_density_matrix
  * (dequantizer/src/mappings/density_matrix.py:16-37). */
