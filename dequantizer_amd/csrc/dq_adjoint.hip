@@ -205,6 +205,59 @@ __device__ void chol_vjp(const cd* R, const cd* Rbar, const int* piv, int n, cd*
   __syncthreads();
 }
 
+// dst[.., a@axis, ..] (+)= sum_b op(M)[a][b] src[.., b@axis, ..] for a tensor
+// with d bonds of size chi and a trailing phys axis; M chi x chi row-major,
+// op 0: M[a][b], 1: conj(M[a][b]), 2: M[b][a], 3: conj(M[b][a]).  dst must
+// not alias src.  Ends with a barrier.
+__device__ void mprod(cd* dst, const cd* src, const cd* M, int chi, int d, int phys, int axis, int op, bool accumulate) {
+  const long long inner = ipow(chi, d - 1 - axis) * phys;
+  const long long total = ipow(chi, d) * phys;
+  for (long long f = threadIdx.x; f < total; f += blockDim.x) {
+    const long long in = f % inner;
+    const long long t = f / inner;
+    const int a = (int)(t % chi);
+    const long long o = t / chi;
+    const cd* sp = src + o * chi * inner + in;
+    cd v = accumulate ? dst[f] : mk<double>(0.0, 0.0);
+    for (int b = 0; b < chi; ++b) {
+      cd m;
+      switch (op) {
+        case 1:
+          m = conj(M[(long long)a * chi + b]);
+          break;
+        case 2:
+          m = M[(long long)b * chi + a];
+          break;
+        case 3:
+          m = conj(M[(long long)b * chi + a]);
+          break;
+        default:
+          m = M[(long long)a * chi + b];
+      }
+      fma_acc(v, m, sp[(long long)b * inner]);
+    }
+    dst[f] = v;
+  }
+  __syncthreads();
+}
+
+// Normalisation adjoint for O = B / tr(B) (complex trace):
+// Bbar = Obar / conj(t) - conj(<Obar, B> / t^2) I.  n x n, block-wide.
+__device__ void trace_norm_vjp(const cd* B, const cd* Ob, int n, cd* Bb) {
+  cd tr = mk<double>(0.0, 0.0), ip = tr;
+  for (int k = 0; k < n; ++k) tr = tr + B[(long long)k * n + k];
+  for (int q = 0; q < n * n; ++q) fma_cacc(ip, Ob[q], B[q]);
+  const cd t2 = tr * tr;
+  const cd c = conj(cdiv(ip, t2));
+  const cd ct = conj(tr);
+  for (int q = threadIdx.x; q < n * n; q += blockDim.x) {
+    cd v = cdiv(Ob[q], ct);
+    if (q / n == q % n) v = v - c;
+    Bb[q] = v;
+  }
+  __syncthreads();
+}
+
 }  // namespace adj
 
 // Per-gate workspace layout (complex doubles unless noted).
@@ -612,6 +665,666 @@ hipError_t launch_su_vjp(const DevGraph& g, const void* Tpre, const double* lam,
 
 long long su_vjp_ws_bytes_host(int chi, int phys, int max_r, int max_deg) {
   return su_vjp_ws_bytes(chi, phys, max_r, max_deg);
+}
+
+// ---------------------------------------------------------------------------
+// BP node update adjoint (one synchronous sweep, tensor_ops.py:36-60):
+// for every node and bond idx, out = B / tr B with B = Gram(conj T, U_idx),
+// U_idx = T with the incoming messages on the other bonds.  Given obar
+// (gradient of the sweep's node->edge outputs) writes ebar (gradient of the
+// sweep's edge->node inputs) and accumulates Tbar.
+__global__ void __launch_bounds__(256)
+    k_bp_node_vjp(DevGraph g, const cd* T, const cd* e2n, const cd* obar, cd* ebar, cd* Tbar, char* ws,
+                  long long wsb) {
+  const int chi = g.chi, p = g.phys;
+  const long long cc = (long long)chi * chi;
+  for (int node = blockIdx.x; node < g.n_nodes; node += gridDim.x) {
+    const int d = g.deg[node], base = g.adj_ptr[node];
+    const long long S = ipow(chi, d) * p;
+    cd* W = (cd*)(ws + (long long)blockIdx.x * wsb);
+    cd* Ub = W + (long long)(d > 1 ? d - 1 : 1) * S;
+    cd* tmp = Ub + S;
+    cd* B = tmp + S;
+    cd* Bb = B + cc;
+    cd* G2 = Bb + cc;
+    cd* Mbar = G2 + cc;
+    const cd* t = T + g.toff[node];
+    cd* tb = Tbar + g.toff[node];
+    for (long long q = threadIdx.x; q < d * cc; q += blockDim.x) Mbar[q] = mk<double>(0.0, 0.0);
+    __syncthreads();
+    for (int idx = 0; idx < d; ++idx) {
+      const cd* prev = t;
+      int ord[kMaxDeg];
+      int m = 0;
+      for (int j = 0; j < d; ++j) {
+        if (j == idx) continue;
+        const cd* Mj = e2n + (2LL * g.adj_edge[base + j] + g.adj_side[base + j]) * cc;
+        adj::mprod(W + (long long)m * S, prev, Mj, chi, d, p, j, 0, false);
+        prev = W + (long long)m * S;
+        ord[m++] = j;
+      }
+      const cd* U = prev;
+      bond_gram<double>(B, t, U, chi, d, p, idx, nullptr);
+      __syncthreads();
+      adj::trace_norm_vjp(B, obar + (2LL * g.adj_edge[base + idx] + g.adj_side[base + idx]) * cc, chi, Bb);
+      adj::mprod(tb, U, Bb, chi, d, p, idx, 1, true);  // Tbar += conj(Bb) U  (bra side)
+      cd* ub = Ub;
+      cd* tp = tmp;
+      adj::mprod(ub, t, Bb, chi, d, p, idx, 2, false);  // Ubar = Bb^T T
+      for (int k = m - 1; k >= 0; --k) {
+        const int j = ord[k];
+        const cd* Wp = k == 0 ? t : W + (long long)(k - 1) * S;
+        const cd* Mj = e2n + (2LL * g.adj_edge[base + j] + g.adj_side[base + j]) * cc;
+        bond_gram<double>(G2, ub, Wp, chi, d, p, j, nullptr);
+        __syncthreads();
+        for (long long q = threadIdx.x; q < cc; q += blockDim.x) Mbar[j * cc + q] = Mbar[j * cc + q] + conj(G2[q]);
+        adj::mprod(tp, ub, Mj, chi, d, p, j, 3, false);  // Ubar <- M^H Ubar on bond j
+        cd* sw = ub;
+        ub = tp;
+        tp = sw;
+      }
+      for (long long f = threadIdx.x; f < S; f += blockDim.x) tb[f] = tb[f] + ub[f];
+      __syncthreads();
+    }
+    for (long long q = threadIdx.x; q < d * cc; q += blockDim.x) {
+      const int j = (int)(q / cc);
+      ebar[(2LL * g.adj_edge[base + j] + g.adj_side[base + j]) * cc + q % cc] = Mbar[q];
+    }
+    __syncthreads();
+  }
+}
+
+// Absorb adjoint: Gamma = Y / |Y|, Y = T x_i X_i^T on every bond
+// (tensor_ops.py:88-93).  Accumulates Tbar, writes Xbar per (node, slot).
+__global__ void __launch_bounds__(256)
+    k_absorb_vjp(DevGraph g, const cd* T, const cd* X, const cd* gbar, cd* Tbar, cd* Xbar, char* ws, long long wsb) {
+  __shared__ double red[8];
+  const int chi = g.chi, p = g.phys;
+  const long long cc = (long long)chi * chi;
+  for (int node = blockIdx.x; node < g.n_nodes; node += gridDim.x) {
+    const int d = g.deg[node], base = g.adj_ptr[node];
+    const long long S = ipow(chi, d) * p;
+    cd* W = (cd*)(ws + (long long)blockIdx.x * wsb);  // W_1..W_d
+    cd* Yb = W + (long long)d * S;
+    cd* tmp = Yb + S;
+    cd* G2 = tmp + S;
+    const cd* t = T + g.toff[node];
+    const cd* prev = t;
+    for (int i = 0; i < d; ++i) {
+      const cd* Xi = X + (2LL * g.adj_edge[base + i] + g.adj_side[base + i]) * cc;
+      adj::mprod(W + (long long)i * S, prev, Xi, chi, d, p, i, 2, false);
+      prev = W + (long long)i * S;
+    }
+    const cd* Y = prev;
+    double nn = 0.0, ip = 0.0;
+    const cd* gb = gbar + g.toff[node];
+    for (long long f = threadIdx.x; f < S; f += blockDim.x) {
+      nn += norm2(Y[f]);
+      ip += Y[f].re * gb[f].re + Y[f].im * gb[f].im;
+    }
+    nn = sqrt(block_sum(nn, red));
+    ip = block_sum(ip, red) / nn;  // Re<Gamma, Gamma-bar>
+    for (long long f = threadIdx.x; f < S; f += blockDim.x) Yb[f] = (1.0 / nn) * (gb[f] - (ip / nn) * Y[f]);
+    __syncthreads();
+    cd* yb = Yb;
+    cd* tp = tmp;
+    for (int i = d - 1; i >= 0; --i) {
+      const cd* Wp = i == 0 ? t : W + (long long)(i - 1) * S;
+      const long long slot = 2LL * g.adj_edge[base + i] + g.adj_side[base + i];
+      const cd* Xi = X + slot * cc;
+      bond_gram<double>(G2, yb, Wp, chi, d, p, i, nullptr);
+      __syncthreads();
+      for (long long q = threadIdx.x; q < cc; q += blockDim.x) {
+        const int a = (int)(q / chi), b = (int)(q % chi);
+        Xbar[slot * cc + (long long)b * chi + a] = conj(G2[q]);
+      }
+      adj::mprod(tp, yb, Xi, chi, d, p, i, 1, false);  // sum_a conj(X[b][a]) Ybar[a]
+      cd* sw = yb;
+      yb = tp;
+      tp = sw;
+    }
+    cd* tb = Tbar + g.toff[node];
+    for (long long f = threadIdx.x; f < S; f += blockDim.x) tb[f] = tb[f] + yb[f];
+    __syncthreads();
+  }
+}
+
+// Matrix-function divided differences for the safe square root
+// f(w) = sqrt(w) (w > eps, else 0) and the safe inverse square root
+// g(w) = 1 / sqrt(w) (w >= eps, else 1 / sqrt(FLT_MAX + w)).
+__device__ __forceinline__ double dd_sqrt(double a, double b, double eps) {
+  const bool pa = a > eps, pb = b > eps;
+  if (!pa && !pb) return 0.0;
+  if (pa && pb) return 1.0 / (sqrt(a) + sqrt(b));
+  const double fa = pa ? sqrt(a) : 0.0, fb = pb ? sqrt(b) : 0.0;
+  return (fa - fb) / (a - b);
+}
+__device__ __forceinline__ double dd_isqrt(double a, double b, double eps) {
+  const bool pa = !(a < eps), pb = !(b < eps);
+  if (!pa && !pb) return 0.0;
+  if (pa && pb) {
+    const double sa = sqrt(a), sb = sqrt(b);
+    return -1.0 / (sa * sb * (sa + sb));
+  }
+  const double ga = pa ? 1.0 / sqrt(a) : 1.0 / sqrt((double)FLT_MAX + a);
+  const double gb = pb ? 1.0 / sqrt(b) : 1.0 / sqrt((double)FLT_MAX + b);
+  return (ga - gb) / (a - b);
+}
+
+// Vidal edge decomposition adjoint (tensor_ops.py:108-130) from the forward's
+// tape.  Given Xbar (orthogonalisers) and lambar writes n2e_bar for both
+// messages of every edge (Hermitian part, as the forward symmetrises).
+__global__ void __launch_bounds__(256)
+    k_edge_svd_vjp(DevGraph g, const char* tape, double eps, const cd* Xbar, const double* lambar, cd* n2e_bar,
+                   char* ws, long long wsb) {
+  __shared__ double sb[64], kap[64], lamv[64];
+  const int n = g.chi;
+  const long long cc = (long long)n * n;
+  for (int e = blockIdx.x; e < g.n_edges; e += gridDim.x) {
+    const EdgeSvdTape tp((char*)tape + (long long)e * edge_svd_tape_bytes(n), n);
+    cd* w = (cd*)(ws + (long long)blockIdx.x * wsb);
+    cd *If = w, *Ib = If + cc, *Sf = Ib + cc, *Sb = Sf + cc, *Ub = Sb + cc, *Vb = Ub + cc, *Ifb = Vb + cc,
+       *Ibb = Ifb + cc, *Gb = Ibb + cc, *w1 = Gb + cc, *w2 = w1 + cc, *w3 = w2 + cc, *Sfb = w3 + cc, *Sbb = Sfb + cc;
+    const double nrm = tp.scal()[0], nu = tp.scal()[1], nv = tp.scal()[2];
+    const cd *Qf = tp.Qf(), *Qb = tp.Qb(), *U = tp.U(), *V = tp.V();
+    const double *wf = tp.wf(), *wb = tp.wb(), *sg = tp.sigma();
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      const double l = nrm > 0.0 ? sg[j] / nrm : 0.0;
+      const double lt = l > eps ? l : 0.0;
+      lamv[j] = lt;
+      kap[j] = lt > eps ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    // matrix functions of both messages
+    for (long long q = threadIdx.x; q < cc; q += blockDim.x) {
+      const int r = (int)(q / n), c = (int)(q % n);
+      cd a = mk<double>(0, 0), b = a, ai = a, bi = a;
+      for (int i = 0; i < n; ++i) {
+        const cd pf = Qf[(long long)r * n + i] * conj(Qf[(long long)c * n + i]);
+        const cd pb = Qb[(long long)r * n + i] * conj(Qb[(long long)c * n + i]);
+        a = a + (wf[i] > eps ? sqrt(wf[i]) : 0.0) * pf;
+        b = b + (wb[i] > eps ? sqrt(wb[i]) : 0.0) * pb;
+        ai = ai + (wf[i] < eps ? 1.0 / sqrt((double)FLT_MAX + wf[i]) : 1.0 / sqrt(wf[i])) * pf;
+        bi = bi + (wb[i] < eps ? 1.0 / sqrt((double)FLT_MAX + wb[i]) : 1.0 / sqrt(wb[i])) * pb;
+      }
+      Sf[q] = a;
+      Sb[q] = b;
+      If[q] = ai;
+      Ib[q] = bi;
+    }
+    __syncthreads();
+    const cd* xf = Xbar + (2LL * e) * cc;
+    const cd* xb = Xbar + (2LL * e + 1) * cc;
+    // Ubar = kappa (If^H Xf-bar) / nu ; If-bar = Xf-bar (U kappa)^H / nu
+    // Vbar = kappa (Ib^T conj(Xb-bar)) / nv ; Ib-bar = Xb-bar (V kappa)^T / nv
+    for (long long q = threadIdx.x; q < cc; q += blockDim.x) {
+      const int i = (int)(q / n), k = (int)(q % n);
+      cd u = mk<double>(0, 0), v = u, fi = u, bi = u;
+      for (int b = 0; b < n; ++b) {
+        fma_acc(u, conj(If[(long long)b * n + i]), xf[(long long)b * n + k]);
+        fma_acc(v, Ib[(long long)b * n + i], conj(xb[(long long)b * n + k]));
+      }
+      for (int kk = 0; kk < n; ++kk) {
+        if (kap[kk] == 0.0) continue;
+        fma_acc(fi, xf[(long long)i * n + kk], conj(U[(long long)k * n + kk]));
+        fma_acc(bi, xb[(long long)i * n + kk], V[(long long)k * n + kk]);
+      }
+      Ub[q] = (kap[k] / nu) * u;
+      Vb[q] = (kap[k] / nv) * v;
+      Ifb[q] = (1.0 / nu) * fi;
+      Ibb[q] = (1.0 / nv) * bi;
+    }
+    // sigma-bar from lambda = sigma / |sigma| (kept entries)
+    {
+      double dot = 0.0;
+      for (int j = 0; j < n; ++j) dot += kap[j] * lamv[j] * lambar[(long long)e * n + j];
+      for (int j = threadIdx.x; j < n; j += blockDim.x)
+        sb[j] = nrm > 0.0 ? kap[j] * (lambar[(long long)e * n + j] - dot * lamv[j]) / nrm : 0.0;
+    }
+    __syncthreads();
+    adj::svd_vjp(U, V, sg, Ub, Vb, sb, n, Gb, w1, w2, w3);
+    // G = Sf Sb^T:  Sf-bar = Gbar conj(Sb),  Sb-bar = Gbar^T conj(Sf)
+    adj::bmm(Sfb, n, Gb, n, adj::N, Sb, n, adj::C, n, n, n, false);
+    adj::bmm(Sbb, n, Gb, n, adj::T, Sf, n, adj::C, n, n, n, false);
+    for (int side = 0; side < 2; ++side) {
+      const cd* Q = side == 0 ? Qf : Qb;
+      const double* wv = side == 0 ? wf : wb;
+      const cd* Fs = side == 0 ? Sfb : Sbb;
+      const cd* Fi = side == 0 ? Ifb : Ibb;
+      // w1 = Q^H Fs Q, w2 = Q^H Fi Q
+      adj::bmm(w3, n, Fs, n, adj::N, Q, n, adj::N, n, n, n, false);
+      adj::bmm(w1, n, Q, n, adj::H, w3, n, adj::N, n, n, n, false);
+      adj::bmm(w3, n, Fi, n, adj::N, Q, n, adj::N, n, n, n, false);
+      adj::bmm(w2, n, Q, n, adj::H, w3, n, adj::N, n, n, n, false);
+      for (long long q = threadIdx.x; q < cc; q += blockDim.x) {
+        const int i = (int)(q / n), j = (int)(q % n);
+        double df, dg;
+        if (i == j) {
+          const double x = wv[i];
+          df = x > eps ? 0.5 / sqrt(x) : 0.0;
+          dg = x < eps ? 0.0 : -0.5 / (x * sqrt(x));
+        } else {
+          df = dd_sqrt(wv[i], wv[j], eps);
+          dg = dd_isqrt(wv[i], wv[j], eps);
+          if (wv[i] == wv[j]) {
+            df = wv[i] > eps ? 0.5 / sqrt(wv[i]) : 0.0;
+            dg = wv[i] < eps ? 0.0 : -0.5 / (wv[i] * sqrt(wv[i]));
+          }
+        }
+        w3[q] = df * w1[q] + dg * w2[q];
+      }
+      __syncthreads();
+      adj::bmm(w1, n, Q, n, adj::N, w3, n, adj::N, n, n, n, false);
+      adj::bmm(w2, n, w1, n, adj::N, Q, n, adj::H, n, n, n, false);
+      cd* out = n2e_bar + (2LL * e + side) * cc;
+      for (long long q = threadIdx.x; q < cc; q += blockDim.x) {
+        const int i = (int)(q / n), j = (int)(q % n);
+        const cd a = w2[q], b = w2[(long long)j * n + i];
+        out[q] = mk<double>(0.5 * (a.re + b.re), 0.5 * (a.im - b.im));
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Symmetric-gauge adjoint: T = Gamma * prod_i sqrt(lambda_i > eps) / |.|
+// (tensor_ops.py:189-199).  Accumulates gam_bar; lam_bar by atomics.
+__global__ void __launch_bounds__(256)
+    k_symmetric_vjp(DevGraph g, const cd* gam, const double* lam, double eps, const cd* tbar, cd* gam_bar,
+                    double* lam_bar) {
+  __shared__ double red[8];
+  __shared__ double lacc[kMaxDeg * 64];
+  const int chi = g.chi, p = g.phys;
+  for (int node = blockIdx.x; node < g.n_nodes; node += gridDim.x) {
+    const int d = g.deg[node], base = g.adj_ptr[node];
+    const long long S = ipow(chi, d) * p;
+    const cd* t = gam + g.toff[node];
+    const cd* tb = tbar + g.toff[node];
+    cd* gb = gam_bar + g.toff[node];
+    for (int q = threadIdx.x; q < d * chi; q += blockDim.x) lacc[q] = 0.0;
+    double nn = 0.0;
+    for (long long f = threadIdx.x; f < S; f += blockDim.x) {
+      long long r = f / p;
+      double w = 1.0;
+      for (int j = d - 1; j >= 0; --j) {
+        const double l = lam[(long long)g.adj_edge[base + j] * chi + r % chi];
+        r /= chi;
+        w *= l > eps ? sqrt(l) : 0.0;
+      }
+      nn += w * w * norm2(t[f]);
+    }
+    nn = sqrt(block_sum(nn, red));
+    double ip = 0.0;  // Re<T, Tbar> with T = Gamma w / nn
+    for (long long f = threadIdx.x; f < S; f += blockDim.x) {
+      long long r = f / p;
+      double w = 1.0;
+      for (int j = d - 1; j >= 0; --j) {
+        const double l = lam[(long long)g.adj_edge[base + j] * chi + r % chi];
+        r /= chi;
+        w *= l > eps ? sqrt(l) : 0.0;
+      }
+      ip += (w / nn) * (t[f].re * tb[f].re + t[f].im * tb[f].im);
+    }
+    ip = block_sum(ip, red);
+    for (long long f = threadIdx.x; f < S; f += blockDim.x) {
+      long long r = f / p;
+      int bj[kMaxDeg];
+      double fj[kMaxDeg];
+      double w = 1.0;
+      for (int j = d - 1; j >= 0; --j) {
+        bj[j] = (int)(r % chi);
+        r /= chi;
+        const double l = lam[(long long)g.adj_edge[base + j] * chi + bj[j]];
+        fj[j] = l > eps ? sqrt(l) : 0.0;
+        w *= fj[j];
+      }
+      // Ybar = (Tbar - ip T) / nn;  Gamma-bar += Ybar w
+      const cd yb = (1.0 / nn) * (tb[f] - (ip * w / nn) * t[f]);
+      gb[f] = gb[f] + w * yb;
+      const double rr = yb.re * t[f].re + yb.im * t[f].im;  // Re conj(Ybar) Gamma
+      if (rr != 0.0)
+        for (int j = 0; j < d; ++j) {
+          if (fj[j] == 0.0) continue;
+          double others = 1.0;
+          for (int i = 0; i < d; ++i)
+            if (i != j) others *= fj[i];
+          atomicAdd(&lacc[j * chi + bj[j]], rr * others * 0.5 / fj[j]);
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < d * chi; q += blockDim.x)
+      if (lacc[q] != 0.0) atomicAdd(&lam_bar[(long long)g.adj_edge[base + q / chi] * chi + q % chi], lacc[q]);
+    __syncthreads();
+  }
+}
+
+// lambdas2messages adjoint (symmetric_gauge.py:50-61): lam_bar[e][k] +=
+// Re of the diagonal of every message gradient of edge e (n2e and e2n,
+// both sides; either pointer may be NULL).
+__global__ void k_lam2msg_vjp(DevGraph g, const cd* n2e_bar, const cd* e2n_bar, double* lam_bar) {
+  const int chi = g.chi;
+  const long long cc = (long long)chi * chi;
+  const long long total = (long long)g.n_edges * chi;
+  for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < total; f += (long long)gridDim.x * blockDim.x) {
+    const long long e = f / chi;
+    const int k = (int)(f % chi);
+    double a = 0.0;
+    for (int s = 0; s < 2; ++s) {
+      const long long q = (2 * e + s) * cc + (long long)k * chi + k;
+      if (n2e_bar) a += n2e_bar[q].re;
+      if (e2n_bar) a += e2n_bar[q].re;
+    }
+    lam_bar[f] += a;
+  }
+}
+
+// One-site density matrix adjoint with diagonal messages diag(lambda)
+// (bp_quantum_emulator.py:194-205 feeds the symmetric tensors and
+// diag(lambda)): rho = rt / tr(rt), rt[s][s'] = sum_a w_a T[a,s] conj(T[a,s']),
+// w_a = prod_j lambda_j[a_j].  Accumulates Tbar; lam_bar by atomics.
+__global__ void __launch_bounds__(256)
+    k_rdm1_vjp(DevGraph g, const cd* T, const double* lam, const int* nodes, const cd* rho_bar, cd* Tbar,
+               double* lam_bar) {
+  __shared__ cd rt[64], rb[64];
+  __shared__ double lacc[kMaxDeg * 64];
+  const int chi = g.chi, p = g.phys;
+  const int node = nodes[blockIdx.x];
+  const int d = g.deg[node], base = g.adj_ptr[node];
+  const long long rows = ipow(chi, d);
+  const cd* t = T + g.toff[node];
+  for (int q = threadIdx.x; q < p * p; q += blockDim.x) {
+    const int s = q / p, sp = q % p;
+    cd acc = mk<double>(0, 0);
+    for (long long a = 0; a < rows; ++a) {
+      long long r = a;
+      double w = 1.0;
+      for (int j = d - 1; j >= 0; --j) {
+        w *= lam[(long long)g.adj_edge[base + j] * chi + r % chi];
+        r /= chi;
+      }
+      fma_acc(acc, w * t[a * p + s], conj(t[a * p + sp]));
+    }
+    rt[q] = acc;
+  }
+  for (int q = threadIdx.x; q < d * chi; q += blockDim.x) lacc[q] = 0.0;
+  __syncthreads();
+  adj::trace_norm_vjp(rt, rho_bar + (long long)blockIdx.x * p * p, p, rb);
+  cd* tb = Tbar + g.toff[node];
+  for (long long a = threadIdx.x; a < rows; a += blockDim.x) {
+    long long r = a;
+    int bj[kMaxDeg];
+    double lj[kMaxDeg];
+    double w = 1.0;
+    for (int j = d - 1; j >= 0; --j) {
+      bj[j] = (int)(r % chi);
+      r /= chi;
+      lj[j] = lam[(long long)g.adj_edge[base + j] * chi + bj[j]];
+      w *= lj[j];
+    }
+    double q2 = 0.0;  // Re sum conj(rb[s][s']) T[a,s] conj(T[a,s'])
+    for (int s = 0; s < p; ++s) {
+      cd v = mk<double>(0, 0);
+      for (int sp = 0; sp < p; ++sp) {
+        const cd x = t[a * p + sp];
+        fma_acc(v, rb[s * p + sp] + conj(rb[sp * p + s]), x);
+        const cd z = conj(rb[s * p + sp]) * t[a * p + s] * conj(x);
+        q2 += z.re;
+      }
+      tb[a * p + s] = tb[a * p + s] + w * v;
+    }
+    if (q2 != 0.0)
+      for (int j = 0; j < d; ++j) {
+        double others = 1.0;
+        for (int i = 0; i < d; ++i)
+          if (i != j) others *= lj[i];
+        atomicAdd(&lacc[j * chi + bj[j]], q2 * others);
+      }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < d * chi; q += blockDim.x)
+    if (lacc[q] != 0.0) atomicAdd(&lam_bar[(long long)g.adj_edge[base + q / chi] * chi + q % chi], lacc[q]);
+}
+
+// Two-site density matrix adjoint with diagonal messages on the outer bonds:
+// rt = sum_{k,k'} E_a[(k s_a),(k' s_a')] E_b[(k s_b),(k' s_b')], rho = rt / tr,
+// E_x[(k s),(k' s')] = sum_o w_x(o) T_x[o,k,s] conj(T_x[o,k',s']).  Tbar and
+// lam_bar by atomics (several edges meet at a node).  ws: 4 (chi p)^2 cd.
+__global__ void __launch_bounds__(256)
+    k_rdm2_vjp(DevGraph g, const cd* T, const double* lam, const int* edges, const cd* rho_bar, cd* Tbar,
+               double* lam_bar, char* ws, long long wsb) {
+  __shared__ cd rt[256], rb[256];
+  const int chi = g.chi, p = g.phys, cp = chi * p, pp = p * p;
+  const int e = edges[blockIdx.x];
+  const int nd[2] = {g.edge_node[2 * e], g.edge_node[2 * e + 1]};
+  const int sl[2] = {find_slot(g, nd[0], e, 0), find_slot(g, nd[1], e, 1)};
+  cd* E[2];
+  cd* Eb[2];
+  E[0] = (cd*)(ws + (long long)blockIdx.x * wsb);
+  E[1] = E[0] + (long long)cp * cp;
+  Eb[0] = E[1] + (long long)cp * cp;
+  Eb[1] = Eb[0] + (long long)cp * cp;
+  int str[2][kMaxDeg];
+  for (int x = 0; x < 2; ++x) {
+    const int d = g.deg[nd[x]];
+    int mm = p;
+    for (int j = d - 1; j >= 0; --j) {
+      str[x][j] = mm;
+      mm *= chi;
+    }
+  }
+  for (int x = 0; x < 2; ++x) {
+    const int nod = nd[x], d = g.deg[nod], base = g.adj_ptr[nod], slot = sl[x];
+    const long long orows = ipow(chi, d - 1);
+    const cd* t = T + g.toff[nod];
+    for (int q = threadIdx.x; q < cp * cp; q += blockDim.x) {
+      const int a = q / cp, b = q % cp;
+      const int k = a / p, s = a % p, kp = b / p, sp = b % p;
+      cd acc = mk<double>(0, 0);
+      for (long long o = 0; o < orows; ++o) {
+        long long r = o;
+        double w = 1.0;
+        for (int j = d - 1; j >= 0; --j) {
+          if (j == slot) continue;
+          w *= lam[(long long)g.adj_edge[base + j] * chi + r % chi];
+          r /= chi;
+        }
+        fma_acc(acc, w * t[elem_off(str[x], d, slot, chi, (int)o, k, s)], conj(t[elem_off(str[x], d, slot, chi, (int)o, kp, sp)]));
+      }
+      E[x][q] = acc;
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < pp * pp; q += blockDim.x) {
+    const int row = q / pp, col = q % pp;
+    const int sa = row / p, sb = row % p, sap = col / p, sbp = col % p;
+    cd v = mk<double>(0, 0);
+    for (int k = 0; k < chi; ++k)
+      for (int kp = 0; kp < chi; ++kp)
+        fma_acc(v, E[0][(k * p + sa) * cp + kp * p + sap], E[1][(k * p + sb) * cp + kp * p + sbp]);
+    rt[q] = v;
+  }
+  __syncthreads();
+  adj::trace_norm_vjp(rt, rho_bar + (long long)blockIdx.x * pp * pp, pp, rb);
+  // Ebar_a[(k sa),(k' sa')] = sum_{sb,sb'} rb[(sa sb),(sa' sb')] conj(E_b[(k sb),(k' sb')]), and vice versa
+  for (int q = threadIdx.x; q < cp * cp; q += blockDim.x) {
+    const int a = q / cp, b = q % cp;
+    const int k = a / p, s = a % p, kp = b / p, sp = b % p;
+    cd va = mk<double>(0, 0), vb = va;
+    for (int u = 0; u < p; ++u)
+      for (int up = 0; up < p; ++up) {
+        fma_acc(va, rb[(s * p + u) * pp + sp * p + up], conj(E[1][(k * p + u) * cp + kp * p + up]));
+        fma_acc(vb, rb[(u * p + s) * pp + up * p + sp], conj(E[0][(k * p + u) * cp + kp * p + up]));
+      }
+    Eb[0][q] = va;
+    Eb[1][q] = vb;
+  }
+  __syncthreads();
+  for (int x = 0; x < 2; ++x) {
+    const int nod = nd[x], d = g.deg[nod], base = g.adj_ptr[nod], slot = sl[x];
+    const long long orows = ipow(chi, d - 1);
+    const cd* t = T + g.toff[nod];
+    cd* tb = Tbar + g.toff[nod];
+    const cd* Ex = Eb[x];
+    for (long long f = threadIdx.x; f < orows * cp; f += blockDim.x) {
+      const int o = (int)(f / cp), ks = (int)(f % cp);
+      const int k = ks / p, s = ks % p;
+      long long r = o;
+      int bj[kMaxDeg];
+      double lj[kMaxDeg];
+      double w = 1.0;
+      for (int j = d - 1; j >= 0; --j) {
+        if (j == slot) continue;
+        bj[j] = (int)(r % chi);
+        r /= chi;
+        lj[j] = lam[(long long)g.adj_edge[base + j] * chi + bj[j]];
+        w *= lj[j];
+      }
+      cd v = mk<double>(0, 0);
+      double q2 = 0.0;
+      const cd tv = t[elem_off(str[x], d, slot, chi, o, k, s)];
+      for (int k2 = 0; k2 < chi; ++k2)
+        for (int s2 = 0; s2 < p; ++s2) {
+          const int ks2 = k2 * p + s2;
+          const cd x2 = t[elem_off(str[x], d, slot, chi, o, k2, s2)];
+          fma_acc(v, Ex[(long long)ks * cp + ks2] + conj(Ex[(long long)ks2 * cp + ks]), x2);
+          const cd z = conj(Ex[(long long)ks * cp + ks2]) * tv * conj(x2);
+          q2 += z.re;
+        }
+      const long long off = elem_off(str[x], d, slot, chi, o, k, s);
+      atomicAdd(&tb[off].re, w * v.re);
+      atomicAdd(&tb[off].im, w * v.im);
+      if (q2 != 0.0)
+        for (int j = 0; j < d; ++j) {
+          if (j == slot) continue;
+          double others = 1.0;
+          for (int i = 0; i < d; ++i)
+            if (i != j && i != slot) others *= lj[i];
+          atomicAdd(&lam_bar[(long long)g.adj_edge[base + j] * chi + bj[j]], q2 * others);
+        }
+    }
+  }
+}
+
+// One-site gate adjoint: Gamma' = Y / |Y|, Y[a,o] = sum_i G[o][i] T[a,i]
+// (tensor_graph.py:268-275).  Tbar turned in place from post- to pre-gate
+// gradients; gbar per gate.
+__global__ void __launch_bounds__(256)
+    k_apply1_vjp(DevGraph g, const cd* Tpre, const int* nodes, const cd* gates, cd* Tbar, cd* gbar) {
+  __shared__ double red[8];
+  __shared__ cd gacc[64];
+  const int chi = g.chi, p = g.phys;
+  const int node = nodes[blockIdx.x];
+  const long long rows = ipow(chi, g.deg[node]);
+  const cd* t = Tpre + g.toff[node];
+  cd* tb = Tbar + g.toff[node];
+  const cd* G = gates + (long long)blockIdx.x * p * p;
+  double nn = 0.0, ip = 0.0;
+  for (long long a = threadIdx.x; a < rows; a += blockDim.x)
+    for (int o = 0; o < p; ++o) {
+      cd y = mk<double>(0, 0);
+      for (int i = 0; i < p; ++i) fma_acc(y, G[o * p + i], t[a * p + i]);
+      nn += norm2(y);
+      const cd yb = tb[a * p + o];
+      ip += y.re * yb.re + y.im * yb.im;
+    }
+  nn = sqrt(block_sum(nn, red));
+  ip = block_sum(ip, red) / nn;
+  for (int q = threadIdx.x; q < p * p; q += blockDim.x) gacc[q] = mk<double>(0, 0);
+  __syncthreads();
+  for (long long a = threadIdx.x; a < rows; a += blockDim.x) {
+    cd yb[8], tin[8];
+    for (int i = 0; i < p; ++i) tin[i] = t[a * p + i];
+    for (int o = 0; o < p; ++o) {
+      cd y = mk<double>(0, 0);
+      for (int i = 0; i < p; ++i) fma_acc(y, G[o * p + i], tin[i]);
+      yb[o] = (1.0 / nn) * (tb[a * p + o] - (ip / nn) * y);
+    }
+    for (int i = 0; i < p; ++i) {
+      cd v = mk<double>(0, 0);
+      for (int o = 0; o < p; ++o) fma_acc(v, conj(G[o * p + i]), yb[o]);
+      tb[a * p + i] = v;
+    }
+    for (int o = 0; o < p; ++o)
+      for (int i = 0; i < p; ++i) {
+        const cd z = yb[o] * conj(tin[i]);
+        atomicAdd(&gacc[o * p + i].re, z.re);
+        atomicAdd(&gacc[o * p + i].im, z.im);
+      }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < p * p; q += blockDim.x) gbar[(long long)blockIdx.x * p * p + q] = gacc[q];
+}
+
+// ------------------------------------------------------------- launchers
+static inline int vjp_grid(long long items) { return (int)(items < 2048 ? (items > 0 ? items : 1) : 2048); }
+
+long long bp_vjp_ws_bytes(int chi, int phys, int max_deg) {
+  const long long S = ipow(chi, max_deg) * phys, cc = (long long)chi * chi;
+  return ((max_deg + 2) * S + (3 + max_deg) * cc) * 16 + 256;
+}
+long long absorb_vjp_ws_bytes(int chi, int phys, int max_deg) {
+  const long long S = ipow(chi, max_deg) * phys, cc = (long long)chi * chi;
+  return ((max_deg + 2) * S + cc) * 16 + 256;
+}
+long long edge_svd_vjp_ws_bytes(int chi) { return 14LL * chi * chi * 16 + 256; }
+long long rdm2_vjp_ws_bytes(int chi, int phys) { return 4LL * (chi * phys) * (chi * phys) * 16 + 256; }
+int vjp_blocks(long long items) { return vjp_grid(items); }
+
+hipError_t launch_bp_node_vjp(const DevGraph& g, const void* T, const void* e2n, const void* obar, void* ebar,
+                              void* Tbar, void* ws, long long wsb, hipStream_t s) {
+  if (g.n_nodes <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bp_node_vjp, dim3(vjp_grid(g.n_nodes)), dim3(256), 0, s, g, (const cd*)T, (const cd*)e2n,
+                     (const cd*)obar, (cd*)ebar, (cd*)Tbar, (char*)ws, wsb);
+  return hipGetLastError();
+}
+hipError_t launch_absorb_vjp(const DevGraph& g, const void* T, const void* X, const void* gbar, void* Tbar, void* Xbar,
+                             void* ws, long long wsb, hipStream_t s) {
+  if (g.n_nodes <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_absorb_vjp, dim3(vjp_grid(g.n_nodes)), dim3(256), 0, s, g, (const cd*)T, (const cd*)X,
+                     (const cd*)gbar, (cd*)Tbar, (cd*)Xbar, (char*)ws, wsb);
+  return hipGetLastError();
+}
+hipError_t launch_edge_svd_vjp(const DevGraph& g, const void* tape, double eps, const void* Xbar, const double* lambar,
+                               void* n2e_bar, void* ws, long long wsb, hipStream_t s) {
+  if (g.n_edges <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_edge_svd_vjp, dim3(vjp_grid(g.n_edges)), dim3(256), 0, s, g, (const char*)tape, eps,
+                     (const cd*)Xbar, lambar, (cd*)n2e_bar, (char*)ws, wsb);
+  return hipGetLastError();
+}
+hipError_t launch_symmetric_vjp(const DevGraph& g, const void* gam, const double* lam, double eps, const void* tbar,
+                                void* gam_bar, double* lam_bar, hipStream_t s) {
+  if (g.n_nodes <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_symmetric_vjp, dim3(vjp_grid(g.n_nodes)), dim3(256), 0, s, g, (const cd*)gam, lam, eps,
+                     (const cd*)tbar, (cd*)gam_bar, lam_bar);
+  return hipGetLastError();
+}
+hipError_t launch_lam2msg_vjp(const DevGraph& g, const void* n2e_bar, const void* e2n_bar, double* lam_bar,
+                              hipStream_t s) {
+  if (g.n_edges <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lam2msg_vjp, dim3(grid_for((long long)g.n_edges * g.chi, 256)), dim3(256), 0, s, g,
+                     (const cd*)n2e_bar, (const cd*)e2n_bar, lam_bar);
+  return hipGetLastError();
+}
+hipError_t launch_rdm1_vjp(const DevGraph& g, const void* T, const double* lam, const int* nodes, int count,
+                           const void* rho_bar, void* Tbar, double* lam_bar, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rdm1_vjp, dim3(count), dim3(256), 0, s, g, (const cd*)T, lam, nodes, (const cd*)rho_bar,
+                     (cd*)Tbar, lam_bar);
+  return hipGetLastError();
+}
+hipError_t launch_rdm2_vjp(const DevGraph& g, const void* T, const double* lam, const int* edges, int count,
+                           const void* rho_bar, void* Tbar, double* lam_bar, void* ws, long long wsb, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rdm2_vjp, dim3(count), dim3(256), 0, s, g, (const cd*)T, lam, edges, (const cd*)rho_bar,
+                     (cd*)Tbar, lam_bar, (char*)ws, wsb);
+  return hipGetLastError();
+}
+hipError_t launch_apply1_vjp(const DevGraph& g, const void* Tpre, const int* nodes, int count, const void* gates,
+                             void* Tbar, void* gbar, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_apply1_vjp, dim3(count), dim3(256), 0, s, g, (const cd*)Tpre, nodes, (const cd*)gates,
+                     (cd*)Tbar, (cd*)gbar);
+  return hipGetLastError();
 }
 
 }  // namespace dq

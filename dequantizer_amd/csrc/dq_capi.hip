@@ -316,8 +316,26 @@ int dq_vidal_gauge(dq_graph* g, int dtype, const void* T, const void* n2e, doubl
   const size_t need_e = (size_t)wse * (g->dev.n_edges > 0 ? g->dev.n_edges : 1);
   const size_t need_n = (size_t)wsn * (g->dev.n_nodes > 0 ? g->dev.n_nodes : 1);
   if ((rc = ws_ensure(g, need_e > need_n ? need_e : need_n)) != DQ_OK) return rc;
-  RET_LAUNCH(launch_edge_svd(rb, g->dev, n2e, eps, g->X, lam, g->ws, wse, inst_active, S(stream)));
+  RET_LAUNCH(launch_edge_svd(rb, g->dev, n2e, eps, g->X, lam, g->ws, wse, inst_active, nullptr, S(stream)));
   RET_LAUNCH(launch_absorb(rb, g->dev, T, g->X, gammas, g->ws, wsn, inst_active, S(stream)));
+  return DQ_OK;
+}
+
+long long dq_edge_svd_tape_bytes(const dq_graph* g) { return g ? edge_svd_tape_bytes(g->dev.chi) : -1; }
+
+int dq_vidal_gauge_tape(dq_graph* g, int dtype, const void* T, const void* n2e, double eps, void* gammas, double* lam,
+                        void* X, void* tape, void* stream) {
+  int rb, rc;
+  if (!g || !X || !tape) return fail(DQ_ERR_INVALID, "NULL argument");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  if (gammas == T) return fail(DQ_ERR_INVALID, "gammas must not alias tensors");
+  const long long wse = (edge_svd_ws_bytes_host(g->dev.chi) + 255) / 256 * 256;
+  const long long wsn = (node_ws_bytes(g, rb) + 255) / 256 * 256;
+  const size_t need_e = (size_t)wse * (g->dev.n_edges > 0 ? g->dev.n_edges : 1);
+  const size_t need_n = (size_t)wsn * (g->dev.n_nodes > 0 ? g->dev.n_nodes : 1);
+  if ((rc = ws_ensure(g, need_e > need_n ? need_e : need_n)) != DQ_OK) return rc;
+  RET_LAUNCH(launch_edge_svd(rb, g->dev, n2e, eps, (cd*)X, lam, g->ws, wse, nullptr, tape, S(stream)));
+  RET_LAUNCH(launch_absorb(rb, g->dev, T, (const cd*)X, gammas, g->ws, wsn, nullptr, S(stream)));
   return DQ_OK;
 }
 
@@ -378,6 +396,90 @@ int dq_simple_update_vjp(dq_graph* g, int dtype, const void* T_pre, const double
   if ((rc = ws_ensure(g, (size_t)wsb * count)) != DQ_OK) return rc;
   RET_LAUNCH(launch_su_vjp(g->dev, T_pre, lam_pre, (const SuGate*)gates, count, (const cd*)halves, max_gate_rank, eps,
                            tape, T_bar, lam_bar, halves_bar, g->ws, wsb, S(stream)));
+  return DQ_OK;
+}
+
+static int need_c128(int dtype) {
+  return dtype == DQ_C128 ? DQ_OK : fail(DQ_ERR_UNSUPPORTED, "the adjoint runs in complex128 only");
+}
+
+int dq_bp_sweep_vjp(dq_graph* g, int dtype, const void* T, const void* e2n_in, const void* n2e_out_bar,
+                    void* e2n_in_bar, void* T_bar, void* stream) {
+  int rc;
+  if (!g || !T || !e2n_in || !n2e_out_bar || !e2n_in_bar || !T_bar) return fail(DQ_ERR_INVALID, "NULL argument");
+  if ((rc = need_c128(dtype)) != DQ_OK) return rc;
+  const long long wsb = (bp_vjp_ws_bytes(g->dev.chi, g->dev.phys, g->max_deg) + 255) / 256 * 256;
+  if ((rc = ws_ensure(g, (size_t)wsb * vjp_blocks(g->dev.n_nodes))) != DQ_OK) return rc;
+  RET_LAUNCH(launch_bp_node_vjp(g->dev, T, e2n_in, n2e_out_bar, e2n_in_bar, T_bar, g->ws, wsb, S(stream)));
+  return DQ_OK;
+}
+
+int dq_vidal_gauge_vjp(dq_graph* g, int dtype, const void* T, const void* X, const void* tape, double eps,
+                       const void* gammas_bar, const double* lam_bar, void* X_bar, void* T_bar, void* n2e_bar,
+                       void* stream) {
+  int rc;
+  if (!g || !T || !X || !tape || !gammas_bar || !lam_bar || !X_bar || !T_bar || !n2e_bar)
+    return fail(DQ_ERR_INVALID, "NULL argument");
+  if ((rc = need_c128(dtype)) != DQ_OK) return rc;
+  const long long wa = (absorb_vjp_ws_bytes(g->dev.chi, g->dev.phys, g->max_deg) + 255) / 256 * 256;
+  const long long we = (edge_svd_vjp_ws_bytes(g->dev.chi) + 255) / 256 * 256;
+  const size_t na = (size_t)wa * vjp_blocks(g->dev.n_nodes), ne = (size_t)we * vjp_blocks(g->dev.n_edges);
+  if ((rc = ws_ensure(g, na > ne ? na : ne)) != DQ_OK) return rc;
+  RET_LAUNCH(launch_absorb_vjp(g->dev, T, X, gammas_bar, T_bar, X_bar, g->ws, wa, S(stream)));
+  RET_LAUNCH(launch_edge_svd_vjp(g->dev, tape, eps, X_bar, lam_bar, n2e_bar, g->ws, we, S(stream)));
+  return DQ_OK;
+}
+
+int dq_symmetric_gauge_vjp(dq_graph* g, int dtype, const void* gammas, const double* lam, double eps,
+                           const void* out_bar, void* gammas_bar, double* lam_bar, void* stream) {
+  int rc;
+  if (!g || !gammas || !lam || !out_bar || !gammas_bar || !lam_bar) return fail(DQ_ERR_INVALID, "NULL argument");
+  if ((rc = need_c128(dtype)) != DQ_OK) return rc;
+  RET_LAUNCH(launch_symmetric_vjp(g->dev, gammas, lam, eps, out_bar, gammas_bar, lam_bar, S(stream)));
+  return DQ_OK;
+}
+
+int dq_lambdas_to_messages_vjp(dq_graph* g, int dtype, const void* n2e_bar, const void* e2n_bar, double* lam_bar,
+                               void* stream) {
+  int rc;
+  if (!g || !lam_bar) return fail(DQ_ERR_INVALID, "NULL argument");
+  if ((rc = need_c128(dtype)) != DQ_OK) return rc;
+  RET_LAUNCH(launch_lam2msg_vjp(g->dev, n2e_bar, e2n_bar, lam_bar, S(stream)));
+  return DQ_OK;
+}
+
+int dq_rdm1_vjp(dq_graph* g, int dtype, const void* T, const double* lam, const int* nodes, int count,
+                const void* rho_bar, void* T_bar, double* lam_bar, void* stream) {
+  int rc;
+  if (!g || count < 0 || (count > 0 && (!T || !lam || !nodes || !rho_bar || !T_bar || !lam_bar)))
+    return fail(DQ_ERR_INVALID, "bad argument");
+  if ((rc = need_c128(dtype)) != DQ_OK) return rc;
+  if (g->dev.phys > 8) return fail(DQ_ERR_UNSUPPORTED, "density-matrix adjoints support phys <= 8");
+  RET_LAUNCH(launch_rdm1_vjp(g->dev, T, lam, nodes, count, rho_bar, T_bar, lam_bar, S(stream)));
+  return DQ_OK;
+}
+
+int dq_rdm2_vjp(dq_graph* g, int dtype, const void* T, const double* lam, const int* edges, int count,
+                const void* rho_bar, void* T_bar, double* lam_bar, void* stream) {
+  int rc;
+  if (!g || count < 0 || (count > 0 && (!T || !lam || !edges || !rho_bar || !T_bar || !lam_bar)))
+    return fail(DQ_ERR_INVALID, "bad argument");
+  if ((rc = need_c128(dtype)) != DQ_OK) return rc;
+  if (g->dev.phys > 4) return fail(DQ_ERR_UNSUPPORTED, "two-site density-matrix adjoint supports phys <= 4");
+  const long long wsb = (rdm2_vjp_ws_bytes(g->dev.chi, g->dev.phys) + 255) / 256 * 256;
+  if ((rc = ws_ensure(g, (size_t)wsb * (count > 0 ? count : 1))) != DQ_OK) return rc;
+  RET_LAUNCH(launch_rdm2_vjp(g->dev, T, lam, edges, count, rho_bar, T_bar, lam_bar, g->ws, wsb, S(stream)));
+  return DQ_OK;
+}
+
+int dq_apply1_vjp(dq_graph* g, int dtype, const void* T_pre, const int* nodes, int count, const void* gates,
+                  void* T_bar, void* gates_bar, void* stream) {
+  int rc;
+  if (!g || count < 0 || (count > 0 && (!T_pre || !nodes || !gates || !T_bar || !gates_bar)))
+    return fail(DQ_ERR_INVALID, "bad argument");
+  if ((rc = need_c128(dtype)) != DQ_OK) return rc;
+  if (g->dev.phys > 8) return fail(DQ_ERR_UNSUPPORTED, "one-site gate adjoint supports phys <= 8");
+  RET_LAUNCH(launch_apply1_vjp(g->dev, T_pre, nodes, count, gates, T_bar, gates_bar, S(stream)));
   return DQ_OK;
 }
 

@@ -26,7 +26,7 @@ __host__ __device__ inline long long edge_svd_ws_bytes(int chi) {
 template <typename R>
 __global__ void __launch_bounds__(256)
     k_edge_svd(DevGraph g, const cx<R>* n2e, double eps, cd* X, double* lam, char* ws, long long wsb,
-               const int* inst_active) {
+               const int* inst_active, char* tape) {
   const int e = blockIdx.x;
   if (inst_active && !inst_active[g.edge_inst[e]]) return;
   const int n = g.chi;
@@ -102,6 +102,28 @@ __global__ void __launch_bounds__(256)
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
     const double lj = nrm > 0.0 ? safe_truncate(sigma[j] / nrm, eps) : 0.0;
     lam[(long long)e * n + j] = lj;
+  }
+  if (tape) {
+    // eigenvectors / eigenvalues of both messages, sorted U and V (row-major),
+    // singular values and the two norms, for the adjoint
+    EdgeSvdTape tp(tape + (long long)e * edge_svd_tape_bytes(n), n);
+    for (int q = threadIdx.x; q < cc; q += blockDim.x) {
+      const int i = q / n, j = q % n;
+      tp.Qf()[q] = Vf[q];
+      tp.Qb()[q] = Vb[q];
+      tp.U()[q] = U[(long long)j * n + i];
+      tp.V()[q] = V[(long long)perm[j] * n + i];
+    }
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      tp.wf()[j] = Mf[j * n + j].re;
+      tp.wb()[j] = Mb[j * n + j].re;
+      tp.sigma()[j] = sigma[j];
+    }
+    if (threadIdx.x == 0) {
+      tp.scal()[0] = nrm;
+      tp.scal()[1] = nu;
+      tp.scal()[2] = nv;
+    }
   }
   cd* Xf = X + (2LL * e) * cc;
   cd* Xb = X + (2LL * e + 1) * cc;
@@ -251,14 +273,14 @@ __global__ void __launch_bounds__(256)
 long long edge_svd_ws_bytes_host(int chi) { return edge_svd_ws_bytes(chi); }
 
 hipError_t launch_edge_svd(int rb, const DevGraph& g, const void* n2e, double eps, cd* X, double* lam, void* ws,
-                           long long wsb, const int* inst_active, hipStream_t s) {
+                           long long wsb, const int* inst_active, void* tape, hipStream_t s) {
   if (g.n_edges <= 0) return hipSuccess;
   if (rb == 4)
     hipLaunchKernelGGL(k_edge_svd<float>, dim3(g.n_edges), dim3(256), 0, s, g, (const cx<float>*)n2e, eps, X, lam,
-                       (char*)ws, wsb, inst_active);
+                       (char*)ws, wsb, inst_active, (char*)tape);
   else
     hipLaunchKernelGGL(k_edge_svd<double>, dim3(g.n_edges), dim3(256), 0, s, g, (const cx<double>*)n2e, eps, X,
-                       lam, (char*)ws, wsb, inst_active);
+                       lam, (char*)ws, wsb, inst_active, (char*)tape);
   return hipGetLastError();
 }
 

@@ -47,8 +47,29 @@ hipError_t launch_messages_distance(int real_bytes, const DevGraph& g, const int
 
 // Vidal gauge, edge part: orthogonalisers X[e][s] (complex double, chi x chi)
 // and lambda[e] from the node->edge messages.
+// Per-edge record of the Vidal edge decomposition for the adjoint:
+// eigenvectors (row-major, column i = vector i) and eigenvalues of both
+// messages, sorted U and V (row-major), singular values, and the scalars
+// (singular-value norm, masked norms of U and V).
+struct EdgeSvdTape {
+  char* base;
+  int n;
+  __host__ __device__ EdgeSvdTape(char* b, int nn) : base(b), n(nn) {}
+  __host__ __device__ cd* Qf() const { return (cd*)base; }
+  __host__ __device__ cd* Qb() const { return Qf() + (long long)n * n; }
+  __host__ __device__ cd* U() const { return Qb() + (long long)n * n; }
+  __host__ __device__ cd* V() const { return U() + (long long)n * n; }
+  __host__ __device__ double* wf() const { return (double*)(V() + (long long)n * n); }
+  __host__ __device__ double* wb() const { return wf() + n; }
+  __host__ __device__ double* sigma() const { return wb() + n; }
+  __host__ __device__ double* scal() const { return sigma() + n; }
+};
+__host__ __device__ inline long long edge_svd_tape_bytes(int chi) {
+  const long long n = chi;
+  return (4 * n * n * 16 + 3 * n * 8 + 32 + 255) / 256 * 256;
+}
 hipError_t launch_edge_svd(int real_bytes, const DevGraph& g, const void* n2e, double eps, cd* X, double* lam,
-                           void* ws, long long ws_bytes_per_block, const int* inst_active, hipStream_t s);
+                           void* ws, long long ws_bytes_per_block, const int* inst_active, void* tape, hipStream_t s);
 
 long long edge_svd_ws_bytes_host(int chi);
 
@@ -149,6 +170,29 @@ hipError_t launch_su_vjp(const DevGraph& g, const void* Tpre, const double* lam,
                          const cd* halves, int max_gate_rank, double eps, const void* tape, void* Tbar,
                          double* lambar, void* hbar, void* ws, long long wsb, hipStream_t s);
 long long su_vjp_ws_bytes_host(int chi, int phys, int max_r, int max_deg);
+
+// Adjoints of the other forward kernels (complex128, dq_adjoint.hip).
+long long bp_vjp_ws_bytes(int chi, int phys, int max_deg);
+long long absorb_vjp_ws_bytes(int chi, int phys, int max_deg);
+long long edge_svd_vjp_ws_bytes(int chi);
+long long rdm2_vjp_ws_bytes(int chi, int phys);
+int vjp_blocks(long long items);
+hipError_t launch_bp_node_vjp(const DevGraph& g, const void* T, const void* e2n, const void* obar, void* ebar,
+                              void* Tbar, void* ws, long long wsb, hipStream_t s);
+hipError_t launch_absorb_vjp(const DevGraph& g, const void* T, const void* X, const void* gbar, void* Tbar, void* Xbar,
+                             void* ws, long long wsb, hipStream_t s);
+hipError_t launch_edge_svd_vjp(const DevGraph& g, const void* tape, double eps, const void* Xbar, const double* lambar,
+                               void* n2e_bar, void* ws, long long wsb, hipStream_t s);
+hipError_t launch_symmetric_vjp(const DevGraph& g, const void* gam, const double* lam, double eps, const void* tbar,
+                                void* gam_bar, double* lam_bar, hipStream_t s);
+hipError_t launch_lam2msg_vjp(const DevGraph& g, const void* n2e_bar, const void* e2n_bar, double* lam_bar,
+                              hipStream_t s);
+hipError_t launch_rdm1_vjp(const DevGraph& g, const void* T, const double* lam, const int* nodes, int count,
+                           const void* rho_bar, void* Tbar, double* lam_bar, hipStream_t s);
+hipError_t launch_rdm2_vjp(const DevGraph& g, const void* T, const double* lam, const int* edges, int count,
+                           const void* rho_bar, void* Tbar, double* lam_bar, void* ws, long long wsb, hipStream_t s);
+hipError_t launch_apply1_vjp(const DevGraph& g, const void* Tpre, const int* nodes, int count, const void* gates,
+                             void* Tbar, void* gbar, hipStream_t s);
 
 // Per-instance convergence bookkeeping of the BP-to-Vidal loop.
 hipError_t launch_update_active(const double* dist, double acc, int* iters, int max_iters, int* active, int n_inst,

@@ -183,6 +183,52 @@ int dq_simple_update_vjp(dq_graph* g, int dtype, const void* T_pre, const double
                          int count, const void* halves, int max_gate_rank, double eps, const void* tape, void* T_bar,
                          double* lam_bar, void* halves_bar, void* stream);
 
+/* Vidal gauge with a record for the adjoint: as dq_vidal_gauge (all
+ * instances), but the orthogonalisers go to the caller's X (device complex128
+ * [2*n_edges][chi][chi], index edge*2+side) and each edge's decomposition is
+ * recorded in tape (n_edges * dq_edge_svd_tape_bytes(g) bytes). */
+long long dq_edge_svd_tape_bytes(const dq_graph* g);
+int dq_vidal_gauge_tape(dq_graph* g, int dtype, const void* tensors, const void* n2e, double eps, void* gammas,
+                        double* lam, void* X, void* tape, void* stream);
+
+/* Adjoints (DQ_C128 only; dL = Re sum conj(zbar) dz).  "_bar" outputs marked
+ * (acc) are accumulated into, the others overwritten.  None of these has a
+ * counterpart in the reference, which has no gradients; each is the adjoint
+ * of the forward entry point named.
+ *
+ * dq_bp_sweep_vjp: one synchronous sweep (dq_bp_sweep, all nodes reading
+ * e2n_in): n2e_out_bar -> e2n_in_bar, tensors_bar (acc). */
+int dq_bp_sweep_vjp(dq_graph* g, int dtype, const void* tensors, const void* e2n_in, const void* n2e_out_bar,
+                    void* e2n_in_bar, void* tensors_bar, void* stream);
+
+/* dq_vidal_gauge_tape: gammas_bar, lam_bar -> tensors_bar (acc), n2e_bar
+ * (Hermitian part; both sides of every edge).  X_bar: device scratch of the
+ * size of X. */
+int dq_vidal_gauge_vjp(dq_graph* g, int dtype, const void* tensors, const void* X, const void* tape, double eps,
+                       const void* gammas_bar, const double* lam_bar, void* X_bar, void* tensors_bar, void* n2e_bar,
+                       void* stream);
+
+/* dq_symmetric_gauge: out_bar -> gammas_bar (acc), lam_bar (acc). */
+int dq_symmetric_gauge_vjp(dq_graph* g, int dtype, const void* gammas, const double* lam, double eps,
+                           const void* out_bar, void* gammas_bar, double* lam_bar, void* stream);
+
+/* dq_lambdas_to_messages: message gradients (either may be NULL) -> lam_bar (acc). */
+int dq_lambdas_to_messages_vjp(dq_graph* g, int dtype, const void* n2e_bar, const void* e2n_bar, double* lam_bar,
+                               void* stream);
+
+/* dq_rdm1 / dq_rdm2 with the diagonal messages diag(lam) (how the emulator
+ * evaluates observables): rho_bar -> tensors_bar (acc), lam_bar (acc). */
+int dq_rdm1_vjp(dq_graph* g, int dtype, const void* tensors, const double* lam, const int* nodes, int count,
+                const void* rho_bar, void* tensors_bar, double* lam_bar, void* stream);
+int dq_rdm2_vjp(dq_graph* g, int dtype, const void* tensors, const double* lam, const int* edges, int count,
+                const void* rho_bar, void* tensors_bar, double* lam_bar, void* stream);
+
+/* dq_apply1: tensors_pre = tensors before the gates; tensors_bar turned in
+ * place from the gradient after to the gradient before; gates_bar
+ * [count][phys][phys]. */
+int dq_apply1_vjp(dq_graph* g, int dtype, const void* tensors_pre, const int* nodes, int count, const void* gates,
+                  void* tensors_bar, void* gates_bar, void* stream);
+
 /* One-site density matrices (device complex128 rho[count][phys][phys]) from
  * tensors and edge->node messages.  Replaces get_one_side_density_matrix
  * (dequantizer/src/mappings/density_matrix.py:16-37). */

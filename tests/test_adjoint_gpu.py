@@ -165,3 +165,164 @@ def test_simple_update_vjp_matches_central_differences(case):
                                  np.real(np.vdot(hb, dh)))):
             fd1 = (L_of(*args[:3]) - L_of(*args[3:])) / (2 * h)
             assert abs(fd1 - val) < 1e-5 * max(1.0, abs(fd1)), (name, trial, fd1, val)
+
+
+# --------------------------------------------------------------------------
+# whole-graph adjoints on a small random 3-regular graph
+
+
+def _graph_handle(n, chi, seed):
+    import networkx as nx
+
+    import dequantizer_amd as dq
+    from dequantizer_amd.engine import Topology
+    g = dq.TensorGraph()
+    for _ in range(n):
+        g.add_node(2)
+    for a, b in nx.random_regular_graph(3, n, seed=seed).edges:
+        g.add_edge((a, b), chi)
+    topo = Topology([list(g.get_traversal_iterator())])
+    return topo, topo.handle(chi, 2, True)
+
+
+def _psd_messages(rng, count, chi):
+    a = _cg(rng, (count, chi, chi))
+    m = a @ np.conj(np.transpose(a, (0, 2, 1)))
+    return m / np.trace(m, axis1=1, axis2=2)[:, None, None]
+
+
+def _d(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(DEV)
+
+
+def _fd_check(L_of, grads_dot, dirs, h=1e-6, tol=1e-5):
+    for d in dirs:
+        fd = (L_of(*[x + h * dx for x, dx in d]) - L_of(*[x - h * dx for x, dx in d])) / (2 * h)
+        an = grads_dot([dx for _, dx in d])
+        assert abs(fd - an) < tol * max(1.0, abs(fd)), (fd, an)
+
+
+def test_bp_sweep_vjp_matches_central_differences():
+    topo, h = _graph_handle(8, 3, 1)
+    chi, E = 3, topo.n_edges
+    rng = np.random.default_rng(3)
+    T0 = _cg(rng, h.tensor_elems)
+    e2n0 = _psd_messages(rng, 2 * E, chi)
+    W = _cg(rng, (2 * E, chi, chi))
+    L = _lib.lib()
+    st = _lib.stream_handle()
+
+    def fwd(T, e2n):
+        dT, de = _d(T), _d(e2n)
+        n_out = torch.zeros_like(de)
+        e_out = torch.zeros_like(de)
+        _lib.check(L.dq_bp_sweep(h.ptr, _lib.DQ_C128, _lib.ptr(dT), _lib.ptr(de), _lib.ptr(de), _lib.ptr(n_out),
+                                 _lib.ptr(e_out), None, st))
+        return n_out.cpu().numpy()
+
+    def L_of(T, e2n):
+        return float(np.real(np.vdot(W, fwd(T, e2n))))
+
+    dT, de = _d(T0), _d(e2n0)
+    ebar = torch.zeros_like(de)
+    Tbar = torch.zeros_like(dT)
+    dW = _d(W)
+    _lib.check(L.dq_bp_sweep_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(dT), _lib.ptr(de), _lib.ptr(dW), _lib.ptr(ebar),
+                                 _lib.ptr(Tbar), st))
+    Tb, eb = Tbar.cpu().numpy(), ebar.cpu().numpy()
+    dirs = [[(T0, _cg(rng, T0.shape)), (e2n0, _cg(rng, e2n0.shape))] for _ in range(3)]
+    _fd_check(L_of, lambda ds: float(np.real(np.vdot(Tb, ds[0]) + np.vdot(eb, ds[1]))), dirs)
+
+
+def test_vidal_gauge_to_observables_vjp_matches_central_differences():
+    """messages -> Vidal gauge (edge SVD + absorb) -> symmetric gauge ->
+    one- and two-site density matrices with diag(lambda) messages; the loss
+    sees only gauge-invariant quantities."""
+    topo, h = _graph_handle(8, 3, 2)
+    chi, E, N = 3, topo.n_edges, topo.n_nodes
+    rng = np.random.default_rng(4)
+    T0 = _cg(rng, h.tensor_elems)
+    n2e0 = _psd_messages(rng, 2 * E, chi)
+    W1 = _cg(rng, (N, 2, 2))
+    W2 = _cg(rng, (E, 4, 4))
+    wl = rng.standard_normal((E, chi))
+    L = _lib.lib()
+    st = _lib.stream_handle()
+    nodes = torch.arange(N, dtype=torch.int32, device=DEV)
+    edges = torch.arange(E, dtype=torch.int32, device=DEV)
+    tape_bytes = int(L.dq_edge_svd_tape_bytes(h.ptr)) * E
+
+    def fwd(T, n2e, keep=False):
+        dT, dn = _d(T), _d(n2e)
+        gam = torch.zeros_like(dT)
+        lam = torch.zeros(E * chi, dtype=torch.float64, device=DEV)
+        X = torch.zeros(2 * E * chi * chi, dtype=torch.complex128, device=DEV)
+        tape = torch.zeros(tape_bytes, dtype=torch.uint8, device=DEV)
+        _lib.check(L.dq_vidal_gauge_tape(h.ptr, _lib.DQ_C128, _lib.ptr(dT), _lib.ptr(dn), 1e-10, _lib.ptr(gam),
+                                         _lib.ptr(lam), _lib.ptr(X), _lib.ptr(tape), st))
+        ts = torch.zeros_like(gam)
+        _lib.check(L.dq_symmetric_gauge(h.ptr, _lib.DQ_C128, _lib.ptr(gam), _lib.ptr(lam), 1e-10, _lib.ptr(ts), st))
+        n_d = torch.zeros(2 * E * chi * chi, dtype=torch.complex128, device=DEV)
+        e_d = torch.zeros_like(n_d)
+        _lib.check(L.dq_lambdas_to_messages(h.ptr, _lib.DQ_C128, _lib.ptr(lam), _lib.ptr(n_d), _lib.ptr(e_d), st))
+        r1 = torch.zeros((N, 2, 2), dtype=torch.complex128, device=DEV)
+        r2 = torch.zeros((E, 4, 4), dtype=torch.complex128, device=DEV)
+        _lib.check(L.dq_rdm1(h.ptr, _lib.DQ_C128, _lib.ptr(ts), _lib.ptr(e_d), _lib.ptr(nodes), N, _lib.ptr(r1), st))
+        _lib.check(L.dq_rdm2(h.ptr, _lib.DQ_C128, _lib.ptr(ts), _lib.ptr(e_d), _lib.ptr(edges), E, _lib.ptr(r2), st))
+        out = (r1.cpu().numpy(), r2.cpu().numpy(), lam.cpu().numpy().reshape(E, chi))
+        return (out, (dT, gam, lam, X, tape, ts)) if keep else out
+
+    def L_of(T, n2e):
+        r1, r2, lm = fwd(T, n2e)
+        return float(np.real(np.vdot(W1, r1) + np.vdot(W2, r2)) + np.sum(wl * lm))
+
+    _, (dT, gam, lam, X, tape, ts) = fwd(T0, n2e0, keep=True)
+    ts_bar = torch.zeros_like(ts)
+    lam_bar = _d(wl.reshape(-1).copy())
+    dW1, dW2 = _d(W1), _d(W2)
+    _lib.check(L.dq_rdm1_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(ts), _lib.ptr(lam), _lib.ptr(nodes), N, _lib.ptr(dW1),
+                             _lib.ptr(ts_bar), _lib.ptr(lam_bar), st))
+    _lib.check(L.dq_rdm2_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(ts), _lib.ptr(lam), _lib.ptr(edges), E, _lib.ptr(dW2),
+                             _lib.ptr(ts_bar), _lib.ptr(lam_bar), st))
+    gam_bar = torch.zeros_like(gam)
+    _lib.check(L.dq_symmetric_gauge_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(gam), _lib.ptr(lam), 1e-10, _lib.ptr(ts_bar),
+                                        _lib.ptr(gam_bar), _lib.ptr(lam_bar), st))
+    Xbar = torch.zeros_like(X)
+    Tbar = torch.zeros_like(dT)
+    nbar = torch.zeros(2 * E * chi * chi, dtype=torch.complex128, device=DEV)
+    _lib.check(L.dq_vidal_gauge_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(dT), _lib.ptr(X), _lib.ptr(tape), 1e-10,
+                                    _lib.ptr(gam_bar), _lib.ptr(lam_bar), _lib.ptr(Xbar), _lib.ptr(Tbar),
+                                    _lib.ptr(nbar), st))
+    Tb, nb = Tbar.cpu().numpy(), nbar.cpu().numpy().reshape(n2e0.shape)
+
+    def herm(a):
+        return 0.5 * (a + np.conj(np.transpose(a, (0, 2, 1))))
+
+    dirs = [[(T0, _cg(rng, T0.shape)), (n2e0, herm(_cg(rng, n2e0.shape)) * 0.1)] for _ in range(3)]
+    _fd_check(L_of, lambda ds: float(np.real(np.vdot(Tb, ds[0]) + np.vdot(nb, ds[1]))), dirs)
+
+
+def test_apply1_vjp_matches_central_differences():
+    topo, h = _graph_handle(8, 3, 5)
+    N = topo.n_nodes
+    rng = np.random.default_rng(6)
+    T0 = _cg(rng, h.tensor_elems)
+    G0 = _cg(rng, (N, 2, 2))
+    W = _cg(rng, T0.shape)
+    L = _lib.lib()
+    st = _lib.stream_handle()
+    nodes = torch.arange(N, dtype=torch.int32, device=DEV)
+
+    def L_of(T, G):
+        dT, dG = _d(T), _d(G)  # keep every device buffer alive across the call
+        _lib.check(L.dq_apply1(h.ptr, _lib.DQ_C128, _lib.ptr(dT), _lib.ptr(nodes), N, _lib.ptr(dG), st))
+        return float(np.real(np.vdot(W, dT.cpu().numpy())))
+
+    Tbar = _d(W.copy())
+    gbar = torch.zeros((N, 2, 2), dtype=torch.complex128, device=DEV)
+    dT0, dG0 = _d(T0), _d(G0)
+    _lib.check(L.dq_apply1_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(dT0), _lib.ptr(nodes), N, _lib.ptr(dG0),
+                               _lib.ptr(Tbar), _lib.ptr(gbar), st))
+    Tb, gb = Tbar.cpu().numpy(), gbar.cpu().numpy()
+    dirs = [[(T0, _cg(rng, T0.shape)), (G0, _cg(rng, G0.shape))] for _ in range(3)]
+    _fd_check(L_of, lambda ds: float(np.real(np.vdot(Tb, ds[0]) + np.vdot(gb, ds[1]))), dirs)
