@@ -326,3 +326,37 @@ def test_apply1_vjp_matches_central_differences():
     Tb, gb = Tbar.cpu().numpy(), gbar.cpu().numpy()
     dirs = [[(T0, _cg(rng, T0.shape)), (G0, _cg(rng, G0.shape))] for _ in range(3)]
     _fd_check(L_of, lambda ds: float(np.real(np.vdot(Tb, ds[0]) + np.vdot(gb, ds[1]))), dirs)
+
+
+# --------------------------------------------------------------------------
+# end to end: dE/d(knots, J) of the annealing run
+
+
+def test_annealing_gradient_matches_central_differences():
+    import dequantizer_amd as dq
+    from dequantizer_amd.adjoint import AnnealingGradient
+    from dequantizer_amd.annealing import Annealer, get_scheduler, knot_schedule, random_regular
+    ef = random_regular(8, 3, seed=3)
+    steps, chi, sweeps = 5, 4, 6
+    runner = AnnealingGradient(ef, dq.PRNGKey(5), chi=chi, n_steps=steps, dt=0.3, n_knots=4, fixed_sweeps=sweeps)
+    knots = np.array([0.1, 0.4, 0.7, 0.95])
+    E, dk, dJ = runner.value_and_grad(knots)
+    assert abs(E - runner.value(knots)) < 1e-10
+    # the same run through the ordinary annealer (SVD gate split) gives the same energy
+    ann = Annealer([ef], [dq.PRNGKey(5)], chi, 1, 1e-5, 100, True, "bfs", dtype=torch.complex128, chi=chi,
+                   fixed_sweeps=sweeps)
+    for mix, cpl in get_scheduler(0.3, knot_schedule(knots), steps):
+        ann.step(mix, cpl)
+    assert abs(ann.finish(two_site=True)[0].bp_energy - E) < 1e-9
+    h = 1e-4
+    for j in range(len(knots)):
+        d = np.zeros_like(knots)
+        d[j] = h
+        fd = (runner.value(knots + d) - runner.value(knots - d)) / (2 * h)
+        assert abs(fd - dk[j]) < 1e-5 * max(1.0, abs(fd)), ("knot", j, fd, dk[j])
+    J0 = np.asarray(ef.coupling_amplitudes, dtype=np.float64)
+    for e in (0, 3, 7, 11):
+        d = np.zeros_like(J0)
+        d[e] = h
+        fd = (runner.value(knots, J0 + d) - runner.value(knots, J0 - d)) / (2 * h)
+        assert abs(fd - dJ[e]) < 1e-5 * max(1.0, abs(fd)), ("J", e, fd, dJ[e])
