@@ -54,6 +54,9 @@ def main():
                     help="BP sweeps per regauge with the convergence test off (fixed work); 0 = converge to 1e-5")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="run the oracle CPU leg (1) or not (0)")
     ap.add_argument("--plain-gpu", type=int, default=1, help="run the torch-eager GPU comparator (1) or not (0)")
+    ap.add_argument("--adjoint-steps", type=int, default=200,
+                    help="Trotter steps of the forward+adjoint timing (0 skips it)")
+    ap.add_argument("--knots", type=int, default=32, help="schedule knots of the forward+adjoint timing")
     ap.add_argument("--gate-order", default="colored", choices=["colored", "pairs"],
                     help="coupling-gate order: pair list sorted by edge colour, or the generator's pair order")
     args = ap.parse_args()
@@ -158,6 +161,9 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if args.adjoint_steps:
+        result["fwd_adjoint"] = adjoint_timing(ef, args)
+        print(f"bench: forward+adjoint done: {result['fwd_adjoint']['seconds']:.1f} s", file=sys.stderr, flush=True)
     if args.plain_gpu:
         result["plain_gpu"] = plain_gpu_baseline(ann, ef, schedule[args.warmup + args.steps], args)
         result["vs_plain_gpu"] = result["value"] / result["plain_gpu"]["value"]
@@ -167,6 +173,35 @@ def main():
         result["cpu_baseline"] = cpu_baseline(ann, ef, schedule[args.warmup + args.steps], args)
         result["vs_cpu_baseline"] = result["value"] / result["cpu_baseline"]["value"]
     print(json.dumps(result))
+
+
+def adjoint_timing(ef, args):
+    """Energy of the annealing run and its gradient with respect to the
+    schedule knots and every coupling (hand-written adjoint, complex128,
+    synchronous BP, fixed sweeps per regauge): wall time of one forward and
+    of one forward + adjoint on the same instance."""
+    import torch
+    from dequantizer_amd import PRNGKey
+    from dequantizer_amd.adjoint import AnnealingGradient
+    t0 = time.perf_counter()
+    run = AnnealingGradient(ef, PRNGKey(args.seed), chi=args.chi, n_steps=args.adjoint_steps, dt=args.dt,
+                            n_knots=args.knots, fixed_sweeps=args.fixed_sweeps or 10)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    knots = np.linspace(0.0, 1.0, args.knots)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e_fwd = run.value(knots)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    e, dk, dj = run.value_and_grad(knots)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return {"seconds": t2 - t1, "forward_seconds": t1 - t0, "setup_seconds": setup, "steps": args.adjoint_steps,
+            "knots": args.knots, "couplings": int(ef.edges_number), "dtype": "c128", "backward_cut": run.cut,
+            "sweeps_per_regauge": args.fixed_sweeps or 10, "energy": e, "energy_forward_only": e_fwd,
+            "grad_norm_knots": float(np.linalg.norm(dk)), "grad_norm_couplings": float(np.linalg.norm(dj)),
+            "peak_device_memory_gb": torch.cuda.max_memory_allocated() / 1e9}
 
 
 def plain_gpu_baseline(ann, ef, step_times, args, sample_sweeps=2, sample_gates=150):

@@ -138,7 +138,7 @@ __device__ void svd_vjp(const cd* U, const cd* V, const double* s, const cd* Uba
 //   G11bar = R11^-1 herm(B) R11^-H,  G12bar = R11^-1 M12 (split over both blocks).
 // w1..w4: n x n scratch (w1 receives Rt, w2 the permuted Rbar).
 __device__ void chol_vjp(const cd* R, const cd* Rbar, const int* piv, int n, cd* Gbar, cd* w1, cd* w2, cd* w3,
-                         cd* w4, int* kc_sh) {
+                         cd* w4, int* kc_sh, double cut) {
   for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
     const int i = f / n, m = f % n;
     w1[f] = R[(long long)i * n + piv[m]];
@@ -146,8 +146,11 @@ __device__ void chol_vjp(const cd* R, const cd* Rbar, const int* piv, int n, cd*
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    // pivots (diagonal of R, descending) below cut * the first are treated
+    // as rank deficiency: the adjoint through them divides by them
+    const double p0 = w1[0].re;
     int kc = 0;
-    while (kc < n && w1[(long long)kc * n + kc].re > 0.0) ++kc;
+    while (kc < n && w1[(long long)kc * n + kc].re > cut * p0) ++kc;
     *kc_sh = kc;
   }
   __syncthreads();
@@ -322,10 +325,10 @@ __device__ __forceinline__ int row_bond(int d, int slot, int chi, int row, int j
 
 __global__ void __launch_bounds__(256)
     k_su_vjp(DevGraph g, const cd* Tpre, const double* lam, const SuGate* gates, const cd* halves, int max_r,
-             double eps, const char* tape, long long tape_bytes, cd* Tbar, double* lambar, cd* hbar, char* ws,
-             long long wsb) {
+             double eps, double cut, const char* tape, long long tape_bytes, cd* Tbar, double* lambar, cd* hbar,
+             char* ws, long long wsb) {
   __shared__ double wl[2][kMaxDeg * 64];
-  __shared__ double sbar[128], lamb[128], le[64];
+  __shared__ double sbar[128], lamb[128], le[64], sgc[128];
   __shared__ double red[8];
   __shared__ double scal[8];
   __shared__ int kc_sh;
@@ -339,7 +342,14 @@ __global__ void __launch_bounds__(256)
   const cd* hh[2] = {halves + (long long)blockIdx.x * 2 * hstride, halves + (long long)blockIdx.x * 2 * hstride + hstride};
   const SuTape tp((char*)tape + (long long)blockIdx.x * tape_bytes, chi * max_r);
   const int kr = tp.meta()[0];
-  const double* sg = tp.sigma();
+  // singular values below cut * the largest are treated as zero by the adjoint
+  {
+    double ms = 0.0;
+    for (int i = 0; i < n; ++i) ms = fmax(ms, tp.sigma()[i]);
+    for (int j = threadIdx.x; j < n; j += blockDim.x) sgc[j] = tp.sigma()[j] > cut * ms ? tp.sigma()[j] : 0.0;
+  }
+  __syncthreads();
+  const double* sg = sgc;
   const cd* Ra = tp.Ra();
   const cd* Rb = tp.Rb();
   const cd* U = tp.U();
@@ -354,7 +364,7 @@ __global__ void __launch_bounds__(256)
   }
   for (int j = threadIdx.x; j < chi; j += blockDim.x) le[j] = lam[(long long)e * chi + j];
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    sbar[j] = (j < kr && sg[j] > 0.0) ? lambar[(long long)e * chi + j] : 0.0;
+    sbar[j] = (j < kr && tp.sigma()[j] > 0.0) ? lambar[(long long)e * chi + j] : 0.0;
     lamb[j] = 0.0;
   }
   __syncthreads();
@@ -541,8 +551,8 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   // Cholesky adjoints -> G-bar (Hermitian)
-  adj::chol_vjp(Ra, Rab, tp.piv_a(), n, Gba, w1, w2, w3, w4, &kc_sh);
-  adj::chol_vjp(Rb, Rbb, tp.piv_b(), n, Gbb, w1, w2, w3, w4, &kc_sh);
+  adj::chol_vjp(Ra, Rab, tp.piv_a(), n, Gba, w1, w2, w3, w4, &kc_sh, cut);
+  adj::chol_vjp(Rb, Rbb, tp.piv_b(), n, Gbb, w1, w2, w3, w4, &kc_sh, cut);
 
   // ---------------- per side: environment Gram adjoint, half gate, lambda of other bonds
   for (int side = 0; side < 2; ++side) {
@@ -654,11 +664,11 @@ __global__ void __launch_bounds__(256)
 }
 
 hipError_t launch_su_vjp(const DevGraph& g, const void* Tpre, const double* lam, const SuGate* gates, int count,
-                         const cd* halves, int max_gate_rank, double eps, const void* tape, void* Tbar,
+                         const cd* halves, int max_gate_rank, double eps, double cut, const void* tape, void* Tbar,
                          double* lambar, void* hbar, void* ws, long long wsb, hipStream_t s) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_su_vjp, dim3(count), dim3(256), 0, s, g, (const cd*)Tpre, lam, gates, halves, max_gate_rank,
-                     eps, (const char*)tape, su_tape_bytes(g.chi, max_gate_rank), (cd*)Tbar, lambar, (cd*)hbar,
+                     eps, cut, (const char*)tape, su_tape_bytes(g.chi, max_gate_rank), (cd*)Tbar, lambar, (cd*)hbar,
                      (char*)ws, wsb);
   return hipGetLastError();
 }
@@ -815,9 +825,10 @@ __device__ __forceinline__ double dd_isqrt(double a, double b, double eps) {
 // tape.  Given Xbar (orthogonalisers) and lambar writes n2e_bar for both
 // messages of every edge (Hermitian part, as the forward symmetrises).
 __global__ void __launch_bounds__(256)
-    k_edge_svd_vjp(DevGraph g, const char* tape, double eps, const cd* Xbar, const double* lambar, cd* n2e_bar,
-                   char* ws, long long wsb) {
-  __shared__ double sb[64], kap[64], lamv[64];
+    k_edge_svd_vjp(DevGraph g, const char* tape, double eps, double cut, const cd* Xbar, const double* lambar,
+                   cd* n2e_bar, char* ws, long long wsb) {
+  __shared__ double sb[64], kap[64], lamv[64], sgc[64];
+  __shared__ unsigned char frz[2][64];
   const int n = g.chi;
   const long long cc = (long long)n * n;
   for (int e = blockIdx.x; e < g.n_edges; e += gridDim.x) {
@@ -827,6 +838,24 @@ __global__ void __launch_bounds__(256)
        *Ibb = Ifb + cc, *Gb = Ibb + cc, *w1 = Gb + cc, *w2 = w1 + cc, *w3 = w2 + cc, *Sfb = w3 + cc, *Sbb = Sfb + cc;
     const double nrm = tp.scal()[0], nu = tp.scal()[1], nv = tp.scal()[2];
     const cd *Qf = tp.Qf(), *Qb = tp.Qb(), *U = tp.U(), *V = tp.V();
+    // Backward cutoff: message eigen-directions and singular values below
+    // cut * the largest on the bond are frozen (no gradient flows through
+    // them).  Through such directions the adjoint is a difference of terms
+    // of size w^-3/2 and 1/sigma that cancel in exact arithmetic only.
+    {
+      double mf = 0.0, mb = 0.0, ms = 0.0;
+      for (int i = 0; i < n; ++i) {
+        mf = fmax(mf, tp.wf()[i]);
+        mb = fmax(mb, tp.wb()[i]);
+        ms = fmax(ms, tp.sigma()[i]);
+      }
+      for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        sgc[j] = tp.sigma()[j] > cut * ms ? tp.sigma()[j] : 0.0;
+        frz[0][j] = cut > 0.0 && tp.wf()[j] < cut * mf;
+        frz[1][j] = cut > 0.0 && tp.wb()[j] < cut * mb;
+      }
+      __syncthreads();
+    }
     const double *wf = tp.wf(), *wb = tp.wb(), *sg = tp.sigma();
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
       const double l = nrm > 0.0 ? sg[j] / nrm : 0.0;
@@ -882,13 +911,14 @@ __global__ void __launch_bounds__(256)
         sb[j] = nrm > 0.0 ? kap[j] * (lambar[(long long)e * n + j] - dot * lamv[j]) / nrm : 0.0;
     }
     __syncthreads();
-    adj::svd_vjp(U, V, sg, Ub, Vb, sb, n, Gb, w1, w2, w3);
+    adj::svd_vjp(U, V, sgc, Ub, Vb, sb, n, Gb, w1, w2, w3);
     // G = Sf Sb^T:  Sf-bar = Gbar conj(Sb),  Sb-bar = Gbar^T conj(Sf)
     adj::bmm(Sfb, n, Gb, n, adj::N, Sb, n, adj::C, n, n, n, false);
     adj::bmm(Sbb, n, Gb, n, adj::T, Sf, n, adj::C, n, n, n, false);
     for (int side = 0; side < 2; ++side) {
       const cd* Q = side == 0 ? Qf : Qb;
       const double* wv = side == 0 ? wf : wb;
+      const unsigned char* fz = frz[side];
       const cd* Fs = side == 0 ? Sfb : Sbb;
       const cd* Fi = side == 0 ? Ifb : Ibb;
       // w1 = Q^H Fs Q, w2 = Q^H Fi Q
@@ -911,6 +941,7 @@ __global__ void __launch_bounds__(256)
             dg = wv[i] < eps ? 0.0 : -0.5 / (wv[i] * sqrt(wv[i]));
           }
         }
+        if (fz[i] || fz[j]) df = dg = 0.0;
         w3[q] = df * w1[q] + dg * w2[q];
       }
       __syncthreads();
@@ -1284,10 +1315,10 @@ hipError_t launch_absorb_vjp(const DevGraph& g, const void* T, const void* X, co
                      (const cd*)gbar, (cd*)Tbar, (cd*)Xbar, (char*)ws, wsb);
   return hipGetLastError();
 }
-hipError_t launch_edge_svd_vjp(const DevGraph& g, const void* tape, double eps, const void* Xbar, const double* lambar,
-                               void* n2e_bar, void* ws, long long wsb, hipStream_t s) {
+hipError_t launch_edge_svd_vjp(const DevGraph& g, const void* tape, double eps, double cut, const void* Xbar,
+                               const double* lambar, void* n2e_bar, void* ws, long long wsb, hipStream_t s) {
   if (g.n_edges <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_edge_svd_vjp, dim3(vjp_grid(g.n_edges)), dim3(256), 0, s, g, (const char*)tape, eps,
+  hipLaunchKernelGGL(k_edge_svd_vjp, dim3(vjp_grid(g.n_edges)), dim3(256), 0, s, g, (const char*)tape, eps, cut,
                      (const cd*)Xbar, lambar, (cd*)n2e_bar, (char*)ws, wsb);
   return hipGetLastError();
 }

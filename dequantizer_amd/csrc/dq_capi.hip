@@ -378,9 +378,10 @@ int dq_apply1(dq_graph* g, int dtype, void* T, const int* nodes, int count, cons
 }
 
 int dq_simple_update_vjp(dq_graph* g, int dtype, const void* T_pre, const double* lam_pre, const int* gates,
-                         int count, const void* halves, int max_gate_rank, double eps, const void* tape, void* T_bar,
-                         double* lam_bar, void* halves_bar, void* stream) {
+                         int count, const void* halves, int max_gate_rank, double eps, double cut, const void* tape,
+                         void* T_bar, double* lam_bar, void* halves_bar, void* stream) {
   if (!g || count < 0) return fail(DQ_ERR_INVALID, "bad argument");
+  if (!(cut >= 0.0 && cut < 1.0)) return fail(DQ_ERR_INVALID, "cut %g outside [0, 1)", cut);
   if (dtype != DQ_C128) return fail(DQ_ERR_UNSUPPORTED, "the adjoint runs in complex128 only");
   if (max_gate_rank < 1 || max_gate_rank > g->dev.phys * g->dev.phys)
     return fail(DQ_ERR_INVALID, "max_gate_rank %d outside [1, phys^2]", max_gate_rank);
@@ -395,7 +396,7 @@ int dq_simple_update_vjp(dq_graph* g, int dtype, const void* T_pre, const double
   int rc;
   if ((rc = ws_ensure(g, (size_t)wsb * count)) != DQ_OK) return rc;
   RET_LAUNCH(launch_su_vjp(g->dev, T_pre, lam_pre, (const SuGate*)gates, count, (const cd*)halves, max_gate_rank, eps,
-                           tape, T_bar, lam_bar, halves_bar, g->ws, wsb, S(stream)));
+                           cut, tape, T_bar, lam_bar, halves_bar, g->ws, wsb, S(stream)));
   return DQ_OK;
 }
 
@@ -415,9 +416,10 @@ int dq_bp_sweep_vjp(dq_graph* g, int dtype, const void* T, const void* e2n_in, c
 }
 
 int dq_vidal_gauge_vjp(dq_graph* g, int dtype, const void* T, const void* X, const void* tape, double eps,
-                       const void* gammas_bar, const double* lam_bar, void* X_bar, void* T_bar, void* n2e_bar,
-                       void* stream) {
+                       double cut, const void* gammas_bar, const double* lam_bar, void* X_bar, void* T_bar,
+                       void* n2e_bar, void* stream) {
   int rc;
+  if (!(cut >= 0.0 && cut < 1.0)) return fail(DQ_ERR_INVALID, "cut %g outside [0, 1)", cut);
   if (!g || !T || !X || !tape || !gammas_bar || !lam_bar || !X_bar || !T_bar || !n2e_bar)
     return fail(DQ_ERR_INVALID, "NULL argument");
   if ((rc = need_c128(dtype)) != DQ_OK) return rc;
@@ -426,7 +428,7 @@ int dq_vidal_gauge_vjp(dq_graph* g, int dtype, const void* T, const void* X, con
   const size_t na = (size_t)wa * vjp_blocks(g->dev.n_nodes), ne = (size_t)we * vjp_blocks(g->dev.n_edges);
   if ((rc = ws_ensure(g, na > ne ? na : ne)) != DQ_OK) return rc;
   RET_LAUNCH(launch_absorb_vjp(g->dev, T, X, gammas_bar, T_bar, X_bar, g->ws, wa, S(stream)));
-  RET_LAUNCH(launch_edge_svd_vjp(g->dev, tape, eps, X_bar, lam_bar, n2e_bar, g->ws, we, S(stream)));
+  RET_LAUNCH(launch_edge_svd_vjp(g->dev, tape, eps, cut, X_bar, lam_bar, n2e_bar, g->ws, we, S(stream)));
   return DQ_OK;
 }
 

@@ -167,7 +167,7 @@ hipError_t launch_decompose_gates(const cd* gates, int count, int p, double eps,
 // gradients of the state after the launch, turned in place into those of
 // the state before it; hbar: per-gate gradients of the two halves.
 hipError_t launch_su_vjp(const DevGraph& g, const void* Tpre, const double* lam, const SuGate* gates, int count,
-                         const cd* halves, int max_gate_rank, double eps, const void* tape, void* Tbar,
+                         const cd* halves, int max_gate_rank, double eps, double cut, const void* tape, void* Tbar,
                          double* lambar, void* hbar, void* ws, long long wsb, hipStream_t s);
 long long su_vjp_ws_bytes_host(int chi, int phys, int max_r, int max_deg);
 
@@ -181,8 +181,8 @@ hipError_t launch_bp_node_vjp(const DevGraph& g, const void* T, const void* e2n,
                               void* Tbar, void* ws, long long wsb, hipStream_t s);
 hipError_t launch_absorb_vjp(const DevGraph& g, const void* T, const void* X, const void* gbar, void* Tbar, void* Xbar,
                              void* ws, long long wsb, hipStream_t s);
-hipError_t launch_edge_svd_vjp(const DevGraph& g, const void* tape, double eps, const void* Xbar, const double* lambar,
-                               void* n2e_bar, void* ws, long long wsb, hipStream_t s);
+hipError_t launch_edge_svd_vjp(const DevGraph& g, const void* tape, double eps, double cut, const void* Xbar,
+                               const double* lambar, void* n2e_bar, void* ws, long long wsb, hipStream_t s);
 hipError_t launch_symmetric_vjp(const DevGraph& g, const void* gam, const double* lam, double eps, const void* tbar,
                                 void* gam_bar, double* lam_bar, hipStream_t s);
 hipError_t launch_lam2msg_vjp(const DevGraph& g, const void* n2e_bar, const void* e2n_bar, double* lam_bar,

@@ -176,12 +176,14 @@ int dq_simple_update_tape(dq_graph* g, int dtype, void* tensors, double* lam, co
  * respect to the state after the launch on entry, with respect to the state
  * before it on return (only the gates' nodes and bonds change); halves_bar
  * (device complex128 [count][2][phys^4]): gradients of the gate halves.
- * Gradients follow dL = Re sum conj(zbar) dz.  The adjoint of the
+ * Gradients follow dL = Re sum conj(zbar) dz.  cut in [0, 1): singular
+ * values and Cholesky pivots below cut * the largest of their gate are
+ * frozen (no gradient through them; 0 = the exact adjoint).  The adjoint of the
  * reference's simple_update (dequantizer/src/tensor_ops.py:318-389), which
  * has none. */
 int dq_simple_update_vjp(dq_graph* g, int dtype, const void* T_pre, const double* lam_pre, const int* gates,
-                         int count, const void* halves, int max_gate_rank, double eps, const void* tape, void* T_bar,
-                         double* lam_bar, void* halves_bar, void* stream);
+                         int count, const void* halves, int max_gate_rank, double eps, double cut, const void* tape,
+                         void* T_bar, double* lam_bar, void* halves_bar, void* stream);
 
 /* Vidal gauge with a record for the adjoint: as dq_vidal_gauge (all
  * instances), but the orthogonalisers go to the caller's X (device complex128
@@ -203,10 +205,11 @@ int dq_bp_sweep_vjp(dq_graph* g, int dtype, const void* tensors, const void* e2n
 
 /* dq_vidal_gauge_tape: gammas_bar, lam_bar -> tensors_bar (acc), n2e_bar
  * (Hermitian part; both sides of every edge).  X_bar: device scratch of the
- * size of X. */
+ * size of X.  cut in [0, 1): message eigen-directions and singular values
+ * below cut * the largest on their edge are frozen (0 = the exact adjoint). */
 int dq_vidal_gauge_vjp(dq_graph* g, int dtype, const void* tensors, const void* X, const void* tape, double eps,
-                       const void* gammas_bar, const double* lam_bar, void* X_bar, void* tensors_bar, void* n2e_bar,
-                       void* stream);
+                       double cut, const void* gammas_bar, const double* lam_bar, void* X_bar, void* tensors_bar,
+                       void* n2e_bar, void* stream);
 
 /* dq_symmetric_gauge: out_bar -> gammas_bar (acc), lam_bar (acc). */
 int dq_symmetric_gauge_vjp(dq_graph* g, int dtype, const void* gammas, const double* lam, double eps,

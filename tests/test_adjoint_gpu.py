@@ -111,7 +111,7 @@ class _SuCase:
         gl_d = torch.from_numpy(glam.reshape(-1).copy()).to(DEV)
         hb = torch.zeros_like(dh)
         _lib.check(L.dq_simple_update_vjp(self.h.ptr, _lib.DQ_C128, _lib.ptr(dT), _lib.ptr(dl), _lib.ptr(dm), 1,
-                                          _lib.ptr(dh), self.r, 1e-10, _lib.ptr(tape), _lib.ptr(gT_d), _lib.ptr(gl_d),
+                                          _lib.ptr(dh), self.r, 1e-10, 0.0, _lib.ptr(tape), _lib.ptr(gT_d), _lib.ptr(gl_d),
                                           _lib.ptr(hb), _lib.stream_handle()))
         return gT_d.cpu().numpy(), gl_d.cpu().numpy().reshape(lam.shape), hb.cpu().numpy()
 
@@ -290,7 +290,7 @@ def test_vidal_gauge_to_observables_vjp_matches_central_differences():
     Xbar = torch.zeros_like(X)
     Tbar = torch.zeros_like(dT)
     nbar = torch.zeros(2 * E * chi * chi, dtype=torch.complex128, device=DEV)
-    _lib.check(L.dq_vidal_gauge_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(dT), _lib.ptr(X), _lib.ptr(tape), 1e-10,
+    _lib.check(L.dq_vidal_gauge_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(dT), _lib.ptr(X), _lib.ptr(tape), 1e-10, 0.0,
                                     _lib.ptr(gam_bar), _lib.ptr(lam_bar), _lib.ptr(Xbar), _lib.ptr(Tbar),
                                     _lib.ptr(nbar), st))
     Tb, nb = Tbar.cpu().numpy(), nbar.cpu().numpy().reshape(n2e0.shape)
@@ -360,3 +360,56 @@ def test_annealing_gradient_matches_central_differences():
         d[e] = h
         fd = (runner.value(knots, J0 + d) - runner.value(knots, J0 - d)) / (2 * h)
         assert abs(fd - dJ[e]) < 1e-5 * max(1.0, abs(fd)), ("J", e, fd, dJ[e])
+
+
+def test_annealing_gradient_taylor_remainder():
+    """E(x + h v) - E(x) - h <grad, v> = O(h^2) along a random direction v of
+    (knots, J): halving h divides the remainder by about 4.  Three steps at
+    chi=8 keep every bond below the cap (a gate at most doubles its bond), so
+    no truncation kink lies on the path."""
+    import dequantizer_amd as dq
+    from dequantizer_amd.adjoint import AnnealingGradient
+    from dequantizer_amd.annealing import random_regular
+    ef = random_regular(10, 3, seed=11)
+    runner = AnnealingGradient(ef, dq.PRNGKey(2), chi=8, n_steps=3, dt=0.3, n_knots=3, fixed_sweeps=6)
+    knots = np.array([0.2, 0.5, 0.9])
+    J0 = np.asarray(ef.coupling_amplitudes, dtype=np.float64)
+    E, dk, dJ = runner.value_and_grad(knots)
+    rng = np.random.default_rng(0)
+    vk, vJ = rng.standard_normal(knots.shape), rng.standard_normal(J0.shape)
+    slope = float(dk @ vk + dJ @ vJ)
+    rem = [abs(runner.value(knots + h * vk, J0 + h * vJ) - E - h * slope) for h in (2e-2, 1e-2, 5e-3)]
+    assert rem[0] > 1e-9, rem  # the second-order term is above rounding
+    for a, b in zip(rem, rem[1:]):
+        assert 3.0 < a / b < 5.0, rem
+    # a gradient off by 10% leaves an O(h) remainder instead
+    bad = [abs(runner.value(knots + h * vk, J0 + h * vJ) - E - h * 1.1 * slope) for h in (1e-3, 5e-4)]
+    assert bad[0] / bad[1] < 3.0, bad
+
+
+def test_annealing_gradient_full_size():
+    """1000-spin random 3-regular instance (seed 7), chi=16, 200 steps of
+    dt=0.1, 32 knots at s_k = t_k: gradients with respect to knots and
+    couplings against central differences (step 1e-4).  Mid and late knots
+    and the couplings match to rounding; knot 3 (steps 13-25, many Schmidt
+    values below 1e-6) is checked to 2e-3, the resolution of the difference
+    quotient there (it moves by ~1e-3 between steps 1e-4 and 1e-5)."""
+    import dequantizer_amd as dq
+    from dequantizer_amd.adjoint import AnnealingGradient
+    from dequantizer_amd.annealing import edge_colored, random_regular
+    ef = edge_colored(random_regular(1000, 3, seed=7, couplings="pm1", field_std=0.1))
+    runner = AnnealingGradient(ef, dq.PRNGKey(7), chi=16, n_steps=200, dt=0.1, n_knots=32, fixed_sweeps=10)
+    knots = np.linspace(0.0, 1.0, 32)
+    J0 = np.asarray(ef.coupling_amplitudes, dtype=np.float64)
+    E, dk, dJ = runner.value_and_grad(knots)
+    assert np.all(np.isfinite(dk)) and np.all(np.isfinite(dJ))
+    h = 1e-4
+    for j, tol in ((16, 1e-5), (3, 2e-3)):
+        d = np.zeros(32)
+        d[j] = h
+        fd = (runner.value(knots + d) - runner.value(knots - d)) / (2 * h)
+        assert abs(fd - dk[j]) < tol * max(1.0, abs(fd)), ("knot", j, fd, dk[j])
+    d = np.zeros_like(J0)
+    d[0] = h
+    fd = (runner.value(knots, J0 + d) - runner.value(knots, J0 - d)) / (2 * h)
+    assert abs(fd - dJ[0]) < 1e-4 * max(1.0, abs(fd)), ("J", 0, fd, dJ[0])
