@@ -6,14 +6,16 @@
 // U = T x_{j != idx} M_j (message j applied on the ket side), out[k,l] =
 // sum conj(T)[..k..] U[..l..] over the other axes, out /= trace(out).
 //
-// One 256-thread block (4 waves) per node.  T (16x16x16x2 complex) and one
-// work tensor live in LDS with a padded layout: element (b0, b1, b2, s) at
-// b0*529 + b1*33 + b2*2 + s (complex), so that the 16 lanes of an MFMA
-// operand fetch that walk along any bond axis hit distinct banks.
+// One 512-thread block (8 waves, two per SIMD so that one wave's LDS
+// fetches overlap the other's MFMAs) per node.  T (16x16x16x2 complex) and
+// one work tensor live in LDS with a padded layout: element (b0, b1, b2, s)
+// at b0*529 + b1*33 + b2*2 + s (complex), so that the 16 lanes of an MFMA
+// operand fetch that walk along any bond axis hit distinct banks.  The two
+// tensors take 135 KiB of the CU's 160 KiB, so one block runs per CU.
 //   mode product  X x_j M: a GEMM (16 x 16) x (16 x 512) split in 32 column
 //                 tiles; per tile 4 k-steps of V_MFMA_F32_16X16X4_F32, each
 //                 complex product = 4 real MFMAs;
-//   Gram          (16 x 512) x (512 x 16), K split over the 4 waves and
+//   Gram          (16 x 512) x (512 x 16), K split over the 8 waves and
 //                 reduced through LDS.
 // Per node: 6 mode products + 3 Grams (U_0 = (T x_2 M2) x_1 M1,
 // U_1 = (T x_2 M2) x_0 M0, U_2 = (T x_0 M0) x_1 M1, the second product of each
@@ -29,6 +31,7 @@ constexpr int RP = 33;              // pitch of a (b0, b1) fiber of 32 complex v
 constexpr int B0P = 16 * RP + 1;    // pitch of b0
 constexpr int NPAD = 16 * B0P;      // padded complex elements of one tensor
 constexpr int S = 16 * 16 * 16 * 2;  // unpadded complex elements
+constexpr int NW = 8;                // waves per BP block
 
 __device__ __forceinline__ int axis_stride(int j) { return j == 0 ? B0P : (j == 1 ? RP : 2); }
 
@@ -45,11 +48,12 @@ __device__ __forceinline__ floatx4 mfma(float a, float b, floatx4 c) {
 }
 
 // dst = src x_j M (may be in place).  m[ks] = M[lane%16][4*ks + lane/16].
+template <int W>
 __device__ __forceinline__ void mode_product(float2* dst, const float2* src, const float2 (&m)[4], int j, int wave,
                                              int lane) {
   const int st = axis_stride(j);
   const int col = lane & 15, kq = lane >> 4;
-  for (int tile = wave; tile < 32; tile += 4) {
+  for (int tile = wave; tile < 32; tile += W) {
     const int cb = col_base(j, tile * 16 + col);
     floatx4 ar = {0.f, 0.f, 0.f, 0.f}, ai = {0.f, 0.f, 0.f, 0.f};
     float2 x[4];
@@ -67,14 +71,15 @@ __device__ __forceinline__ void mode_product(float2* dst, const float2* src, con
   }
 }
 
-// Partial Gram of this wave: sum over its 128 columns of conj(T[k]) U[l].
+// Partial Gram of this wave: sum over its 512/NW columns of conj(T[k]) U[l].
 __device__ __forceinline__ void gram_partial(const float2* t, const float2* u, int idx, int wave, int lane,
                                              floatx4& gr, floatx4& gi) {
   const int st = axis_stride(idx);
   const int row = lane & 15, kq = lane >> 4;
+  constexpr int per = 128 / NW;
   gr = floatx4{0.f, 0.f, 0.f, 0.f};
   gi = floatx4{0.f, 0.f, 0.f, 0.f};
-  for (int ks = wave * 32; ks < wave * 32 + 32; ++ks) {
+  for (int ks = wave * per; ks < wave * per + per; ++ks) {
     const int off = col_base(idx, 4 * ks + kq) + row * st;
     const float2 a = t[off], b = u[off];
     // conj(a) * b
@@ -86,20 +91,20 @@ __device__ __forceinline__ void gram_partial(const float2* t, const float2* u, i
 }
 }  // namespace fast3
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64 * fast3::NW)
     k_bp_nodes_d3c16(DevGraph g, const int* nodes, const float2* T, const float2* n2e_old, const float2* e2n_old,
                      float2* n2e_new, const uint8_t* node_src, const int* inst_active) {
   using namespace fast3;
   __shared__ float2 tl[NPAD];
   __shared__ float2 wk[NPAD];
-  __shared__ float2 part[4][256];
+  __shared__ float2 part[NW][256];
   const int node = nodes[blockIdx.x];
   if (inst_active && !inst_active[g.node_inst[node]]) return;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int base = g.adj_ptr[node];
   // stage T into LDS (padded)
   const float2* tg = T + g.toff[node];
-  for (int f = tid; f < S; f += 256) {
+  for (int f = tid; f < S; f += 64 * NW) {
     const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
     tl[b0 * B0P + b1 * RP + r] = tg[f];
   }
@@ -123,39 +128,102 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int step = 0; step < 3; ++step) {
     const int j1 = plan[step][0], j2 = plan[step][1], idx = plan[step][2];
-    mode_product(wk, tl, m[j1], j1, wave, lane);
+    mode_product<NW>(wk, tl, m[j1], j1, wave, lane);
     __syncthreads();
-    mode_product(wk, wk, m[j2], j2, wave, lane);
+    mode_product<NW>(wk, wk, m[j2], j2, wave, lane);
     __syncthreads();
     floatx4 gr, gi;
     gram_partial(tl, wk, idx, wave, lane, gr, gi);
 #pragma unroll
     for (int v = 0; v < 4; ++v) part[wave][(4 * (lane >> 4) + v) * 16 + (lane & 15)] = make_float2(gr[v], gi[v]);
     __syncthreads();
-    float2 sum = part[0][tid];
-    for (int w = 1; w < 4; ++w) {
-      sum.x += part[w][tid].x;
-      sum.y += part[w][tid].y;
-    }
-    float tr_r = 0.f, tr_i = 0.f;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const int q = k * 17;
-      float2 d = part[0][q];
-      for (int w = 1; w < 4; ++w) {
-        d.x += part[w][q].x;
-        d.y += part[w][q].y;
+    if (tid < 256) {
+      float2 sum = part[0][tid];
+      for (int w = 1; w < NW; ++w) {
+        sum.x += part[w][tid].x;
+        sum.y += part[w][tid].y;
       }
-      tr_r += d.x;
-      tr_i += d.y;
+      float tr_r = 0.f, tr_i = 0.f;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int q = k * 17;
+        float2 d = part[0][q];
+        for (int w = 1; w < NW; ++w) {
+          d.x += part[w][q].x;
+          d.y += part[w][q].y;
+        }
+        tr_r += d.x;
+        tr_i += d.y;
+      }
+      const float den = tr_r * tr_r + tr_i * tr_i;
+      float2 o;
+      o.x = (sum.x * tr_r + sum.y * tr_i) / den;
+      o.y = (sum.y * tr_r - sum.x * tr_i) / den;
+      n2e_new[(2LL * oe[idx] + os[idx]) * 256 + tid] = o;
     }
-    const float den = tr_r * tr_r + tr_i * tr_i;
-    float2 o;
-    o.x = (sum.x * tr_r + sum.y * tr_i) / den;
-    o.y = (sum.y * tr_r - sum.x * tr_i) / den;
-    n2e_new[(2LL * oe[idx] + os[idx]) * 256 + tid] = o;
     __syncthreads();
   }
+}
+
+// Vidal absorb (tensor_ops.py:88-93) on matrix cores for the same shape:
+// Gamma = T x_0 X_0^T x_1 X_1^T x_2 X_2^T / |.|, X_j the orthogonaliser of the
+// node's side of bond j (complex128, rounded to complex64 operands, FP32
+// accumulation like the generic complex64 kernel).  One 256-thread block per
+// node with a single padded tensor in LDS (68 KiB, two blocks per CU): the
+// three mode products run in place (each column tile reads and rewrites only
+// its own columns).
+__global__ void __launch_bounds__(256)
+    k_absorb_d3c16(DevGraph g, const float2* T, const cd* X, float2* gam, const int* inst_active) {
+  using namespace fast3;
+  __shared__ float2 tl[NPAD];
+  __shared__ double red[8];
+  const int node = blockIdx.x;
+  if (inst_active && !inst_active[g.node_inst[node]]) return;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int base = g.adj_ptr[node];
+  const float2* tg = T + g.toff[node];
+  for (int f = tid; f < S; f += 256) {
+    const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
+    tl[b0 * B0P + b1 * RP + r] = tg[f];
+  }
+  // m[j][ks] = X_j^T[lane%16][4 ks + lane/16] = X_j[4 ks + lane/16][lane%16]
+  float2 m[3][4];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const cd* x = X + (2LL * g.adj_edge[base + k] + g.adj_side[base + k]) * 256;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const cd v = x[(4 * ks + (lane >> 4)) * 16 + (lane & 15)];
+      m[k][ks] = make_float2((float)v.re, (float)v.im);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    mode_product<4>(tl, tl, m[j], j, wave, lane);
+    __syncthreads();
+  }
+  double acc = 0.0;
+  for (int f = tid; f < S; f += 256) {
+    const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
+    const float2 v = tl[b0 * B0P + b1 * RP + r];
+    acc += (double)(v.x * v.x + v.y * v.y);
+  }
+  const float inv = (float)(1.0 / sqrt(block_sum(acc, red)));
+  float2* out = gam + g.toff[node];
+  for (int f = tid; f < S; f += 256) {
+    const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
+    const float2 v = tl[b0 * B0P + b1 * RP + r];
+    out[f] = make_float2(inv * v.x, inv * v.y);
+  }
+}
+
+hipError_t launch_absorb_fast(const DevGraph& g, const void* T, const cd* X, void* gam, const int* inst_active,
+                              hipStream_t s) {
+  if (g.n_nodes <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_absorb_d3c16, dim3(g.n_nodes), dim3(256), 0, s, g, (const float2*)T, X, (float2*)gam,
+                     inst_active);
+  return hipGetLastError();
 }
 
 bool bp_fast_applicable(int rb, const DevGraph& g, int min_deg, int max_deg) {
@@ -166,7 +234,7 @@ hipError_t launch_bp_nodes_fast(const DevGraph& g, const int* nodes, int count, 
                                 const void* e2n_old, void* n2e_new, const uint8_t* node_src, const int* inst_active,
                                 hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_bp_nodes_d3c16, dim3(count), dim3(256), 0, s, g, nodes, (const float2*)T,
+  hipLaunchKernelGGL(k_bp_nodes_d3c16, dim3(count), dim3(64 * fast3::NW), 0, s, g, nodes, (const float2*)T,
                      (const float2*)n2e_old, (const float2*)e2n_old, (float2*)n2e_new, node_src, inst_active);
   return hipGetLastError();
 }

@@ -317,7 +317,10 @@ int dq_vidal_gauge(dq_graph* g, int dtype, const void* T, const void* n2e, doubl
   const size_t need_n = (size_t)wsn * (g->dev.n_nodes > 0 ? g->dev.n_nodes : 1);
   if ((rc = ws_ensure(g, need_e > need_n ? need_e : need_n)) != DQ_OK) return rc;
   RET_LAUNCH(launch_edge_svd(rb, g->dev, n2e, eps, g->X, lam, g->ws, wse, inst_active, nullptr, S(stream)));
-  RET_LAUNCH(launch_absorb(rb, g->dev, T, g->X, gammas, g->ws, wsn, inst_active, S(stream)));
+  if (bp_fast_applicable(rb, g->dev, g->min_deg, g->max_deg))
+    RET_LAUNCH(launch_absorb_fast(g->dev, T, g->X, gammas, inst_active, S(stream)));
+  else
+    RET_LAUNCH(launch_absorb(rb, g->dev, T, g->X, gammas, g->ws, wsn, inst_active, S(stream)));
   return DQ_OK;
 }
 
@@ -335,7 +338,10 @@ int dq_vidal_gauge_tape(dq_graph* g, int dtype, const void* T, const void* n2e, 
   const size_t need_n = (size_t)wsn * (g->dev.n_nodes > 0 ? g->dev.n_nodes : 1);
   if ((rc = ws_ensure(g, need_e > need_n ? need_e : need_n)) != DQ_OK) return rc;
   RET_LAUNCH(launch_edge_svd(rb, g->dev, n2e, eps, (cd*)X, lam, g->ws, wse, nullptr, tape, S(stream)));
-  RET_LAUNCH(launch_absorb(rb, g->dev, T, (const cd*)X, gammas, g->ws, wsn, nullptr, S(stream)));
+  if (bp_fast_applicable(rb, g->dev, g->min_deg, g->max_deg))
+    RET_LAUNCH(launch_absorb_fast(g->dev, T, (const cd*)X, gammas, nullptr, S(stream)));
+  else
+    RET_LAUNCH(launch_absorb(rb, g->dev, T, (const cd*)X, gammas, g->ws, wsn, nullptr, S(stream)));
   return DQ_OK;
 }
 
