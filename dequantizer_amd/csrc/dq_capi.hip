@@ -534,8 +534,7 @@ static int simple_update_impl(dq_graph* g, int dtype, void* T, double* lam, cons
   if (chi * p > 64 || chi * p * p > 128)
     return fail(DQ_ERR_UNSUPPORTED, "simple update supports chi*phys <= 64 and chi*phys^2 <= 128 (chi=%d, phys=%d)",
                 chi, p);
-  const bool lds = su_lds_fits(chi, p, max_gate_rank);
-  const long long wsb = lds ? 256 : su_ws_bytes(chi, p, max_gate_rank);
+  const long long wsb = su_ws_bytes(chi, p, max_gate_rank);
   if ((rc = ws_ensure(g, (size_t)wsb * count)) != DQ_OK) return rc;
   RET_LAUNCH(launch_su(rb, g->dev, T, lam, (const SuGate*)gates, count, (const cd*)halves, max_gate_rank, eps,
                        max_rank, err, entropy, g->ws, wsb, tape, S(stream)));

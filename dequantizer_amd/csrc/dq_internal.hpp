@@ -97,6 +97,33 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// One DPP lane move of a double (two 32-bit moves).  Every source lane of
+// the controls used below lies in the destination's own 16-lane row, and the
+// callers run with the whole wave active.
+template <int kCtrl>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), kCtrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), kCtrl, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over aligned groups of L lanes (L = 1, 2, 4, 8, 16) by DPP moves inside
+// a row: quad_perm(1,0,3,2) pairs lanes i, i^1; quad_perm(2,3,0,1) pairs
+// i, i^2; row_half_mirror pairs i, 7-i within 8 lanes; row_mirror pairs
+// i, 15-i within 16.  After each step all lanes of the group formed so far
+// hold the same value (a + b and b + a round identically), so every lane of
+// a group gets the same sum.
+template <int L>
+__device__ __forceinline__ double group_sum(double v) {
+  static_assert(L == 1 || L == 2 || L == 4 || L == 8 || L == 16, "group of 1..16 lanes");
+  if (L >= 2) v += dpp_move<0xB1>(v);
+  if (L >= 4) v += dpp_move<0x4E>(v);
+  if (L >= 8) v += dpp_move<0x141>(v);
+  if (L >= 16) v += dpp_move<0x140>(v);
+  return v;
+}
+
 // Block-wide sum of a double; `red` must hold blockDim.x/64 doubles.  All
 // threads get the result.
 __device__ inline double block_sum(double v, double* red) {

@@ -138,7 +138,7 @@ hipError_t launch_su(int real_bytes, const DevGraph& g, void* T, double* lam, co
 // Per-block workspace of the simple update (bytes) and whether the small
 // FP64 matrices fit the LDS variant.
 long long su_ws_bytes(int chi, int phys, int max_r);
-bool su_lds_fits(int chi, int phys, int max_r);
+int su_mem_mode(int chi, int phys, int max_r);
 
 // One-site density matrices with messages e2n: rho[i] (phys x phys).
 hipError_t launch_rdm1(int real_bytes, const DevGraph& g, const void* T, const void* e2n, const int* nodes, int count,

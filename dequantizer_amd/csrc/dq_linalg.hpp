@@ -36,13 +36,17 @@ __device__ __forceinline__ void rr_pair(int r, int k, int n2, int& p, int& q) {
 // Returns false when the block is already diagonal to working precision.
 __device__ __forceinline__ bool jacobi_params(double app, double aqq, cd apq, double& c, double& s, double& er,
                                               double& ei, double abs_tol) {
-  const double mag = sqrt(norm2(apq));
-  // 1e-13 sits above the orthogonality FP64 can reach for n <= 64 (~n eps),
-  // so converged pairs stop rotating; convergence is quadratic past it
-  if (!(mag > 1e-13 * sqrt(fabs(app * aqq))) || !(mag > abs_tol) || mag < 1e-300) return false;
-  er = apq.re / mag;
-  ei = -apq.im / mag;
-  const double tau = (aqq - app) / (2.0 * mag);
+  const double n2 = norm2(apq);
+  // |apq| > 1e-13 sqrt(|app aqq|), squared: 1e-13 sits above the
+  // orthogonality FP64 can reach for n <= 64 (~n eps), so converged pairs
+  // stop rotating; convergence is quadratic past it
+  if (!(n2 > 1e-26 * fabs(app * aqq)) || !(n2 > abs_tol * abs_tol)) return false;
+  const double mag = sqrt(n2);
+  if (mag < 1e-300) return false;
+  const double inv = 1.0 / mag;
+  er = apq.re * inv;
+  ei = -apq.im * inv;
+  const double tau = 0.5 * (aqq - app) * inv;
   const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
   c = 1.0 / sqrt(1.0 + t * t);
   s = t * c;
@@ -377,13 +381,10 @@ __device__ inline void wave_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, int
         gr += x[t].re * y[t].re + x[t].im * y[t].im;
         gi += x[t].re * y[t].im - x[t].im * y[t].re;
       }
-#pragma unroll
-      for (int o = LPP / 2; o > 0; o >>= 1) {
-        al += __shfl_xor(al, o, 64);
-        be += __shfl_xor(be, o, 64);
-        gr += __shfl_xor(gr, o, 64);
-        gi += __shfl_xor(gi, o, 64);
-      }
+      al = group_sum<LPP>(al);
+      be = group_sum<LPP>(be);
+      gr = group_sum<LPP>(gr);
+      gi = group_sum<LPP>(gi);
       double c, s, er, ei;
       if (jacobi_params(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
         rotated = 1;
@@ -480,13 +481,10 @@ __device__ inline double wave_nuclear_norm_n(cd* G) {
         gr += x[t].re * y[t].re + x[t].im * y[t].im;
         gi += x[t].re * y[t].im - x[t].im * y[t].re;
       }
-#pragma unroll
-      for (int o = LPP / 2; o > 0; o >>= 1) {
-        al += __shfl_xor(al, o, 64);
-        be += __shfl_xor(be, o, 64);
-        gr += __shfl_xor(gr, o, 64);
-        gi += __shfl_xor(gi, o, 64);
-      }
+      al = group_sum<LPP>(al);
+      be = group_sum<LPP>(be);
+      gr = group_sum<LPP>(gr);
+      gi = group_sum<LPP>(gi);
       double c, s, er, ei;
       if (jacobi_params(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
         rotated = 1;
@@ -548,13 +546,10 @@ __device__ inline void block_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, in
         gr += x[t].re * y[t].re + x[t].im * y[t].im;
         gi += x[t].re * y[t].im - x[t].im * y[t].re;
       }
-#pragma unroll
-      for (int o = LPP / 2; o > 0; o >>= 1) {
-        al += __shfl_xor(al, o, 64);
-        be += __shfl_xor(be, o, 64);
-        gr += __shfl_xor(gr, o, 64);
-        gi += __shfl_xor(gi, o, 64);
-      }
+      al = group_sum<LPP>(al);
+      be = group_sum<LPP>(be);
+      gr = group_sum<LPP>(gr);
+      gi = group_sum<LPP>(gi);
       double c, s, er, ei;
       if (jacobi_params(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
         rotated = 1;

@@ -172,20 +172,25 @@ __device__ __forceinline__ su_d4 su_mfma(double a, double b, su_d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// Row o of the other bonds -> element offset (bond `slot` and phys at 0) and
-// the product of fn(lambda) over those bonds.
+// Rows o of the other bonds enumerate them in row-major order, so with
+// chi % 16 == 0 the 16 rows o0 .. o0 + 15 of a tile (o0 % 16 == 0) differ only
+// in the index of the last non-slot bond jl: b_jl = o0 % chi + delta.  Returns
+// the offset and fn(lambda) product of the other bonds of row o0, and jl, b0.
 template <bool kMask>
-__device__ __forceinline__ long long su_row(const DevGraph& g, int d, int chi, int slot, const int* str,
-                                            const double* wl, double eps, int o, double& wt) {
+__device__ __forceinline__ long long su_tile_row(int d, int chi, int slot, const int* str, const double* wl,
+                                                 double eps, int o0, double& whi, int& jl, int& b0) {
+  jl = slot == d - 1 ? d - 2 : d - 1;
+  b0 = o0 % chi;
+  int o = o0 / chi;
   long long off = 0;
-  wt = 1.0;
-  for (int j = d - 1; j >= 0; --j) {
+  whi = 1.0;
+  for (int j = jl - 1; j >= 0; --j) {
     if (j == slot) continue;
     const int q = o / chi, bj = o - q * chi;
     o = q;
     off += (long long)bj * str[j];
     const double l = wl[j * chi + bj];
-    wt *= kMask ? lam_mask(l, eps) : l;
+    whi *= kMask ? lam_mask(l, eps) : l;
   }
   return off;
 }
@@ -211,15 +216,33 @@ __device__ void su_environment_mfma(const DevGraph& g, const cx<R>* t, int node,
   const long long offx = (long long)((x0 + i) / p) * str[slot] + (x0 + i) % p;
   const long long offy = (long long)((y0 + i) / p) * str[slot] + (y0 + i) % p;
   su_d4 are = {0.0, 0.0, 0.0, 0.0}, aim = {0.0, 0.0, 0.0, 0.0};
+  // the loads of row group o0 + 16 are issued before the MFMAs of group o0
+  cx<R> na[4], nb[4];
+  double nw[4];
+  auto load_group = [&](int o0) {
+    double whi;
+    int jl, b0;
+    const long long offh = su_tile_row<false>(d, chi, slot, str, wl, 0.0, o0, whi, jl, b0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int b = b0 + 4 * u + kk;
+      const long long off = offh + (long long)b * str[jl];
+      nw[u] = whi * wl[jl * chi + b];
+      na[u] = t[off + offx];
+      nb[u] = t[off + offy];
+    }
+  };
+  load_group(0);
   for (int o0 = 0; o0 < rows; o0 += 16) {
     cx<R> va[4], vb[4];
     double wt[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const long long off = su_row<false>(g, d, chi, slot, str, wl, 0.0, o0 + 4 * u + kk, wt[u]);
-      va[u] = t[off + offx];
-      vb[u] = t[off + offy];
+      va[u] = na[u];
+      vb[u] = nb[u];
+      wt[u] = nw[u];
     }
+    if (o0 + 16 < rows) load_group(o0 + 16);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const double ar = wt[u] * (double)va[u].re, ai = wt[u] * (double)va[u].im;
@@ -253,16 +276,27 @@ __device__ void su_write_gamma_mfma(const DevGraph& g, cx<R>* t, int node, int s
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int i = lane & 15, kk = lane >> 4;
-  for (int rt = w; rt * 16 < rows; rt += 4) {
+  // the loads of tile rt + 4 are issued before the MFMAs of tile rt; a tile
+  // is read before it is overwritten, by the same wave
+  cx<R> nv[8];
+  auto load_tile = [&](int rt) {
     double unused;
-    const long long offa = su_row<false>(g, d, chi, slot, str, wl, 0.0, rt * 16 + i, unused);
-    su_d4 r0 = {0.0, 0.0, 0.0, 0.0}, i0 = r0, r1 = r0, i1 = r0;
-    cx<R> va[8];
+    int jl, b0;
+    const long long offa =
+        su_tile_row<false>(d, chi, slot, str, wl, 0.0, rt * 16, unused, jl, b0) + (long long)(b0 + i) * str[jl];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int x = 4 * u + kk;
-      va[u] = t[offa + (long long)(x / p) * str[slot] + x % p];
+      nv[u] = t[offa + (long long)(x / p) * str[slot] + x % p];
     }
+  };
+  if (w * 16 < rows) load_tile(w);
+  for (int rt = w; rt * 16 < rows; rt += 4) {
+    su_d4 r0 = {0.0, 0.0, 0.0, 0.0}, i0 = r0, r1 = r0, i1 = r0;
+    cx<R> va[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) va[u] = nv[u];
+    if ((rt + 4) * 16 < rows) load_tile(rt + 4);
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int x = 4 * u + kk;
@@ -277,12 +311,14 @@ __device__ void su_write_gamma_mfma(const DevGraph& g, cx<R>* t, int node, int s
       i1 = su_mfma(ar, z1.im, i1);
       i1 = su_mfma(ai, z1.re, i1);
     }
+    double mhi;
+    int jl, b0;
+    const long long offh = su_tile_row<true>(d, chi, slot, str, wl, eps, rt * 16, mhi, jl, b0);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      double wt;
-      const int o = rt * 16 + kk + 4 * v;
-      const long long off = su_row<true>(g, d, chi, slot, str, wl, eps, o, wt);
-      const double f = scale * wt;
+      const int b = b0 + kk + 4 * v;
+      const long long off = offh + (long long)b * str[jl];
+      const double f = scale * mhi * lam_mask(wl[jl * chi + b], eps);
       const int c0 = i, c1 = 16 + i;
       t[off + (long long)(c0 / p) * str[slot] + c0 % p] = cvt<R>(f * mk<double>(r0[v], i0[v]));
       t[off + (long long)(c1 / p) * str[slot] + c1 % p] = cvt<R>(f * mk<double>(r1[v], i1[v]));
@@ -420,22 +456,41 @@ __device__ void su_write_gamma(const DevGraph& g, cx<R>* t, int node, int slot, 
   }
 }
 
-// Workspace (complex doubles) of one gate: E (cp^2, later U), G (n^2, later
-// the core and W), V, Ra, Rb (n^2 each), Z (cp^2), chunk (rows x cp).
-__host__ __device__ inline long long su_ws_cd(int chi, int phys, int max_r, int chunk_rows) {
-  const long long n = (long long)chi * max_r, cp = (long long)chi * phys;
-  const long long en = cp * cp > n * n ? cp * cp : n * n;  // E doubles as U (n x n)
-  return en + 4 * n * n + cp * cp + chunk_rows * cp + 64;
+// Matrices of one gate (complex doubles): E (cp^2 or n^2; later U, then Z),
+// G (n^2; Gram, core, then W_a), V (n^2; right singular vectors, then W_b),
+// Ra, Rb (n^2 each), chunk (rows x cp, generic environment path only).
+// Storage modes:
+//   kSuGlobal   all in the global workspace;
+//   kSuLds      all in LDS (n, cp <= 32; ~112 KiB of LDS, one block per CU);
+//   kSuCompact  E, G, V in LDS (48 KiB), Ra, Rb, chunk in the workspace
+//               (L2-resident): two blocks per CU.  For cp = 32, chi % 16 == 0
+//               (the matrix-core shapes), n <= 32.
+constexpr int kSuGlobal = 0, kSuLds = 1, kSuCompact = 2;
+constexpr int kSuChunk = 32;
+constexpr int kSuLdsN = 32;
+
+__host__ __device__ inline long long su_e_cd(long long n, long long cp) { return cp * cp > n * n ? cp * cp : n * n; }
+
+int su_mem_mode(int chi, int phys, int max_r) {
+  if (chi * phys == 32 && chi % 16 == 0 && chi * max_r <= kSuLdsN) return kSuCompact;
+  if (chi * max_r <= kSuLdsN && chi * phys <= kSuLdsN) return kSuLds;
+  return kSuGlobal;
 }
 
-constexpr int kSuChunk = 32;
-constexpr int kSuLdsN = 32;  // LDS variant: n = chi*r <= 32 and chi*phys <= 32
+// Workspace complex doubles per gate in each mode.
+__host__ __device__ inline long long su_ws_cd(int mode, int chi, int phys, int max_r) {
+  const long long n = (long long)chi * max_r, cp = (long long)chi * phys;
+  if (mode == kSuLds) return 16;
+  if (mode == kSuCompact) return 2 * n * n + kSuChunk * cp + 64;
+  return su_e_cd(n, cp) + 4 * n * n + kSuChunk * cp + 64;
+}
 
-template <typename R, bool kLds>
-__global__ void __launch_bounds__(256)
+template <typename R, int kMem>
+__global__ void __launch_bounds__(256, kMem == kSuCompact ? 2 : 1)
     k_su(DevGraph g, cx<R>* T, double* lam, const SuGate* gates, const cd* halves, int max_r, double eps,
          int max_rank, double* err_out, double* ent_out, char* ws, long long wsb, char* tape, long long tape_bytes) {
-  __shared__ cd sm[kLds ? (6 * kSuLdsN * kSuLdsN + kSuChunk * kSuLdsN + 64) : 1];
+  __shared__ cd sm[kMem == kSuLds ? (5 * kSuLdsN * kSuLdsN + kSuChunk * kSuLdsN + 64)
+                                  : (kMem == kSuCompact ? 3 * kSuLdsN * kSuLdsN : 1)];
   __shared__ double red[8];
   __shared__ double rot[4 * 64 + 8];
   __shared__ double sigma[128];
@@ -459,13 +514,13 @@ __global__ void __launch_bounds__(256)
   const long long hstride = (long long)p * p * p * p;
   const cd* h1 = halves + (long long)blockIdx.x * 2 * hstride;
   const cd* h2 = h1 + hstride;
-  cd* E = kLds ? sm : (cd*)(ws + (long long)blockIdx.x * wsb);
-  cd* Gm = E + (cp * cp > n * n ? cp * cp : n * n);
+  cd* wsp = (cd*)(ws + (long long)blockIdx.x * wsb);
+  cd* E = kMem == kSuGlobal ? wsp : sm;
+  cd* Gm = E + su_e_cd(n, cp);
   cd* V = Gm + n * n;
-  cd* Ra = V + n * n;
+  cd* Ra = kMem == kSuCompact ? wsp : V + n * n;
   cd* Rb = Ra + n * n;
-  cd* Z = Rb + n * n;
-  cd* chunk = Z + cp * cp;
+  cd* chunk = Rb + n * n;
   cd* C = Gm;  // core (after both R factors)
   cd* U = E;   // left singular vectors (after both environments)
   cd* Vs = V;  // right singular vectors
@@ -482,8 +537,8 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
 
-  const bool mfma_a = cp == 32 && blockDim.x == 256 && ipow(chi, g.deg[na] - 1) % 16 == 0;
-  const bool mfma_b = cp == 32 && blockDim.x == 256 && ipow(chi, g.deg[nb] - 1) % 16 == 0;
+  const bool mfma_a = cp == 32 && chi % 16 == 0 && blockDim.x == 256 && g.deg[na] >= 2;
+  const bool mfma_b = cp == 32 && chi % 16 == 0 && blockDim.x == 256 && g.deg[nb] >= 2;
   if (mfma_a)
     su_environment_mfma<R>(g, ta, na, ia, wla, E);
   else
@@ -505,15 +560,19 @@ __global__ void __launch_bounds__(256)
     for (int i = threadIdx.x; i < n; i += blockDim.x) tp.piv_b()[i] = perm[i];
     __syncthreads();
   }
-  // core = R_a diag(lambda_rep) R_b^T, column-major
+  // core = R_a diag(lambda_rep) R_b^T, column-major.  R_b^T goes to the
+  // free environment buffer first so that lanes (along j) read contiguous
+  // words of it while R_a[i][c] is a broadcast.
   __shared__ double le[128];
   for (int j = threadIdx.x; j < chi; j += blockDim.x) le[j] = lam[(long long)e * chi + j];
+  cd* RbT = E;
+  for (int f = threadIdx.x; f < n * n; f += blockDim.x) RbT[(f % n) * n + f / n] = Rb[f];
   __syncthreads();
   for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
-    const int j = f / n, i = f % n;
+    const int i = f / n, j = f % n;
     cd acc = mk<double>(0, 0);
-    for (int c = 0; c < n; ++c) fma_acc(acc, le[c / r] * Ra[i * n + c], Rb[j * n + c]);
-    C[f] = acc;
+    for (int c = 0; c < n; ++c) fma_acc(acc, le[c / r] * Ra[i * n + c], RbT[c * n + j]);
+    C[j * n + i] = acc;
   }
   __syncthreads();
   if (n == 32 && blockDim.x == 256) {
@@ -566,11 +625,14 @@ __global__ void __launch_bounds__(256)
       tp.meta()[1] = n;
     }
   }
-  // W_a = Lambda R_b^T V Sigma^-1 -> Z_a;  W_b = Lambda R_a^T conj(U) Sigma^-1 -> Z_b
+  // W_a = Lambda R_b^T V Sigma^-1 (into G, the core is no longer needed),
+  // then W_b = Lambda R_a^T conj(U) Sigma^-1 (into V, read for W_a only);
+  // both n x chi.  Z_a, Z_b then go to E (U is no longer needed).
   for (int side = 0; side < 2; ++side) {
-    cd* W = Gm;  // n x chi (the core is no longer needed)
+    cd* W = side == 0 ? Gm : V;
     for (int f = threadIdx.x; f < n * chi; f += blockDim.x) {
-      const int x = f / chi, j = f % chi;
+      const int x = f % n, j = f / n;  // lanes along x: R rows are read contiguously
+      const int fw = x * chi + j;
       cd acc = mk<double>(0, 0);
       if (j < rank && sigma[j] > 0.0) {
         for (int i = 0; i < n; ++i) {
@@ -581,9 +643,13 @@ __global__ void __launch_bounds__(256)
         }
         acc = (le[x / r] / sigma[j]) * acc;
       }
-      W[f] = acc;
+      W[fw] = acc;
     }
     __syncthreads();
+  }
+  cd* Z = E;
+  for (int side = 0; side < 2; ++side) {
+    const cd* W = side == 0 ? Gm : V;
     const cd* h = side == 0 ? h1 : h2;
     for (int f = threadIdx.x; f < cp * cp; f += blockDim.x) {
       const int x = f / cp, y = f % cp;
@@ -624,31 +690,33 @@ hipError_t launch_su(int rb, const DevGraph& g, void* T, double* lam, const SuGa
                      const cd* halves, int max_gate_rank, double eps, int max_rank, double* err, double* entropy,
                      void* ws, long long wsb, void* tape, hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  const bool lds = su_lds_fits(g.chi, g.phys, max_gate_rank);
+  const int mode = su_mem_mode(g.chi, g.phys, max_gate_rank);
   const long long tb = su_tape_bytes(g.chi, max_gate_rank);
   char* tp = (char*)tape;
+#define DQ_SU_LAUNCH(RT, MODE)                                                                                   \
+  hipLaunchKernelGGL((k_su<RT, MODE>), dim3(count), dim3(256), 0, s, g, (cx<RT>*)T, lam, gates, halves,         \
+                     max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb, tp, tb)
   if (rb == 4) {
-    if (lds)
-      hipLaunchKernelGGL((k_su<float, true>), dim3(count), dim3(256), 0, s, g, (cx<float>*)T, lam, gates, halves,
-                         max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb, tp, tb);
+    if (mode == kSuCompact)
+      DQ_SU_LAUNCH(float, kSuCompact);
+    else if (mode == kSuLds)
+      DQ_SU_LAUNCH(float, kSuLds);
     else
-      hipLaunchKernelGGL((k_su<float, false>), dim3(count), dim3(256), 0, s, g, (cx<float>*)T, lam, gates, halves,
-                         max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb, tp, tb);
+      DQ_SU_LAUNCH(float, kSuGlobal);
   } else {
-    if (lds)
-      hipLaunchKernelGGL((k_su<double, true>), dim3(count), dim3(256), 0, s, g, (cx<double>*)T, lam, gates, halves,
-                         max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb, tp, tb);
+    if (mode == kSuCompact)
+      DQ_SU_LAUNCH(double, kSuCompact);
+    else if (mode == kSuLds)
+      DQ_SU_LAUNCH(double, kSuLds);
     else
-      hipLaunchKernelGGL((k_su<double, false>), dim3(count), dim3(256), 0, s, g, (cx<double>*)T, lam, gates, halves,
-                         max_gate_rank, eps, max_rank, err, entropy, (char*)ws, wsb, tp, tb);
+      DQ_SU_LAUNCH(double, kSuGlobal);
   }
+#undef DQ_SU_LAUNCH
   return hipGetLastError();
 }
 
-bool su_lds_fits(int chi, int phys, int max_r) { return chi * max_r <= kSuLdsN && chi * phys <= kSuLdsN; }
-
 long long su_ws_bytes(int chi, int phys, int max_r) {
-  return (su_ws_cd(chi, phys, max_r, kSuChunk) * (long long)sizeof(cd) + 255) / 256 * 256;
+  return (su_ws_cd(su_mem_mode(chi, phys, max_r), chi, phys, max_r) * (long long)sizeof(cd) + 255) / 256 * 256;
 }
 
 }  // namespace dq
