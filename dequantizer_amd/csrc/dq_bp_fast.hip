@@ -92,76 +92,104 @@ __device__ __forceinline__ void gram_partial(const float2* t, const float2* u, i
 }  // namespace fast3
 
 __global__ void __launch_bounds__(64 * fast3::NW)
-    k_bp_nodes_d3c16(DevGraph g, const int* nodes, const float2* T, const float2* n2e_old, const float2* e2n_old,
-                     float2* n2e_new, const uint8_t* node_src, const int* inst_active) {
+    k_bp_nodes_d3c16(DevGraph g, const int* nodes, int count, const float2* T, const float2* n2e_old,
+                     const float2* e2n_old, float2* n2e_new, const uint8_t* node_src, const int* inst_active) {
   using namespace fast3;
+  constexpr int NT = 64 * NW;   // threads
+  constexpr int PER = S / NT;   // tensor elements staged per thread
   __shared__ float2 tl[NPAD];
   __shared__ float2 wk[NPAD];
   __shared__ float2 part[NW][256];
-  const int node = nodes[blockIdx.x];
-  if (inst_active && !inst_active[g.node_inst[node]]) return;
+  __shared__ float2 diag[16];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int base = g.adj_ptr[node];
-  // stage T into LDS (padded)
-  const float2* tg = T + g.toff[node];
-  for (int f = tid; f < S; f += 64 * NW) {
-    const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
-    tl[b0 * B0P + b1 * RP + r] = tg[f];
+  // Persistent blocks (one per CU: the two tensors take most of the LDS):
+  // block b updates nodes[b], nodes[b + gridDim.x], ...  The next node's
+  // tensor is loaded into registers while the current one is processed, so
+  // its HBM latency is hidden behind the matrix-core work.
+  auto active = [&](int it) { return !inst_active || inst_active[g.node_inst[nodes[it]]]; };
+  int it = blockIdx.x;
+  while (it < count && !active(it)) it += gridDim.x;
+  if (it >= count) return;
+  float2 pre[PER];
+  {
+    const float2* tg = T + g.toff[nodes[it]];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * NT];
   }
-  // message operands in registers: m[j][ks] = M_j[lane%16][4 ks + lane/16]
-  float2 m[3][4];
-  int oe[3], os[3];
+  while (true) {
+    const int node = nodes[it];
+    const int base = g.adj_ptr[node];
+    // stage T into LDS (padded); the previous node's last reads are behind
+    // the barrier that ended its loop iteration
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const int e = g.adj_edge[base + k], s = g.adj_side[base + k];
-    oe[k] = e;
-    os[k] = s;
-    const int code = node_src ? node_src[base + k] : 0;
-    const float2* src = code == 0 ? e2n_old + (2LL * e + s) * 256
-                                  : (code == 1 ? n2e_new : n2e_old) + (2LL * e + 1 - s) * 256;
+    for (int k = 0; k < PER; ++k) {
+      const int f = tid + k * NT;
+      const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
+      tl[b0 * B0P + b1 * RP + r] = pre[k];
+    }
+    // message operands in registers: m[j][ks] = M_j[lane%16][4 ks + lane/16]
+    float2 m[3][4];
+    int oe[3], os[3];
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) m[k][ks] = src[(lane & 15) * 16 + 4 * ks + (lane >> 4)];
-  }
-  __syncthreads();
-  // (first product, second product, Gram axis) for idx = 0, 1, 2
-  const int plan[3][3] = {{2, 1, 0}, {2, 0, 1}, {0, 1, 2}};
+    for (int k = 0; k < 3; ++k) {
+      const int e = g.adj_edge[base + k], sd = g.adj_side[base + k];
+      oe[k] = e;
+      os[k] = sd;
+      const int code = node_src ? node_src[base + k] : 0;
+      const float2* src = code == 0 ? e2n_old + (2LL * e + sd) * 256
+                                    : (code == 1 ? n2e_new : n2e_old) + (2LL * e + 1 - sd) * 256;
 #pragma unroll
-  for (int step = 0; step < 3; ++step) {
-    const int j1 = plan[step][0], j2 = plan[step][1], idx = plan[step][2];
-    mode_product<NW>(wk, tl, m[j1], j1, wave, lane);
-    __syncthreads();
-    mode_product<NW>(wk, wk, m[j2], j2, wave, lane);
-    __syncthreads();
-    floatx4 gr, gi;
-    gram_partial(tl, wk, idx, wave, lane, gr, gi);
+      for (int ks = 0; ks < 4; ++ks) m[k][ks] = src[(lane & 15) * 16 + 4 * ks + (lane >> 4)];
+    }
+    int next = it + gridDim.x;
+    while (next < count && !active(next)) next += gridDim.x;
+    if (next < count) {
+      const float2* tg = T + g.toff[nodes[next]];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) part[wave][(4 * (lane >> 4) + v) * 16 + (lane & 15)] = make_float2(gr[v], gi[v]);
-    __syncthreads();
-    if (tid < 256) {
-      float2 sum = part[0][tid];
-      for (int w = 1; w < NW; ++w) {
-        sum.x += part[w][tid].x;
-        sum.y += part[w][tid].y;
-      }
-      float tr_r = 0.f, tr_i = 0.f;
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        const int q = k * 17;
-        float2 d = part[0][q];
-        for (int w = 1; w < NW; ++w) {
-          d.x += part[w][q].x;
-          d.y += part[w][q].y;
-        }
-        tr_r += d.x;
-        tr_i += d.y;
-      }
-      const float den = tr_r * tr_r + tr_i * tr_i;
-      float2 o;
-      o.x = (sum.x * tr_r + sum.y * tr_i) / den;
-      o.y = (sum.y * tr_r - sum.x * tr_i) / den;
-      n2e_new[(2LL * oe[idx] + os[idx]) * 256 + tid] = o;
+      for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * NT];
     }
     __syncthreads();
+    // (first product, second product, Gram axis) for idx = 0, 1, 2
+    const int plan[3][3] = {{2, 1, 0}, {2, 0, 1}, {0, 1, 2}};
+#pragma unroll
+    for (int step = 0; step < 3; ++step) {
+      const int j1 = plan[step][0], j2 = plan[step][1], idx = plan[step][2];
+      mode_product<NW>(wk, tl, m[j1], j1, wave, lane);
+      __syncthreads();
+      mode_product<NW>(wk, wk, m[j2], j2, wave, lane);
+      __syncthreads();
+      floatx4 gr, gi;
+      gram_partial(tl, wk, idx, wave, lane, gr, gi);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) part[wave][(4 * (lane >> 4) + v) * 16 + (lane & 15)] = make_float2(gr[v], gi[v]);
+      __syncthreads();
+      float2 sum = make_float2(0.f, 0.f);
+      if (tid < 256) {
+        sum = part[0][tid];
+        for (int w = 1; w < NW; ++w) {
+          sum.x += part[w][tid].x;
+          sum.y += part[w][tid].y;
+        }
+        if (tid % 17 == 0) diag[tid / 17] = sum;
+      }
+      __syncthreads();
+      if (tid < 256) {
+        float tr_r = 0.f, tr_i = 0.f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          tr_r += diag[k].x;
+          tr_i += diag[k].y;
+        }
+        const float den = tr_r * tr_r + tr_i * tr_i;
+        float2 o;
+        o.x = (sum.x * tr_r + sum.y * tr_i) / den;
+        o.y = (sum.y * tr_r - sum.x * tr_i) / den;
+        n2e_new[(2LL * oe[idx] + os[idx]) * 256 + tid] = o;
+      }
+    }
+    __syncthreads();
+    if (next >= count) break;
+    it = next;
   }
 }
 
@@ -234,7 +262,8 @@ hipError_t launch_bp_nodes_fast(const DevGraph& g, const int* nodes, int count, 
                                 const void* e2n_old, void* n2e_new, const uint8_t* node_src, const int* inst_active,
                                 hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_bp_nodes_d3c16, dim3(count), dim3(64 * fast3::NW), 0, s, g, nodes, (const float2*)T,
+  const int grid = count < device_cu_count() ? count : device_cu_count();
+  hipLaunchKernelGGL(k_bp_nodes_d3c16, dim3(grid), dim3(64 * fast3::NW), 0, s, g, nodes, count, (const float2*)T,
                      (const float2*)n2e_old, (const float2*)e2n_old, (float2*)n2e_new, node_src, inst_active);
   return hipGetLastError();
 }

@@ -10,6 +10,19 @@
 namespace dq {
 
 // Grid size of a grid-stride loop over `total` items (capped at 65536 blocks).
+// Compute units of the current device (cached per device).
+inline int device_cu_count() {
+  static int cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cache[dev] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    cache[dev] = v;
+  }
+  return cache[dev];
+}
+
 inline int grid_for(long long total, int block) {
   long long gsz = (total + block - 1) / block;
   if (gsz > 65536) gsz = 65536;
