@@ -144,6 +144,137 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The same decomposition for chi = 16 with wave-level Jacobi: one edge per
+// 128-thread block; wave 0 diagonalises the forward message while wave 1
+// diagonalises the backward one, then wave 0 takes the SVD of the core.
+// Seven 16 x 16 complex128 matrices in LDS (28 KiB): Mf (then Sf, then U),
+// Mb (then Sb, then V), Qf, Qb, If, Ib, G.
+template <typename R>
+__global__ void __launch_bounds__(128)
+    k_edge_svd16(DevGraph g, const cx<R>* n2e, double eps, cd* X, double* lam, const int* inst_active, char* tape) {
+  constexpr int n = 16, cc = 256;
+  __shared__ cd Mf[cc], Mb[cc], Qf[cc], Qb[cc], If[cc], Ib[cc], G[cc];
+  __shared__ double wf[n], wb[n], sigma[n], scal[2 * n];
+  __shared__ int perm[n];
+  const int e = blockIdx.x;
+  if (inst_active && !inst_active[g.edge_inst[e]]) return;
+  const int tid = threadIdx.x;
+  const cx<R>* m0 = n2e + (2LL * e) * cc;
+  const cx<R>* m1 = n2e + (2LL * e + 1) * cc;
+  // Hermitian part (x + x^H) / 2, as the reference's eigh symmetrises its input
+  for (int q = tid; q < cc; q += blockDim.x) {
+    const int k = q / n, l = q % n;
+    const cd a = cvt<double>(m0[q]), at = cvt<double>(m0[l * n + k]);
+    const cd b = cvt<double>(m1[q]), bt = cvt<double>(m1[l * n + k]);
+    Mf[q] = mk<double>(0.5 * (a.re + at.re), 0.5 * (a.im - at.im));
+    Mb[q] = mk<double>(0.5 * (b.re + bt.re), 0.5 * (b.im - bt.im));
+  }
+  __syncthreads();
+  if (tid < 64)
+    wave_jacobi_eigh_n<n>(Mf, Qf);
+  else
+    wave_jacobi_eigh_n<n>(Mb, Qb);
+  __syncthreads();
+  if (tid < n) {
+    wf[tid] = Mf[tid * (n + 1)].re;
+    wb[tid] = Mb[tid * (n + 1)].re;
+  }
+  __syncthreads();
+  // m^{1/2} and m^{-1/2} (safe) from the eigendecompositions
+  cd* Sf = Mf;
+  cd* Sb = Mb;
+  for (int q = tid; q < cc; q += blockDim.x) {
+    const int r = q / n, c = q % n;
+    cd sf = mk<double>(0, 0), sb = sf, inf = sf, inb = sf;
+#pragma unroll 4
+    for (int i = 0; i < n; ++i) {
+      const cd pf = Qf[r * n + i] * conj(Qf[c * n + i]);
+      const cd pb = Qb[r * n + i] * conj(Qb[c * n + i]);
+      const double a = sqrt(safe_truncate(wf[i], eps)), b = sqrt(safe_truncate(wb[i], eps));
+      const double ai = sqrt(safe_inverse(wf[i], eps)), bi = sqrt(safe_inverse(wb[i], eps));
+      sf = sf + a * pf;
+      sb = sb + b * pb;
+      inf = inf + ai * pf;
+      inb = inb + bi * pb;
+    }
+    Sf[q] = sf;
+    Sb[q] = sb;
+    If[q] = inf;
+    Ib[q] = inb;
+  }
+  __syncthreads();
+  // G (column-major) = Sf Sb^T
+  for (int q = tid; q < cc; q += blockDim.x) {
+    const int c = q / n, r = q % n;
+    cd acc = mk<double>(0, 0);
+#pragma unroll 4
+    for (int k = 0; k < n; ++k) fma_acc(acc, Sf[r * n + k], Sb[c * n + k]);
+    G[q] = acc;
+  }
+  __syncthreads();
+  cd* U = Mf;
+  cd* V = Mb;
+  if (tid < 64) wave_jacobi_svd_n<n>(G, U, V, sigma, perm, scal);
+  __syncthreads();
+  double nrm = 0.0;
+  for (int j = 0; j < n; ++j) nrm += sigma[j] * sigma[j];
+  nrm = sqrt(nrm);
+  // masked Frobenius norms of U and Vh
+  double nu = 0.0, nv = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const double lj = nrm > 0.0 ? sigma[j] / nrm : 0.0;
+    if (safe_truncate(lj, eps) > eps) {
+      for (int i = 0; i < n; ++i) {
+        nu += norm2(U[j * n + i]);
+        nv += norm2(V[perm[j] * n + i]);
+      }
+    }
+  }
+  nu = sqrt(nu);
+  nv = sqrt(nv);
+  for (int j = tid; j < n; j += blockDim.x) {
+    const double lj = nrm > 0.0 ? safe_truncate(sigma[j] / nrm, eps) : 0.0;
+    lam[(long long)e * n + j] = lj;
+  }
+  if (tape) {
+    EdgeSvdTape tp(tape + (long long)e * edge_svd_tape_bytes(n), n);
+    for (int q = tid; q < cc; q += blockDim.x) {
+      const int i = q / n, j = q % n;
+      tp.Qf()[q] = Qf[q];
+      tp.Qb()[q] = Qb[q];
+      tp.U()[q] = U[(long long)j * n + i];
+      tp.V()[q] = V[(long long)perm[j] * n + i];
+    }
+    for (int j = tid; j < n; j += blockDim.x) {
+      tp.wf()[j] = wf[j];
+      tp.wb()[j] = wb[j];
+      tp.sigma()[j] = sigma[j];
+    }
+    if (tid == 0) {
+      tp.scal()[0] = nrm;
+      tp.scal()[1] = nu;
+      tp.scal()[2] = nv;
+    }
+  }
+  cd* Xf = X + (2LL * e) * cc;
+  cd* Xb = X + (2LL * e + 1) * cc;
+  for (int q = tid; q < cc; q += blockDim.x) {
+    const int b = q / n, k = q % n;
+    const double lk = nrm > 0.0 ? safe_truncate(sigma[k] / nrm, eps) : 0.0;
+    cd xf = mk<double>(0, 0), xb = xf;
+    if (lk > eps) {
+      for (int i = 0; i < n; ++i) {
+        fma_acc(xf, If[b * n + i], U[k * n + i]);
+        fma_acc(xb, Ib[b * n + i], conj(V[perm[k] * n + i]));
+      }
+      xf = (1.0 / nu) * xf;
+      xb = (1.0 / nv) * xb;
+    }
+    Xf[q] = xf;
+    Xb[q] = xb;
+  }
+}
+
 template <typename R>
 __global__ void __launch_bounds__(256)
     k_absorb(DevGraph g, const cx<R>* T, const cd* X, cx<R>* gam, char* ws, long long wsb, const int* inst_active) {
@@ -275,6 +406,15 @@ long long edge_svd_ws_bytes_host(int chi) { return edge_svd_ws_bytes(chi); }
 hipError_t launch_edge_svd(int rb, const DevGraph& g, const void* n2e, double eps, cd* X, double* lam, void* ws,
                            long long wsb, const int* inst_active, void* tape, hipStream_t s) {
   if (g.n_edges <= 0) return hipSuccess;
+  if (g.chi == 16) {
+    if (rb == 4)
+      hipLaunchKernelGGL(k_edge_svd16<float>, dim3(g.n_edges), dim3(128), 0, s, g, (const cx<float>*)n2e, eps, X, lam,
+                         inst_active, (char*)tape);
+    else
+      hipLaunchKernelGGL(k_edge_svd16<double>, dim3(g.n_edges), dim3(128), 0, s, g, (const cx<double>*)n2e, eps, X,
+                         lam, inst_active, (char*)tape);
+    return hipGetLastError();
+  }
   if (rb == 4)
     hipLaunchKernelGGL(k_edge_svd<float>, dim3(g.n_edges), dim3(256), 0, s, g, (const cx<float>*)n2e, eps, X, lam,
                        (char*)ws, wsb, inst_active, (char*)tape);

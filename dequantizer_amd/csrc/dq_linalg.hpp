@@ -136,6 +136,81 @@ __device__ inline void jacobi_eigh(cd* A, cd* V, int n, double* rot, int* flag) 
   }
 }
 
+// Wave-level two-sided Jacobi for an N x N Hermitian A (row-major, in LDS or
+// global memory), N even, N / 2 pairs per round on groups of 128 / N lanes:
+// the same rotations and convergence rule as jacobi_eigh, with the round's
+// column and row updates separated by wave-level fences instead of block
+// barriers.  A -> eigenvalues on the diagonal; V columns = eigenvectors.
+// Call with all 64 lanes of one wave.
+template <int N>
+__device__ inline void wave_jacobi_eigh_n(cd* A, cd* V) {
+  constexpr int NP = N / 2;
+  constexpr int LPP = 64 / NP;  // lanes per pair
+  constexpr int RPL = N / LPP;  // rows (columns) per lane
+  static_assert(N % 2 == 0 && LPP * NP == 64 && RPL * LPP == N, "wave eigh sizing");
+  const int lane = threadIdx.x & 63;
+  const int k = lane / LPP, sub = lane % LPP;
+  for (int i = lane; i < N * N; i += 64) V[i] = mk<double>((i / N) == (i % N) ? 1.0 : 0.0, 0.0);
+  double f = 0.0;
+  for (int i = lane; i < N * N; i += 64) f += norm2(A[i]);
+  const double abs_tol = 1e-16 * sqrt(wave_sum(f));
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    int rotated = 0;
+    for (int r = 0; r < N - 1; ++r) {
+      int p, q;
+      rr_pair(r, k, N, p, q);
+      double c = 1.0, s = 0.0, er = 1.0, ei = 0.0;
+      const bool rot = jacobi_params(A[p * N + p].re, A[q * N + q].re, A[p * N + q], c, s, er, ei, abs_tol);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (rot) {
+        rotated = 1;
+        const cd e = mk<double>(er, ei);
+        // A <- A J and V <- V J (columns p, q)
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          const int i = sub + t * LPP;
+          const cd x = A[i * N + p], y = A[i * N + q];
+          const cd ey = e * y;
+          A[i * N + p] = mk<double>(c * x.re - s * ey.re, c * x.im - s * ey.im);
+          A[i * N + q] = mk<double>(s * x.re + c * ey.re, s * x.im + c * ey.im);
+          const cd vx = V[i * N + p], vy = V[i * N + q];
+          const cd evy = e * vy;
+          V[i * N + p] = mk<double>(c * vx.re - s * evy.re, c * vx.im - s * evy.im);
+          V[i * N + q] = mk<double>(s * vx.re + c * evy.re, s * vx.im + c * evy.im);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (rot) {
+        // A <- J^H A (rows p, q)
+        const cd ec = mk<double>(er, -ei);
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          const int j = sub + t * LPP;
+          const cd x = A[p * N + j], y = A[q * N + j];
+          const cd ey = ec * y;
+          A[p * N + j] = mk<double>(c * x.re - s * ey.re, c * x.im - s * ey.im);
+          A[q * N + j] = mk<double>(s * x.re + c * ey.re, s * x.im + c * ey.im);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (rot && sub == 0) {
+        A[p * N + q] = mk<double>(0.0, 0.0);
+        A[q * N + p] = mk<double>(0.0, 0.0);
+        A[p * N + p].im = 0.0;
+        A[q * N + q].im = 0.0;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (!__any(rotated)) break;
+  }
+}
+
 // One-sided Jacobi SVD.  G: m x n column-major (ldg), m >= n, overwritten.
 // Outputs sigma (descending), U (m x n, column-major, ldu; zero columns for
 // sigma == 0) and the UNSORTED right vectors V (n x n column-major): with

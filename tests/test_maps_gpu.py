@@ -42,6 +42,7 @@ CASES = {
     "lattice3x3_chi3": lambda: lattice_pair([3, 3], 2, 3),
     "cube2_chi2": lambda: lattice_pair([2, 2, 2], 2, 2, open_boundary=False),
     "rr3_12_chi4": lambda: regular_pair(12, 4, 3),
+    "rr3_10_chi16": lambda: regular_pair(10, 16, 3),
 }
 
 
@@ -223,3 +224,41 @@ def test_vidal_distance_matrix_core_path_chi16(zero_tail):
         msgs = bp(t64, msgs)
     gam, lam = dq.get_vidal_gauge_fixing_map(trav_p, 1e-10)(t64, msgs)
     assert float(dq.get_vidal_gauge_distance_map(trav_p, 1e-10)(gam, lam)) < 1e-4
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-10), (np.complex64, 2e-4)])
+def test_vidal_gauge_chi16_indefinite_messages(dtype, tol):
+    """chi=16 edges run the wave-level decomposition (two-sided Jacobi on each
+    message, one-sided Jacobi on the core).  Hermitian messages with negative
+    and clustered eigenvalues: lambdas and gauge-invariant products against
+    the oracle's eigh/SVD path."""
+    gp, go = regular_pair(10, 16, 4)
+    rng = np.random.default_rng(17)
+    tensors = random_tensors(gp, rng)
+    msgs = {}
+    for el in gp.get_traversal_iterator():
+        for nb in el.neighbors:
+            q, _ = np.linalg.qr(rng.standard_normal((16, 16)) + 1j * rng.standard_normal((16, 16)))
+            w = rng.uniform(-0.2, 1.0, 16)
+            w[:3] = 0.5  # a threefold eigenvalue
+            w[3] = 1e-13  # below eps
+            m = (q * w) @ q.conj().T
+            msgs[dq.MessageID(el.id, nb.id)] = m
+    trav_p = list(gp.get_traversal_iterator())
+    trav_o = go.traversal()
+    gam_o, lam_o = om.vidal_gauge_fixing_map(trav_o, 1e-10)(tensors, to_oracle_messages(msgs))
+    gam_p, lam_p = dq.get_vidal_gauge_fixing_map(trav_p, 1e-10)(
+        {k: v.astype(dtype) for k, v in tensors.items()}, {k: v.astype(dtype) for k, v in msgs.items()})
+    for eid, lm in lam_o.items():
+        assert np.abs(host(lam_p[eid]) - lm).max() < tol * 10, eid
+    # Gamma up to the gauge of degenerate singular values: the symmetric-gauge
+    # one-site density matrices are invariant
+    gam_h = {nid: host(gam_p[nid]).astype(np.complex128) for nid in gam_o}
+    lam_h = {eid: host(lam_p[eid]) for eid in lam_o}
+    sg_o = om.symmetric_gauge_fixing_map(trav_o, 1e-10)
+    t_o, t_p = sg_o(gam_o, lam_o), sg_o(gam_h, lam_h)
+    msg_o, msg_p = om.lambdas2messages(lam_o), om.lambdas2messages(lam_h)
+    for node in go.nodes.values():
+        r_o = om.one_side_density_matrix(node, t_o, msg_o)
+        r_p = om.one_side_density_matrix(node, t_p, msg_p)
+        assert np.abs(r_o - r_p).max() < tol * 100, node.id
