@@ -57,6 +57,9 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="Trotter steps of the forward+adjoint timing (0 skips it)")
     ap.add_argument("--knots", type=int, default=32, help="schedule knots of the forward+adjoint timing")
+    ap.add_argument("--batch-instances", type=int, default=64,
+                    help="instances of the batched annealing leg (the whole 200-step run of the random 3-regular "
+                         "instance set, N in 100..1000, in one engine); 0 skips it")
     ap.add_argument("--gate-order", default="colored", choices=["colored", "pairs"],
                     help="coupling-gate order: pair list sorted by edge colour, or the generator's pair order")
     args = ap.parse_args()
@@ -161,6 +164,9 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if args.batch_instances:
+        result["batched"] = batched_run(args)
+        print(f"bench: batched run done: {result['batched']['seconds']:.1f} s", file=sys.stderr, flush=True)
     if args.adjoint_steps:
         result["fwd_adjoint"] = adjoint_timing(ef, args)
         print(f"bench: forward+adjoint done: {result['fwd_adjoint']['seconds']:.1f} s", file=sys.stderr, flush=True)
@@ -173,6 +179,48 @@ def main():
         result["cpu_baseline"] = cpu_baseline(ann, ef, schedule[args.warmup + args.steps], args)
         result["vs_cpu_baseline"] = result["value"] / result["cpu_baseline"]["value"]
     print(json.dumps(result))
+
+
+def batched_run(args):
+    """The instance set (random 3-regular graphs, N drawn from 100..1000,
+    +-1 couplings, N(0, 0.1^2) fields) annealed together in one engine over
+    the whole schedule: every launch covers all instances.  EU/s over the
+    whole run (regauges, simple updates, one-site gates and the final one-
+    and two-site density matrices), and the BP-estimated and sign-
+    configuration energy per spin of each instance."""
+    import torch
+    from dequantizer_amd import PRNGKey
+    from dequantizer_amd.annealing import Annealer, edge_colored, get_scheduler, linear_schedule
+    from dequantizer_amd.drivers import annealing_instances
+    t0 = time.perf_counter()
+    efs = [edge_colored(ef) for ef in annealing_instances(args.batch_instances)]
+    keys = [PRNGKey(1000 + k) for k in range(len(efs))]
+    schedule = get_scheduler(args.dt, linear_schedule, args.schedule_steps)
+    ann = Annealer(efs, keys, max_chi=args.chi, layers_per_regauging=1, accuracy=1e-5,
+                   max_belief_propagation_iterations=100, synchronous_update=args.sync,
+                   traversal_type=args.traversal, dtype=torch.complex64, chi=args.chi,
+                   fixed_sweeps=args.fixed_sweeps or None)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for k, (mix, cpl) in enumerate(schedule):
+        ann.step(mix, cpl)
+        if k % 50 == 0:
+            print(f"bench: batched step {k} queued", file=sys.stderr, flush=True)
+    res = ann.finish(two_site=True)
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t0
+    n_eu = sum(sum(r.bp_iterations) * 2 * ef.edges_number for r, ef in zip(res, efs))
+    n_gates = len(schedule) * sum(ef.edges_number for ef in efs)
+    e_bp = np.asarray([r.bp_energy / ef.nodes_number for r, ef in zip(res, efs)])
+    e_sign = np.asarray([r.sign_energy / ef.nodes_number for r, ef in zip(res, efs)])
+    return {"instances": len(efs), "nodes": int(sum(ef.nodes_number for ef in efs)),
+            "edges": int(sum(ef.edges_number for ef in efs)), "steps": len(schedule), "seconds": seconds,
+            "setup_seconds": setup, "value": n_eu / seconds, "unit": "edge-updates/s",
+            "su_gates_per_s": n_gates / seconds, "ms_per_step": 1e3 * seconds / len(schedule),
+            "energy_per_spin_bp": [round(float(x), 6) for x in e_bp],
+            "energy_per_spin_sign": [round(float(x), 6) for x in e_sign],
+            "energy_per_spin_bp_mean": float(e_bp.mean())}
 
 
 def adjoint_timing(ef, args):
