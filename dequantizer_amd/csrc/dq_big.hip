@@ -1,0 +1,289 @@
+// Node updates and the Vidal absorb for large node tensors: S = chi^d p of a
+// million elements and more (degree 6 at chi = 16 is 33.5 M elements, 256 MiB
+// in complex64), where one workgroup per node leaves the chip idle and the
+// tensors are far beyond LDS.  Same arithmetic as the generic kernels
+// (pass_through_node, dequantizer/src/tensor_ops.py:36-60; absorb,
+// tensor_ops.py:88-93), split into HBM-streaming launches over all nodes of a
+// level at once:
+//   mode product   dst = src x_axis M: one thread per column (all indices but
+//                  the axis), the chi values of the column in registers, M in
+//                  LDS; grid = column blocks x nodes.  Output slot idx of a
+//                  degree-d node takes the d - 1 products of the other axes
+//                  (product t acts on axis t, or t + 1 from idx on), ping-
+//                  ponging between two per-node buffers.
+//   Gram partial   conj(T) . U over a chunk of columns per block, columns
+//                  staged through LDS in tiles of 64, FP64 accumulation.
+//   Gram reduce    per node: sum of the chunks, division by the trace.
+// Each element of T and of the intermediates is read once per pass: the
+// launches are bound by HBM bandwidth.
+#include "dq_launch.hpp"
+
+namespace dq {
+
+constexpr int kBigThreads = 256;
+constexpr int kBigTile = 64;  // Gram columns per LDS tile
+
+__host__ __device__ inline int big_axis(int t, int idx) { return t < idx ? t : t + 1; }
+
+template <typename R, int CHI, bool kAbsorb>
+__global__ void __launch_bounds__(kBigThreads)
+    k_big_mode_product(DevGraph g, const int* nodes, int t, int idx, const cx<R>* T, const cx<R>* n2e_old,
+                       const cx<R>* e2n_old, const cx<R>* n2e_new, const uint8_t* node_src, const cd* X,
+                       cx<R>* buf, long long smax, const int* inst_active) {
+  constexpr int CC = CHI * CHI;
+  __shared__ cx<R> Ms[CC];
+  const int k = blockIdx.y;
+  const int node = nodes[k];
+  if (inst_active && !inst_active[g.node_inst[node]]) return;
+  const int d = g.deg[node];
+  const int axis = kAbsorb ? t : big_axis(t, idx);
+  if (axis >= d || (!kAbsorb && idx >= d)) return;
+  const int base = g.adj_ptr[node];
+  const long long S = ipow(CHI, d) * g.phys;
+  const long long inner = ipow(CHI, d - 1 - axis) * g.phys;
+  const long long cols = S / CHI;
+  const int e = g.adj_edge[base + axis], sd = g.adj_side[base + axis];
+  if (kAbsorb) {
+    // Ms[a][b] = X[b][a]: the orthogonaliser's column index is the new bond
+    const cd* x = X + (2LL * e + sd) * CC;
+    for (int q = threadIdx.x; q < CC; q += blockDim.x) Ms[q] = cvt<R>(x[(q % CHI) * CHI + q / CHI]);
+  } else {
+    // the incoming message, applied on the ket side: out[a] = sum_b M[a][b] in[b]
+    const int code = node_src ? node_src[base + axis] : 0;
+    const cx<R>* m = code == 0 ? e2n_old + (2LL * e + sd) * CC
+                               : (code == 1 ? n2e_new : n2e_old) + (2LL * e + 1 - sd) * CC;
+    for (int q = threadIdx.x; q < CC; q += blockDim.x) Ms[q] = m[q];
+  }
+  __syncthreads();
+  const cx<R>* src = t == 0 ? T + g.toff[node] : buf + (2LL * k + ((t - 1) & 1)) * smax;
+  cx<R>* dst = buf + (2LL * k + (t & 1)) * smax;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < cols;
+       c += (long long)gridDim.x * blockDim.x) {
+    const long long o = c / inner, i = c - o * inner;
+    const long long b0 = o * CHI * inner + i;
+    cx<R> in[CHI];
+#pragma unroll
+    for (int b = 0; b < CHI; ++b) in[b] = src[b0 + b * inner];
+#pragma unroll 2
+    for (int a = 0; a < CHI; ++a) {
+      cx<R> acc = mk<R>(0, 0);
+#pragma unroll
+      for (int b = 0; b < CHI; ++b) fma_acc(acc, Ms[a * CHI + b], in[b]);
+      dst[b0 + a * inner] = acc;
+    }
+  }
+}
+
+// Partial Gram of output slot idx over this block's columns:
+// part[(k * chunks + chunk)][kk * CHI + l] = sum conj(T[..kk..]) U[..l..].
+template <typename R, int CHI>
+__global__ void __launch_bounds__(kBigThreads)
+    k_big_gram(DevGraph g, const int* nodes, int idx, const cx<R>* T, const cx<R>* buf, long long smax, cd* part,
+               const int* inst_active) {
+  constexpr int CC = CHI * CHI, PP = CC / kBigThreads;
+  static_assert(PP >= 1 && PP * kBigThreads == CC, "one or more (k, l) pairs per thread");
+  __shared__ cx<R> Tt[kBigTile][CHI + 1], Ut[kBigTile][CHI + 1];
+  const int k = blockIdx.y;
+  const int node = nodes[k];
+  if (inst_active && !inst_active[g.node_inst[node]]) return;
+  const int d = g.deg[node];
+  if (idx >= d) return;
+  const cx<R>* t = T + g.toff[node];
+  const cx<R>* u = d >= 2 ? buf + (2LL * k + ((d - 2) & 1)) * smax : t;
+  const long long S = ipow(CHI, d) * g.phys;
+  const long long inner = ipow(CHI, d - 1 - idx) * g.phys;
+  const long long cols = S / CHI;
+  double ar[PP], ai[PP];
+#pragma unroll
+  for (int v = 0; v < PP; ++v) ar[v] = ai[v] = 0.0;
+  for (long long c0 = (long long)blockIdx.x * kBigTile; c0 < cols; c0 += (long long)gridDim.x * kBigTile) {
+    for (int q = threadIdx.x; q < kBigTile * CHI; q += blockDim.x) {
+      const int r = q / kBigTile, cc = q % kBigTile;
+      const long long c = c0 + cc;
+      cx<R> a = mk<R>(0, 0), b = a;
+      if (c < cols) {
+        const long long o = c / inner, i = c - o * inner;
+        const long long off = o * CHI * inner + i + r * inner;
+        a = t[off];
+        b = u[off];
+      }
+      Tt[cc][r] = a;
+      Ut[cc][r] = b;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < PP; ++v) {
+      const int pr = threadIdx.x + v * kBigThreads;
+      const int kk = pr / CHI, l = pr % CHI;
+      double sr = 0.0, si = 0.0;
+#pragma unroll 8
+      for (int cc = 0; cc < kBigTile; ++cc) {
+        const cx<R> a = Tt[cc][kk], b = Ut[cc][l];
+        sr += (double)a.re * b.re + (double)a.im * b.im;
+        si += (double)a.re * b.im - (double)a.im * b.re;
+      }
+      ar[v] += sr;
+      ai[v] += si;
+    }
+    __syncthreads();
+  }
+  cd* pp = part + ((long long)k * gridDim.x + blockIdx.x) * CC;
+#pragma unroll
+  for (int v = 0; v < PP; ++v) pp[threadIdx.x + v * kBigThreads] = mk<double>(ar[v], ai[v]);
+}
+
+template <typename R, int CHI>
+__global__ void __launch_bounds__(kBigThreads)
+    k_big_gram_reduce(DevGraph g, const int* nodes, int idx, const cd* part, int chunks, cx<R>* n2e_new,
+                      const int* inst_active) {
+  constexpr int CC = CHI * CHI;
+  __shared__ cd G[CC];
+  __shared__ cd tr;
+  const int k = blockIdx.x;
+  const int node = nodes[k];
+  if (inst_active && !inst_active[g.node_inst[node]]) return;
+  if (idx >= g.deg[node]) return;
+  for (int q = threadIdx.x; q < CC; q += blockDim.x) {
+    cd s = mk<double>(0, 0);
+    for (int c = 0; c < chunks; ++c) s = s + part[((long long)k * chunks + c) * CC + q];
+    G[q] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    cd s = mk<double>(0, 0);
+    for (int a = 0; a < CHI; ++a) s = s + G[a * CHI + a];
+    tr = s;
+  }
+  __syncthreads();
+  const int base = g.adj_ptr[node];
+  const int e = g.adj_edge[base + idx], sd = g.adj_side[base + idx];
+  cx<R>* dst = n2e_new + (2LL * e + sd) * CC;
+  for (int q = threadIdx.x; q < CC; q += blockDim.x) dst[q] = cvt<R>(cdiv(G[q], tr));
+}
+
+// Absorb epilogue: per-chunk sums of |x|^2 of the final product, then
+// Gamma = x / |x| written to the node's slot.
+template <typename R>
+__global__ void __launch_bounds__(kBigThreads)
+    k_big_norm(DevGraph g, const int* nodes, const cx<R>* buf, long long smax, int chi, double* part) {
+  __shared__ double red[kBigThreads / 64];
+  const int k = blockIdx.y;
+  const int d = g.deg[nodes[k]];
+  const long long S = ipow(chi, d) * g.phys;
+  const cx<R>* x = buf + (2LL * k + ((d - 1) & 1)) * smax;
+  double acc = 0.0;
+  if (d >= 1)
+    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < S; f += (long long)gridDim.x * blockDim.x)
+      acc += (double)norm2(x[f]);
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) part[(long long)k * gridDim.x + blockIdx.x] = acc;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBigThreads)
+    k_big_scale_out(DevGraph g, const int* nodes, const cx<R>* buf, long long smax, int chi, const double* part,
+                    int chunks, cx<R>* gam, const int* inst_active) {
+  const int k = blockIdx.y;
+  const int node = nodes[k];
+  if (inst_active && !inst_active[g.node_inst[node]]) return;
+  const int d = g.deg[node];
+  if (d < 1) return;
+  double s = 0.0;
+  for (int c = 0; c < chunks; ++c) s += part[(long long)k * chunks + c];
+  const R inv = (R)(1.0 / sqrt(s));
+  const long long S = ipow(chi, d) * g.phys;
+  const cx<R>* x = buf + (2LL * k + ((d - 1) & 1)) * smax;
+  cx<R>* out = gam + g.toff[node];
+  for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < S; f += (long long)gridDim.x * blockDim.x)
+    out[f] = inv * x[f];
+}
+
+// ------------------------------------------------------------- launchers
+static int big_col_blocks(long long cols) {
+  long long b = (cols + kBigThreads - 1) / kBigThreads;
+  return (int)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+}
+static int big_gram_chunks(long long cols) {
+  long long b = (cols + kBigTile - 1) / kBigTile;
+  return (int)(b < 1 ? 1 : (b > 256 ? 256 : b));
+}
+
+bool big_applicable(const DevGraph& g, int max_deg) {
+  return (g.chi == 16 || g.chi == 32) && ipow(g.chi, max_deg) * g.phys >= (1LL << 20);
+}
+
+long long big_ws_bytes(const DevGraph& g, int max_deg, int count, int rb) {
+  const long long smax = ipow(g.chi, max_deg) * g.phys;
+  const long long cols = smax / g.chi;
+  const long long part = (long long)count * big_gram_chunks(cols) * g.chi * g.chi * (long long)sizeof(cd);
+  const long long norm = (long long)count * big_col_blocks(smax) * (long long)sizeof(double);
+  return 2LL * count * smax * 2 * rb + (part > norm ? part : norm) + 256;
+}
+
+template <typename R, int CHI>
+static hipError_t big_bp_level(const DevGraph& g, int max_deg, const int* nodes, int count, const void* T,
+                               const void* n2e_old, const void* e2n_old, void* n2e_new, const uint8_t* node_src,
+                               void* ws, const int* inst_active, hipStream_t s) {
+  const long long smax = ipow(CHI, max_deg) * g.phys;
+  const long long cols = smax / CHI;
+  cx<R>* buf = (cx<R>*)ws;
+  cd* part = (cd*)((char*)ws + 2LL * count * smax * sizeof(cx<R>));
+  const int chunks = big_gram_chunks(cols);
+  for (int idx = 0; idx < max_deg; ++idx) {
+    for (int t = 0; t + 1 < max_deg; ++t)
+      hipLaunchKernelGGL((k_big_mode_product<R, CHI, false>), dim3(big_col_blocks(cols), count), dim3(kBigThreads), 0,
+                         s, g, nodes, t, idx, (const cx<R>*)T, (const cx<R>*)n2e_old, (const cx<R>*)e2n_old,
+                         (const cx<R>*)n2e_new, node_src, (const cd*)nullptr, buf, smax, inst_active);
+    hipLaunchKernelGGL((k_big_gram<R, CHI>), dim3(chunks, count), dim3(kBigThreads), 0, s, g, nodes, idx,
+                       (const cx<R>*)T, (const cx<R>*)buf, smax, part, inst_active);
+    hipLaunchKernelGGL((k_big_gram_reduce<R, CHI>), dim3(count), dim3(kBigThreads), 0, s, g, nodes, idx,
+                       (const cd*)part, chunks, (cx<R>*)n2e_new, inst_active);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_bp_nodes_big(int rb, const DevGraph& g, int max_deg, const int* nodes, int count, const void* T,
+                               const void* n2e_old, const void* e2n_old, void* n2e_new, const uint8_t* node_src,
+                               void* ws, const int* inst_active, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (rb == 4)
+    return g.chi == 16 ? big_bp_level<float, 16>(g, max_deg, nodes, count, T, n2e_old, e2n_old, n2e_new, node_src,
+                                                 ws, inst_active, s)
+                       : big_bp_level<float, 32>(g, max_deg, nodes, count, T, n2e_old, e2n_old, n2e_new, node_src,
+                                                 ws, inst_active, s);
+  return g.chi == 16 ? big_bp_level<double, 16>(g, max_deg, nodes, count, T, n2e_old, e2n_old, n2e_new, node_src, ws,
+                                                inst_active, s)
+                     : big_bp_level<double, 32>(g, max_deg, nodes, count, T, n2e_old, e2n_old, n2e_new, node_src, ws,
+                                                inst_active, s);
+}
+
+template <typename R, int CHI>
+static hipError_t big_absorb(const DevGraph& g, int max_deg, const int* nodes, int count, const void* T,
+                             const cd* X, void* gam, void* ws, const int* inst_active, hipStream_t s) {
+  const long long smax = ipow(CHI, max_deg) * g.phys;
+  const long long cols = smax / CHI;
+  cx<R>* buf = (cx<R>*)ws;
+  double* part = (double*)((char*)ws + 2LL * count * smax * sizeof(cx<R>));
+  for (int t = 0; t < max_deg; ++t)
+    hipLaunchKernelGGL((k_big_mode_product<R, CHI, true>), dim3(big_col_blocks(cols), count), dim3(kBigThreads), 0,
+                       s, g, nodes, t, 0, (const cx<R>*)T, (const cx<R>*)nullptr, (const cx<R>*)nullptr,
+                       (const cx<R>*)nullptr, (const uint8_t*)nullptr, X, buf, smax, inst_active);
+  const int nb = big_col_blocks(smax);
+  hipLaunchKernelGGL((k_big_norm<R>), dim3(nb, count), dim3(kBigThreads), 0, s, g, nodes, (const cx<R>*)buf, smax,
+                     CHI, part);
+  hipLaunchKernelGGL((k_big_scale_out<R>), dim3(nb, count), dim3(kBigThreads), 0, s, g, nodes, (const cx<R>*)buf,
+                     smax, CHI, (const double*)part, nb, (cx<R>*)gam, inst_active);
+  return hipGetLastError();
+}
+
+hipError_t launch_absorb_big(int rb, const DevGraph& g, int max_deg, const int* nodes, int count, const void* T,
+                             const cd* X, void* gam, void* ws, const int* inst_active, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (rb == 4)
+    return g.chi == 16 ? big_absorb<float, 16>(g, max_deg, nodes, count, T, X, gam, ws, inst_active, s)
+                       : big_absorb<float, 32>(g, max_deg, nodes, count, T, X, gam, ws, inst_active, s);
+  return g.chi == 16 ? big_absorb<double, 16>(g, max_deg, nodes, count, T, X, gam, ws, inst_active, s)
+                     : big_absorb<double, 32>(g, max_deg, nodes, count, T, X, gam, ws, inst_active, s);
+}
+
+}  // namespace dq

@@ -276,6 +276,18 @@ int dq_bp_sweep(dq_graph* g, int dtype, const void* T, const void* n2e_in, const
   if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
   if (n2e_out == n2e_in || e2n_out == e2n_in || n2e_out == e2n_in || e2n_out == n2e_in)
     return fail(DQ_ERR_INVALID, "bp sweep outputs must not alias inputs");
+  if (big_applicable(g->dev, g->max_deg)) {
+    if ((rc = ws_ensure(g, (size_t)big_ws_bytes(g->dev, g->max_deg, g->max_level > 0 ? g->max_level : 1, rb))) !=
+        DQ_OK)
+      return rc;
+    for (int l = 0; l < g->n_levels; ++l) {
+      const int b = g->level_ptr[l], cnt = g->level_ptr[l + 1] - b;
+      RET_LAUNCH(launch_bp_nodes_big(rb, g->dev, g->max_deg, g->d_level_nodes + b, cnt, T, n2e_in, e2n_in, n2e_out,
+                                     g->d_node_src, g->ws, inst_active, S(stream)));
+    }
+    RET_LAUNCH(launch_bp_edges(rb, g->dev, n2e_out, n2e_in, e2n_out, g->d_edge_src, inst_active, S(stream)));
+    return DQ_OK;
+  }
   const long long wsb = (node_ws_bytes(g, rb) + 255) / 256 * 256;
   if ((rc = ws_ensure(g, (size_t)wsb * (g->max_level > 0 ? g->max_level : 1))) != DQ_OK) return rc;
   for (int l = 0; l < g->n_levels; ++l) {
@@ -315,9 +327,16 @@ int dq_vidal_gauge(dq_graph* g, int dtype, const void* T, const void* n2e, doubl
   const long long wsn = (node_ws_bytes(g, rb) + 255) / 256 * 256;
   const size_t need_e = (size_t)wse * (g->dev.n_edges > 0 ? g->dev.n_edges : 1);
   const size_t need_n = (size_t)wsn * (g->dev.n_nodes > 0 ? g->dev.n_nodes : 1);
-  if ((rc = ws_ensure(g, need_e > need_n ? need_e : need_n)) != DQ_OK) return rc;
+  const bool big = big_applicable(g->dev, g->max_deg);
+  const size_t need_b = big ? (size_t)big_ws_bytes(g->dev, g->max_deg, g->dev.n_nodes, rb) : 0;
+  if ((rc = ws_ensure(g, need_e > need_n ? (need_e > need_b ? need_e : need_b) : (need_n > need_b ? need_n : need_b))) !=
+      DQ_OK)
+    return rc;
   RET_LAUNCH(launch_edge_svd(rb, g->dev, n2e, eps, g->X, lam, g->ws, wse, inst_active, nullptr, S(stream)));
-  if (bp_fast_applicable(rb, g->dev, g->min_deg, g->max_deg))
+  if (big)
+    RET_LAUNCH(launch_absorb_big(rb, g->dev, g->max_deg, g->d_all_nodes, g->dev.n_nodes, T, g->X, gammas, g->ws,
+                                 inst_active, S(stream)));
+  else if (bp_fast_applicable(rb, g->dev, g->min_deg, g->max_deg))
     RET_LAUNCH(launch_absorb_fast(g->dev, T, g->X, gammas, inst_active, S(stream)));
   else
     RET_LAUNCH(launch_absorb(rb, g->dev, T, g->X, gammas, g->ws, wsn, inst_active, S(stream)));
@@ -336,9 +355,16 @@ int dq_vidal_gauge_tape(dq_graph* g, int dtype, const void* T, const void* n2e, 
   const long long wsn = (node_ws_bytes(g, rb) + 255) / 256 * 256;
   const size_t need_e = (size_t)wse * (g->dev.n_edges > 0 ? g->dev.n_edges : 1);
   const size_t need_n = (size_t)wsn * (g->dev.n_nodes > 0 ? g->dev.n_nodes : 1);
-  if ((rc = ws_ensure(g, need_e > need_n ? need_e : need_n)) != DQ_OK) return rc;
+  const bool big = big_applicable(g->dev, g->max_deg);
+  const size_t need_b = big ? (size_t)big_ws_bytes(g->dev, g->max_deg, g->dev.n_nodes, rb) : 0;
+  if ((rc = ws_ensure(g, need_e > need_n ? (need_e > need_b ? need_e : need_b) : (need_n > need_b ? need_n : need_b))) !=
+      DQ_OK)
+    return rc;
   RET_LAUNCH(launch_edge_svd(rb, g->dev, n2e, eps, (cd*)X, lam, g->ws, wse, nullptr, tape, S(stream)));
-  if (bp_fast_applicable(rb, g->dev, g->min_deg, g->max_deg))
+  if (big)
+    RET_LAUNCH(launch_absorb_big(rb, g->dev, g->max_deg, g->d_all_nodes, g->dev.n_nodes, T, (const cd*)X, gammas,
+                                 g->ws, nullptr, S(stream)));
+  else if (bp_fast_applicable(rb, g->dev, g->min_deg, g->max_deg))
     RET_LAUNCH(launch_absorb_fast(g->dev, T, (const cd*)X, gammas, nullptr, S(stream)));
   else
     RET_LAUNCH(launch_absorb(rb, g->dev, T, (const cd*)X, gammas, g->ws, wsn, nullptr, S(stream)));

@@ -42,6 +42,16 @@ hipError_t launch_bp_nodes(int real_bytes, const DevGraph& g, const int* nodes, 
 bool bp_fast_applicable(int real_bytes, const DevGraph& g, int min_deg, int max_deg);
 hipError_t launch_absorb_fast(const DevGraph& g, const void* T, const cd* X, void* gammas, const int* inst_active,
                               hipStream_t s);
+// Large node tensors (chi in {16, 32}, chi^max_deg * phys >= 2^20): node
+// updates and the absorb as HBM-streaming launches over all nodes at once.
+bool big_applicable(const DevGraph& g, int max_deg);
+long long big_ws_bytes(const DevGraph& g, int max_deg, int count, int real_bytes);
+hipError_t launch_bp_nodes_big(int real_bytes, const DevGraph& g, int max_deg, const int* nodes, int count,
+                               const void* T, const void* n2e_old, const void* e2n_old, void* n2e_new,
+                               const uint8_t* node_src, void* ws, const int* inst_active, hipStream_t s);
+hipError_t launch_absorb_big(int real_bytes, const DevGraph& g, int max_deg, const int* nodes, int count,
+                             const void* T, const cd* X, void* gammas, void* ws, const int* inst_active,
+                             hipStream_t s);
 hipError_t launch_bp_nodes_fast(const DevGraph& g, const int* nodes, int count, const void* T, const void* n2e_old,
                                 const void* e2n_old, void* n2e_new, const uint8_t* node_src, const int* inst_active,
                                 hipStream_t s);

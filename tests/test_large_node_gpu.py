@@ -1,0 +1,88 @@
+"""The node update at its largest configured shape: degree 6, chi = 16,
+phys 2 (one node tensor of 33.5 M elements, 256 MiB in complex64), as on the
+16^3 periodic lattice.  A star whose centre has six chi = 16 bonds is a tree,
+so its BP messages are exact marginals; one synchronous sweep is compared
+with the oracle (complex128 to rounding, complex64 to FP32 accumulation)."""
+import numpy as np
+import pytest
+
+from tests.conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
+
+import dequantizer_amd as dq  # noqa: E402
+from oracle import graph as og  # noqa: E402
+from oracle import mappings as om  # noqa: E402
+from tests.helpers import host, to_oracle_messages  # noqa: E402
+
+
+def _star():
+    gp, go = dq.TensorGraph(), og.TensorGraph()
+    for g in (gp, go):
+        for _ in range(7):
+            g.add_node(2)
+        for i in range(1, 7):
+            g.add_edge((0, i), 16)
+    return gp, go
+
+
+@pytest.fixture(scope="module")
+def star_case():
+    gp, go = _star()
+    rng = np.random.default_rng(3)
+    tensors = {}
+    for n in go.nodes.values():
+        shape = n.bond_shape + (2,)
+        tensors[n.id] = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2.0 * 16 ** 6)
+    msgs = {}
+    for el in gp.get_traversal_iterator():
+        for nb in el.neighbors:
+            a = rng.standard_normal((16, 16)) + 1j * rng.standard_normal((16, 16))
+            m = a @ a.conj().T
+            msgs[dq.MessageID(el.id, nb.id)] = m / np.trace(m)
+    trav_o = go.traversal()
+    ref = om.belief_propagation_map(trav_o, True)(tensors, to_oracle_messages(msgs))
+    return gp, tensors, msgs, ref
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-11), (np.complex64, 2e-5)])
+def test_degree6_chi16_node_update_matches_oracle(star_case, dtype, tol):
+    gp, tensors, msgs, ref = star_case
+    bp = dq.get_belief_propagation_map(list(gp.get_traversal_iterator()), True)
+    out = bp({k: v.astype(dtype) for k, v in tensors.items()}, {k: v.astype(dtype) for k, v in msgs.items()})
+    worst = 0.0
+    for k, v in ref.items():
+        got = host(out[dq.MessageID(k.src, k.dst)])
+        worst = max(worst, np.linalg.norm(got - v) / np.linalg.norm(v))
+        if k.src == 0:  # the centre's outputs: Hermitian, unit trace
+            assert abs(np.trace(got) - 1.0) < 10 * tol
+            assert np.abs(got - got.conj().T).max() < 10 * tol
+    assert worst < tol, worst
+
+
+def test_degree6_chi16_vidal_gauge_matches_oracle(star_case):
+    """BP on the star converges in two synchronous sweeps (a tree of depth
+    one); the Vidal gauge of the converged messages (edge decompositions and
+    the absorb into the 33.5 M-element centre) against the oracle: lambdas to
+    rounding and the symmetric-gauge one-site density matrices."""
+    gp, tensors, msgs, _ = star_case
+    go = _star()[1]
+    trav_p, trav_o = list(gp.get_traversal_iterator()), go.traversal()
+    bp_o = om.belief_propagation_map(trav_o, True)
+    mo = to_oracle_messages(msgs)
+    for _ in range(2):
+        mo = bp_o(tensors, mo)
+    gam_o, lam_o = om.vidal_gauge_fixing_map(trav_o, 1e-10)(tensors, mo)
+    mp = {dq.MessageID(k.src, k.dst): v for k, v in mo.items()}
+    gam_p, lam_p = dq.get_vidal_gauge_fixing_map(trav_p, 1e-10)(tensors, mp)
+    for eid, lm in lam_o.items():
+        assert np.abs(host(lam_p[eid]) - lm).max() < 1e-10, eid
+    gam_h = {nid: host(gam_p[nid]) for nid in gam_o}
+    lam_h = {eid: host(lam_p[eid]) for eid in lam_o}
+    sg = om.symmetric_gauge_fixing_map(trav_o, 1e-10)
+    t_o, t_p = sg(gam_o, lam_o), sg(gam_h, lam_h)
+    m_o, m_p = om.lambdas2messages(lam_o), om.lambdas2messages(lam_h)
+    for node in go.nodes.values():
+        r_o = om.one_side_density_matrix(node, t_o, m_o)
+        r_p = om.one_side_density_matrix(node, t_p, m_p)
+        assert np.abs(r_o - r_p).max() < 1e-9, node.id
