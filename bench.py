@@ -22,6 +22,11 @@ import numpy as np
 METRIC = "edge-updates/s & SU gates/s at χ=16 vs plain GPU and CPU; fwd+adjoint time"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+# HBM bytes of one BP sweep (node kernel + edge kernel) of the default
+# workload (N=1000, chi=16, complex64), from FETCH_SIZE / WRITE_SIZE counter
+# passes scaled by a 256 MiB copy (FETCH_SIZE reads half its bytes, WRITE_SIZE
+# all): 2 x (35301.2 + 3055.6) KiB read + (6000 + 6000) KiB written.
+TRAFFIC_BP_SWEEP_DEFAULT = 2 * (35301.2 + 3055.6) * 1024 + (6000.0 + 6000.0) * 1024
 
 
 def eu_flops(d: int, chi: int, p: int) -> int:
@@ -60,6 +65,12 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=64,
                     help="instances of the batched annealing leg (the whole 200-step run of the random 3-regular "
                          "instance set, N in 100..1000, in one engine); 0 skips it")
+    ap.add_argument("--lattice-size", type=int, default=4,
+                    help="L of the periodic L^3 lattice leg (degree 6, chi 16, phys 2, Gaussian tensors): BP to "
+                         "1e-8 and the Vidal gauge in complex64 and complex128; 0 skips it")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="before the timed steps, one device-to-device copy of this many MiB (a kernel of known "
+                         "traffic against which hardware byte counters are scaled); 0 skips it")
     ap.add_argument("--gate-order", default="colored", choices=["colored", "pairs"],
                     help="coupling-gate order: pair list sorted by edge colour, or the generator's pair order")
     args = ap.parse_args()
@@ -96,6 +107,12 @@ def main():
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t_build
 
+    if args.copy_probe_mb:
+        src = torch.ones(args.copy_probe_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     it_before = sum(emu.bp_iterations[0])
     emu.sweep_events = []
     torch.cuda.synchronize()
@@ -156,7 +173,8 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": (achieved_tflops / PEAK_FP32_TFLOPS) if achieved_tflops else None,
-            "traffic": None,
+            "traffic": (TRAFFIC_BP_SWEEP_DEFAULT if (args.nodes, args.chi) == (1000, 16) else None),
+            "traffic_unit": "B per launch",
             "algorithmic_flops_per_launch": flops_per_sweep,
             "algorithmic_bytes_per_launch": bytes_per_sweep,
             "avg_launch_s": sweep_s,
@@ -167,6 +185,10 @@ def main():
     if args.batch_instances:
         result["batched"] = batched_run(args)
         print(f"bench: batched run done: {result['batched']['seconds']:.1f} s", file=sys.stderr, flush=True)
+    if args.lattice_size:
+        result["lattice"] = lattice_run(args)
+        print(f"bench: lattice leg done: {result['lattice']['sweep_ms_c64']:.0f} ms per complex64 sweep",
+              file=sys.stderr, flush=True)
     if args.adjoint_steps:
         result["fwd_adjoint"] = adjoint_timing(ef, args)
         print(f"bench: forward+adjoint done: {result['fwd_adjoint']['seconds']:.1f} s", file=sys.stderr, flush=True)
@@ -223,6 +245,71 @@ def batched_run(args):
             "energy_per_spin_bp_mean": float(e_bp.mean())}
 
 
+def lattice_run(args, chi=16, tol=1e-8, max_sweeps=60):
+    """Periodic L^3 cubic lattice, every node of degree 6, chi = 16, phys 2
+    (256 MiB per node tensor in complex64): node tensors with entries
+    (x + iy) / sqrt(2 chi^6), x, y ~ N(0, 1) (torch generator, seed 5),
+    messages A A^H / tr.  Synchronous BP until the mean message change is
+    below 1e-8, then the Vidal gauge, in complex64 and in complex128; the
+    complex64 messages and lambdas are compared with the complex128 ones
+    (that path matches the oracle at this node shape in the tests)."""
+    import torch
+    import dequantizer_amd as dq
+    lat = dq.get_nd_lattice([args.lattice_size] * 3, 2, chi, False)
+    trav = list(lat.get_traversal_iterator())
+    out = {"L": args.lattice_size, "nodes": lat.nodes_number, "edges": lat.edges_number, "degree": 6, "chi": chi}
+    res = {}
+    for name, dt, np_dt in (("c64", torch.complex64, np.complex64), ("c128", torch.complex128, np.complex128)):
+        gen = torch.Generator(device="cuda").manual_seed(5)
+        tensors = {}
+        for el in trav:
+            if isinstance(el, dq.Node):
+                x = torch.randn((*el.bond_shape, 2), dtype=torch.float64, device="cuda", generator=gen)
+                y = torch.randn((*el.bond_shape, 2), dtype=torch.float64, device="cuda", generator=gen)
+                tensors[el.id] = (torch.complex(x, y) / np.sqrt(2.0 * chi ** 6)).to(dt)
+                del x, y
+        rng = np.random.default_rng(5)
+        msgs = {}
+        for el in trav:
+            for nb in el.neighbors:
+                a = rng.standard_normal((chi, chi)) + 1j * rng.standard_normal((chi, chi))
+                m = a @ a.conj().T
+                msgs[dq.MessageID(el.id, nb.id)] = (m / np.trace(m)).astype(np_dt)
+        bp = dq.get_belief_propagation_map(trav, True)
+        vg = dq.get_vidal_gauge_fixing_map(trav, 1e-10)
+        m = bp(tensors, msgs)  # first sweep: upload and workspace set-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sweeps, r = 1, float("inf")
+        while sweeps < max_sweeps:
+            new = bp(tensors, m)
+            r = float(dq.messages_frob_distance(new, m))
+            m = new
+            sweeps += 1
+            if r < tol:
+                break
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        gam, lam = vg(tensors, m)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        out[f"sweeps_{name}"] = sweeps
+        out[f"residual_{name}"] = r
+        out[f"sweep_ms_{name}"] = 1e3 * (t1 - t0) / (sweeps - 1)
+        out[f"vidal_gauge_ms_{name}"] = 1e3 * (t2 - t1)
+        res[name] = ({k: v.cpu().numpy() for k, v in m.items()}, {k: v.cpu().numpy() for k, v in lam.items()})
+        del tensors, gam, lam, m, new, bp, vg
+        torch.cuda.empty_cache()
+    eu = 2 * lat.edges_number
+    out["eu_per_s_c64"] = eu / (out["sweep_ms_c64"] * 1e-3)
+    out["algorithmic_tflops_c64"] = eu * eu_flops(6, chi, 2) / (out["sweep_ms_c64"] * 1e-3) / 1e12
+    (m64, l64), (m128, l128) = res["c64"], res["c128"]
+    out["message_rel_err_c64_vs_c128"] = float(max(np.linalg.norm(m64[k] - m128[k]) / np.linalg.norm(m128[k])
+                                                   for k in m128))
+    out["lambda_abs_err_c64_vs_c128"] = float(max(np.abs(l64[k] - l128[k]).max() for k in l128))
+    return out
+
+
 def adjoint_timing(ef, args):
     """Energy of the annealing run and its gradient with respect to the
     schedule knots and every coupling (hand-written adjoint, complex128,
@@ -238,6 +325,7 @@ def adjoint_timing(ef, args):
     setup = time.perf_counter() - t0
     knots = np.linspace(0.0, 1.0, args.knots)
     torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats()
     t0 = time.perf_counter()
     e_fwd = run.value(knots)
     torch.cuda.synchronize()
