@@ -138,6 +138,8 @@ hipError_t launch_bp_nodes(int rb, const DevGraph& g, const int* nodes, int coun
   if (count <= 0) return hipSuccess;
   if (bp_fast_applicable(rb, g, min_deg, max_deg))
     return launch_bp_nodes_fast(g, nodes, count, T, n2e_old, e2n_old, n2e_new, node_src, inst_active, s);
+  if (rb == 8 && g.chi == 16 && g.phys == 2 && min_deg == 3 && max_deg == 3)
+    return launch_bp_nodes_fast64(g, nodes, count, T, n2e_old, e2n_old, n2e_new, node_src, inst_active, s);
   if (rb == 4)
     hipLaunchKernelGGL(k_bp_nodes<float>, dim3(count), dim3(256), 0, s, g, nodes, (const cx<float>*)T,
                        (const cx<float>*)n2e_old, (const cx<float>*)e2n_old, (cx<float>*)n2e_new, node_src,

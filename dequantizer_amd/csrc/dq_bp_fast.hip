@@ -193,6 +193,146 @@ __global__ void __launch_bounds__(64 * fast3::NW)
   }
 }
 
+// The same node update in complex128 on FP64 matrix cores
+// (V_MFMA_F64_16X16X4F64: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16];
+// the result D[l / 16 + 4 v][l % 16] stays in four accumulators).  Messages act
+// on bond axes only and the Gram sums over the physical index, so the two
+// physical components are processed one after the other: per component the
+// 16^3 tensor and one work tensor (complex128, element (b0, b1, b2) at
+// b0*273 + b1*17 + b2, bank-spread for 16-byte elements along every axis)
+// take 137 KiB of LDS.  Gram partials stay in registers across both passes
+// and are reduced over the waves at the end.
+namespace fast64 {
+constexpr int RP = 17;
+constexpr int B0P = 16 * RP + 1;
+constexpr int NPAD = 16 * B0P;
+constexpr int NW = 8;
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int axis_stride(int j) { return j == 0 ? B0P : (j == 1 ? RP : 1); }
+// column c (0..255) of the matrix view of mode j: the two other axes, row-major
+__device__ __forceinline__ int col_base(int j, int c) {
+  if (j == 0) return (c >> 4) * RP + (c & 15);
+  if (j == 1) return (c >> 4) * B0P + (c & 15);
+  return (c >> 4) * B0P + (c & 15) * RP;
+}
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+// dst = src x_j M (may be in place); m[ks] = M[lane % 16][4 ks + lane / 16]
+__device__ __forceinline__ void mode_product(cd* dst, const cd* src, const cd (&m)[4], int j, int wave, int lane) {
+  const int st = axis_stride(j);
+  const int col = lane & 15, kq = lane >> 4;
+  for (int tile = wave; tile < 16; tile += NW) {
+    const int cb = col_base(j, tile * 16 + col);
+    d4 ar = {0.0, 0.0, 0.0, 0.0}, ai = ar;
+    cd x[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) x[ks] = src[cb + (4 * ks + kq) * st];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      ar = mfma(m[ks].re, x[ks].re, ar);
+      ar = mfma(-m[ks].im, x[ks].im, ar);
+      ai = mfma(m[ks].re, x[ks].im, ai);
+      ai = mfma(m[ks].im, x[ks].re, ai);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) dst[cb + (kq + 4 * v) * st] = mk<double>(ar[v], ai[v]);
+  }
+}
+// this wave's share of sum_r conj(T[k][r]) U[l][r] along axis idx
+__device__ __forceinline__ void gram_partial(const cd* t, const cd* u, int idx, int wave, int lane, d4& gr, d4& gi) {
+  const int st = axis_stride(idx);
+  const int row = lane & 15, kq = lane >> 4;
+  constexpr int per = 64 / NW;
+  for (int ks = wave * per; ks < wave * per + per; ++ks) {
+    const int off = col_base(idx, 4 * ks + kq) + row * st;
+    const cd a = t[off], b = u[off];
+    gr = mfma(a.re, b.re, gr);
+    gr = mfma(a.im, b.im, gr);
+    gi = mfma(a.re, b.im, gi);
+    gi = mfma(-a.im, b.re, gi);
+  }
+}
+}  // namespace fast64
+
+__global__ void __launch_bounds__(64 * fast64::NW)
+    k_bp_nodes_d3c16_f64(DevGraph g, const int* nodes, const cd* T, const cd* n2e_old, const cd* e2n_old,
+                         cd* n2e_new, const uint8_t* node_src, const int* inst_active) {
+  using namespace fast64;
+  __shared__ cd tl[NPAD];
+  __shared__ cd wk[NPAD];
+  __shared__ cd diag[16];
+  const int node = nodes[blockIdx.x];
+  if (inst_active && !inst_active[g.node_inst[node]]) return;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int base = g.adj_ptr[node];
+  const cd* tg = T + g.toff[node];
+  cd m[3][4];
+  int oe[3], os[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int e = g.adj_edge[base + k], sd = g.adj_side[base + k];
+    oe[k] = e;
+    os[k] = sd;
+    const int code = node_src ? node_src[base + k] : 0;
+    const cd* src = code == 0 ? e2n_old + (2LL * e + sd) * 256
+                              : (code == 1 ? n2e_new : n2e_old) + (2LL * e + 1 - sd) * 256;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) m[k][ks] = src[(lane & 15) * 16 + 4 * ks + (lane >> 4)];
+  }
+  const int plan[3][3] = {{2, 1, 0}, {2, 0, 1}, {0, 1, 2}};
+  d4 gr[3], gi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) gr[k] = gi[k] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int s = 0; s < 2; ++s) {
+    for (int f = tid; f < 4096; f += 64 * NW) tl[(f >> 8) * B0P + ((f >> 4) & 15) * RP + (f & 15)] = tg[2 * f + s];
+    __syncthreads();
+#pragma unroll
+    for (int step = 0; step < 3; ++step) {
+      const int j1 = plan[step][0], j2 = plan[step][1], idx = plan[step][2];
+      mode_product(wk, tl, m[j1], j1, wave, lane);
+      __syncthreads();
+      mode_product(wk, wk, m[j2], j2, wave, lane);
+      __syncthreads();
+      gram_partial(tl, wk, idx, wave, lane, gr[idx], gi[idx]);
+      __syncthreads();
+    }
+  }
+  // reduce the wave partials through LDS (the work tensor is free), one
+  // output at a time, then divide by the trace
+  cd* part = wk;  // NW x 256
+#pragma unroll
+  for (int idx = 0; idx < 3; ++idx) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      part[wave * 256 + ((lane >> 4) + 4 * v) * 16 + (lane & 15)] = mk<double>(gr[idx][v], gi[idx][v]);
+    __syncthreads();
+    cd sum = mk<double>(0.0, 0.0);
+    if (tid < 256) {
+      for (int w = 0; w < NW; ++w) sum = sum + part[w * 256 + tid];
+      if (tid % 17 == 0) diag[tid / 17] = sum;
+    }
+    __syncthreads();
+    if (tid < 256) {
+      cd tr = mk<double>(0.0, 0.0);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) tr = tr + diag[k];
+      n2e_new[(2LL * oe[idx] + os[idx]) * 256 + tid] = cdiv(sum, tr);
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_bp_nodes_fast64(const DevGraph& g, const int* nodes, int count, const void* T, const void* n2e_old,
+                                  const void* e2n_old, void* n2e_new, const uint8_t* node_src, const int* inst_active,
+                                  hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bp_nodes_d3c16_f64, dim3(count), dim3(64 * fast64::NW), 0, s, g, nodes, (const cd*)T,
+                     (const cd*)n2e_old, (const cd*)e2n_old, (cd*)n2e_new, node_src, inst_active);
+  return hipGetLastError();
+}
+
 // Vidal absorb (tensor_ops.py:88-93) on matrix cores for the same shape:
 // Gamma = T x_0 X_0^T x_1 X_1^T x_2 X_2^T / |.|, X_j the orthogonaliser of the
 // node's side of bond j (complex128, rounded to complex64 operands, FP32

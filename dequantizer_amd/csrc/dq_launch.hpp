@@ -40,6 +40,9 @@ hipError_t launch_bp_nodes(int real_bytes, const DevGraph& g, const int* nodes, 
 // Matrix-core node update for graphs whose nodes all have degree 3, chi 16,
 // phys 2, complex64 (dq_bp_fast.hip).
 bool bp_fast_applicable(int real_bytes, const DevGraph& g, int min_deg, int max_deg);
+hipError_t launch_bp_nodes_fast64(const DevGraph& g, const int* nodes, int count, const void* T, const void* n2e_old,
+                                  const void* e2n_old, void* n2e_new, const uint8_t* node_src, const int* inst_active,
+                                  hipStream_t s);
 hipError_t launch_absorb_fast(const DevGraph& g, const void* T, const cd* X, void* gammas, const int* inst_active,
                               hipStream_t s);
 // Large node tensors (chi in {16, 32}, chi^max_deg * phys >= 2^20): node
