@@ -17,6 +17,7 @@
 // the forward's singular-vector phases.
 #include <float.h>
 
+#include "dq_fast64.hpp"
 #include "dq_launch.hpp"
 #include "dq_linalg.hpp"
 
@@ -744,6 +745,171 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The same sweep adjoint for degree-3, chi = 16, phys = 2 nodes on FP64
+// matrix cores, one node per 512-thread block, one physical component of T
+// in LDS at a time.  For output slot idx with the other axes j1 < j2 and
+// forward U = (T x_j1 M1) x_j2 M2, O = G / tr(G), G = sum conj(T) U:
+//   pass 1  G over both components, then Gbar = trace_norm_vjp(G, Obar);
+//   pass 2  per component, with Ubar = T x_idx Gbar^T:
+//     Ubar1 = Ubar x_j2 M2^H,  M1bar += conj(Gram_j1(Ubar1, T)),
+//     Tbar += Ubar1 x_j1 M1^H;
+//     X = Ubar x_j1 M1^H,      M2bar += conj(Gram_j2(X, T))
+//       (Gram_j2(Ubar, T x_j1 M1) = Gram_j2(Ubar x_j1 M1^H, T): products on
+//        different axes commute, so no intermediate of the forward is kept);
+//     Tbar += U x_idx conj(Gbar)  (the bra side, U recomputed).
+// The same arithmetic as k_bp_node_vjp with every contraction a 16 x 16
+// complex MFMA tile; M bars accumulate in registers over both passes.
+__global__ void __launch_bounds__(64 * fast64::NW)
+    k_bp_node_vjp_d3c16(DevGraph g, const cd* T, const cd* e2n, const cd* obar, cd* ebar, cd* Tbar) {
+  using namespace fast64;
+  __shared__ cd tl[NPAD];
+  __shared__ cd wk[NPAD];
+  __shared__ cd Gb[3][256];
+  __shared__ cd diag[16];
+  __shared__ double red[2][8];
+  const int node = blockIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int base = g.adj_ptr[node];
+  const cd* tg = T + g.toff[node];
+  cd* tb = Tbar + g.toff[node];
+  const cd* Mg[3];
+  long long slot[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    slot[k] = 2LL * g.adj_edge[base + k] + g.adj_side[base + k];
+    Mg[k] = e2n + slot[k] * 256;
+  }
+  auto others = [](int idx, int& j1, int& j2) {
+    j1 = idx == 0 ? 1 : 0;
+    j2 = idx == 2 ? 1 : 2;
+  };
+  cd m1[4], m2[4], op[4];
+  // pass 1: the forward Grams
+  d4 gr[3], gi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) gr[k] = gi[k] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int s = 0; s < 2; ++s) {
+    stage_component(tl, tg, s);
+    __syncthreads();
+#pragma unroll
+    for (int idx = 0; idx < 3; ++idx) {
+      int j1, j2;
+      others(idx, j1, j2);
+      load_operand(Mg[j1], 0, lane, m1);
+      load_operand(Mg[j2], 0, lane, m2);
+      mode_product(wk, tl, m1, j1, wave, lane);
+      __syncthreads();
+      mode_product(wk, wk, m2, j2, wave, lane);
+      __syncthreads();
+      gram_partial(tl, wk, idx, wave, lane, gr[idx], gi[idx]);
+      __syncthreads();
+    }
+  }
+  // Gbar = Obar / conj(tr) - conj(<Obar, G> / tr^2) I per output slot
+  cd* part = wk;  // NW x 256
+#pragma unroll
+  for (int idx = 0; idx < 3; ++idx) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      part[wave * 256 + ((lane >> 4) + 4 * v) * 16 + (lane & 15)] = mk<double>(gr[idx][v], gi[idx][v]);
+    __syncthreads();
+    cd G = mk<double>(0.0, 0.0), ob = G;
+    double ipr = 0.0, ipi = 0.0;
+    if (tid < 256) {
+      for (int w = 0; w < NW; ++w) G = G + part[w * 256 + tid];
+      if (tid % 17 == 0) diag[tid / 17] = G;
+      ob = obar[slot[idx] * 256 + tid];
+      ipr = ob.re * G.re + ob.im * G.im;  // conj(Obar) G
+      ipi = ob.re * G.im - ob.im * G.re;
+    }
+    ipr = wave_sum(ipr);
+    ipi = wave_sum(ipi);
+    if (lane == 0) {
+      red[0][wave] = ipr;
+      red[1][wave] = ipi;
+    }
+    __syncthreads();
+    if (tid < 256) {
+      cd tr = mk<double>(0.0, 0.0), ip = tr;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) tr = tr + diag[k];
+      for (int w = 0; w < NW; ++w) ip = ip + mk<double>(red[0][w], red[1][w]);
+      cd v = cdiv(ob, conj(tr));
+      if (tid % 17 == 0) v = v - conj(cdiv(ip, tr * tr));
+      Gb[idx][tid] = v;
+    }
+    __syncthreads();
+  }
+  // pass 2: starts with component 1, still in LDS
+  d4 mr[3], mi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) mr[k] = mi[k] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int s = 1; s >= 0; --s) {
+    if (s == 0) {
+      stage_component(tl, tg, 0);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int idx = 0; idx < 3; ++idx) {
+      int j1, j2;
+      others(idx, j1, j2);
+      load_operand(Gb[idx], 2, lane, op);  // Gbar^T
+      load_operand(Mg[j1], 3, lane, m1);   // M1^H
+      load_operand(Mg[j2], 3, lane, m2);   // M2^H
+      // Ubar1 = (T x_idx Gbar^T) x_j2 M2^H
+      mode_product(wk, tl, op, idx, wave, lane);
+      __syncthreads();
+      mode_product(wk, wk, m2, j2, wave, lane);
+      __syncthreads();
+      gram_partial(wk, tl, j1, wave, lane, mr[j1], mi[j1]);
+      __syncthreads();
+      mode_product(wk, wk, m1, j1, wave, lane);
+      __syncthreads();
+      for (int f = tid; f < 4096; f += blockDim.x) {
+        const int q = (f >> 8) * B0P + ((f >> 4) & 15) * RP + (f & 15);
+        tb[2 * f + s] = tb[2 * f + s] + wk[q];
+      }
+      __syncthreads();
+      // X = (T x_idx Gbar^T) x_j1 M1^H
+      mode_product(wk, tl, op, idx, wave, lane);
+      __syncthreads();
+      mode_product(wk, wk, m1, j1, wave, lane);
+      __syncthreads();
+      gram_partial(wk, tl, j2, wave, lane, mr[j2], mi[j2]);
+      __syncthreads();
+      // bra side: U x_idx conj(Gbar)
+      load_operand(Mg[j1], 0, lane, m1);
+      load_operand(Mg[j2], 0, lane, m2);
+      load_operand(Gb[idx], 1, lane, op);
+      mode_product(wk, tl, m1, j1, wave, lane);
+      __syncthreads();
+      mode_product(wk, wk, m2, j2, wave, lane);
+      __syncthreads();
+      mode_product(wk, wk, op, idx, wave, lane);
+      __syncthreads();
+      for (int f = tid; f < 4096; f += blockDim.x) {
+        const int q = (f >> 8) * B0P + ((f >> 4) & 15) * RP + (f & 15);
+        tb[2 * f + s] = tb[2 * f + s] + wk[q];
+      }
+      __syncthreads();
+    }
+  }
+  // Mbar_j = conj(sum of the Gram partials), reduced over the waves
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      part[wave * 256 + ((lane >> 4) + 4 * v) * 16 + (lane & 15)] = mk<double>(mr[j][v], mi[j][v]);
+    __syncthreads();
+    if (tid < 256) {
+      cd sum = mk<double>(0.0, 0.0);
+      for (int w = 0; w < NW; ++w) sum = sum + part[w * 256 + tid];
+      ebar[slot[j] * 256 + tid] = conj(sum);
+    }
+    __syncthreads();
+  }
+}
+
 // Absorb adjoint: Gamma = Y / |Y|, Y = T x_i X_i^T on every bond
 // (tensor_ops.py:88-93).  Accumulates Tbar, writes Xbar per (node, slot).
 __global__ void __launch_bounds__(256)
@@ -1301,6 +1467,13 @@ long long edge_svd_vjp_ws_bytes(int chi) { return 14LL * chi * chi * 16 + 256; }
 long long rdm2_vjp_ws_bytes(int chi, int phys) { return 4LL * (chi * phys) * (chi * phys) * 16 + 256; }
 int vjp_blocks(long long items) { return vjp_grid(items); }
 
+hipError_t launch_bp_node_vjp_fast(const DevGraph& g, const void* T, const void* e2n, const void* obar, void* ebar,
+                                   void* Tbar, hipStream_t s) {
+  if (g.n_nodes <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bp_node_vjp_d3c16, dim3(g.n_nodes), dim3(64 * fast64::NW), 0, s, g, (const cd*)T,
+                     (const cd*)e2n, (const cd*)obar, (cd*)ebar, (cd*)Tbar);
+  return hipGetLastError();
+}
 hipError_t launch_bp_node_vjp(const DevGraph& g, const void* T, const void* e2n, const void* obar, void* ebar,
                               void* Tbar, void* ws, long long wsb, hipStream_t s) {
   if (g.n_nodes <= 0) return hipSuccess;

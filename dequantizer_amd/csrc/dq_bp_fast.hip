@@ -20,6 +20,7 @@
 // Per node: 6 mode products + 3 Grams (U_0 = (T x_2 M2) x_1 M1,
 // U_1 = (T x_2 M2) x_0 M0, U_2 = (T x_0 M0) x_1 M1, the second product of each
 // in place).
+#include "dq_fast64.hpp"
 #include "dq_launch.hpp"
 
 namespace dq {
@@ -202,59 +203,6 @@ __global__ void __launch_bounds__(64 * fast3::NW)
 // b0*273 + b1*17 + b2, bank-spread for 16-byte elements along every axis)
 // take 137 KiB of LDS.  Gram partials stay in registers across both passes
 // and are reduced over the waves at the end.
-namespace fast64 {
-constexpr int RP = 17;
-constexpr int B0P = 16 * RP + 1;
-constexpr int NPAD = 16 * B0P;
-constexpr int NW = 8;
-typedef double d4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ int axis_stride(int j) { return j == 0 ? B0P : (j == 1 ? RP : 1); }
-// column c (0..255) of the matrix view of mode j: the two other axes, row-major
-__device__ __forceinline__ int col_base(int j, int c) {
-  if (j == 0) return (c >> 4) * RP + (c & 15);
-  if (j == 1) return (c >> 4) * B0P + (c & 15);
-  return (c >> 4) * B0P + (c & 15) * RP;
-}
-__device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-// dst = src x_j M (may be in place); m[ks] = M[lane % 16][4 ks + lane / 16]
-__device__ __forceinline__ void mode_product(cd* dst, const cd* src, const cd (&m)[4], int j, int wave, int lane) {
-  const int st = axis_stride(j);
-  const int col = lane & 15, kq = lane >> 4;
-  for (int tile = wave; tile < 16; tile += NW) {
-    const int cb = col_base(j, tile * 16 + col);
-    d4 ar = {0.0, 0.0, 0.0, 0.0}, ai = ar;
-    cd x[4];
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) x[ks] = src[cb + (4 * ks + kq) * st];
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      ar = mfma(m[ks].re, x[ks].re, ar);
-      ar = mfma(-m[ks].im, x[ks].im, ar);
-      ai = mfma(m[ks].re, x[ks].im, ai);
-      ai = mfma(m[ks].im, x[ks].re, ai);
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) dst[cb + (kq + 4 * v) * st] = mk<double>(ar[v], ai[v]);
-  }
-}
-// this wave's share of sum_r conj(T[k][r]) U[l][r] along axis idx
-__device__ __forceinline__ void gram_partial(const cd* t, const cd* u, int idx, int wave, int lane, d4& gr, d4& gi) {
-  const int st = axis_stride(idx);
-  const int row = lane & 15, kq = lane >> 4;
-  constexpr int per = 64 / NW;
-  for (int ks = wave * per; ks < wave * per + per; ++ks) {
-    const int off = col_base(idx, 4 * ks + kq) + row * st;
-    const cd a = t[off], b = u[off];
-    gr = mfma(a.re, b.re, gr);
-    gr = mfma(a.im, b.im, gr);
-    gi = mfma(a.re, b.im, gi);
-    gi = mfma(-a.im, b.re, gi);
-  }
-}
-}  // namespace fast64
 
 __global__ void __launch_bounds__(64 * fast64::NW)
     k_bp_nodes_d3c16_f64(DevGraph g, const int* nodes, const cd* T, const cd* n2e_old, const cd* e2n_old,

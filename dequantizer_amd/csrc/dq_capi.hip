@@ -441,6 +441,10 @@ int dq_bp_sweep_vjp(dq_graph* g, int dtype, const void* T, const void* e2n_in, c
   int rc;
   if (!g || !T || !e2n_in || !n2e_out_bar || !e2n_in_bar || !T_bar) return fail(DQ_ERR_INVALID, "NULL argument");
   if ((rc = need_c128(dtype)) != DQ_OK) return rc;
+  if (g->dev.chi == 16 && g->dev.phys == 2 && g->min_deg == 3 && g->max_deg == 3) {
+    RET_LAUNCH(launch_bp_node_vjp_fast(g->dev, T, e2n_in, n2e_out_bar, e2n_in_bar, T_bar, S(stream)));
+    return DQ_OK;
+  }
   const long long wsb = (bp_vjp_ws_bytes(g->dev.chi, g->dev.phys, g->max_deg) + 255) / 256 * 256;
   if ((rc = ws_ensure(g, (size_t)wsb * vjp_blocks(g->dev.n_nodes))) != DQ_OK) return rc;
   RET_LAUNCH(launch_bp_node_vjp(g->dev, T, e2n_in, n2e_out_bar, e2n_in_bar, T_bar, g->ws, wsb, S(stream)));

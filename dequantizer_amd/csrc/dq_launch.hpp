@@ -201,6 +201,8 @@ long long su_vjp_ws_bytes_host(int chi, int phys, int max_r, int max_deg);
 
 // Adjoints of the other forward kernels (complex128, dq_adjoint.hip).
 long long bp_vjp_ws_bytes(int chi, int phys, int max_deg);
+hipError_t launch_bp_node_vjp_fast(const DevGraph& g, const void* T, const void* e2n, const void* obar, void* ebar,
+                                   void* Tbar, hipStream_t s);
 long long absorb_vjp_ws_bytes(int chi, int phys, int max_deg);
 long long edge_svd_vjp_ws_bytes(int chi);
 long long rdm2_vjp_ws_bytes(int chi, int phys);

@@ -202,9 +202,11 @@ def _fd_check(L_of, grads_dot, dirs, h=1e-6, tol=1e-5):
         assert abs(fd - an) < tol * max(1.0, abs(fd)), (fd, an)
 
 
-def test_bp_sweep_vjp_matches_central_differences():
-    topo, h = _graph_handle(8, 3, 1)
-    chi, E = 3, topo.n_edges
+@pytest.mark.parametrize("chi", [3, 16])
+def test_bp_sweep_vjp_matches_central_differences(chi):
+    """chi=16 on a 3-regular graph runs the matrix-core adjoint of the sweep."""
+    topo, h = _graph_handle(8, chi, 1)
+    E = topo.n_edges
     rng = np.random.default_rng(3)
     T0 = _cg(rng, h.tensor_elems)
     e2n0 = _psd_messages(rng, 2 * E, chi)
