@@ -41,10 +41,68 @@ __device__ __forceinline__ cd opget(const cd* A, int lda, int op, int i, int j) 
   }
 }
 
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// The same product on FP64 matrix cores (V_MFMA_F64_16X16X4F64: lane l holds
+// A[l % 16][l / 16], B[l / 16][l % 16]; D[l / 16 + 4 v][l % 16]) for m, n
+// multiples of 16 and k of 4: one 16 x 16 output tile per wave at a time, 4
+// real MFMAs per complex k-step.  With fewer tiles than waves the k range is
+// split over the waves and the partial tiles summed in a fixed order through
+// LDS (at most four waves take part in a split).
+__device__ void bmm_mfma(cd* Cm, int ldc, const cd* A, int lda, int opA, const cd* B, int ldb, int opB, int m, int n,
+                         int k, bool accumulate) {
+  __shared__ cd part[4][256];
+  const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tn = n >> 4, tiles = (m >> 4) * tn;
+  int split = 1;
+  while (tiles * split * 2 <= (nw < 4 ? nw : 4) && (k % (split * 2 * 4)) == 0) split *= 2;
+  const int items = tiles * split, kc = k / split;
+  const int r = lane & 15, kq = lane >> 4;
+  for (int it = wave; it < items; it += nw) {
+    const int tile = it / split, ks = it % split;
+    const int i0 = (tile / tn) * 16, j0 = (tile % tn) * 16;
+    d4 cr = {0.0, 0.0, 0.0, 0.0}, ci = cr;
+    for (int l0 = ks * kc; l0 < (ks + 1) * kc; l0 += 4) {
+      const cd a = opget(A, lda, opA, i0 + r, l0 + kq);
+      const cd b = opget(B, ldb, opB, l0 + kq, j0 + r);
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.re, cr, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, b.im, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.im, ci, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.re, ci, 0, 0, 0);
+    }
+    if (split == 1) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        cd* c = Cm + (long long)(i0 + kq + 4 * v) * ldc + j0 + r;
+        const cd val = mk<double>(cr[v], ci[v]);
+        *c = accumulate ? *c + val : val;
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) part[it][(kq + 4 * v) * 16 + r] = mk<double>(cr[v], ci[v]);
+    }
+  }
+  if (split > 1) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < tiles * 256; q += blockDim.x) {
+      const int tile = q >> 8, e = q & 255;
+      cd sum = part[tile * split][e];
+      for (int ks = 1; ks < split; ++ks) sum = sum + part[tile * split + ks][e];
+      cd* c = Cm + (long long)((tile / tn) * 16 + (e >> 4)) * ldc + (tile % tn) * 16 + (e & 15);
+      *c = accumulate ? *c + sum : sum;
+    }
+  }
+  __syncthreads();
+}
+
 // Cm (m x n, row-major, ldc) (+)= op(A) (m x k) op(B) (k x n).  Block-wide;
 // ends with a barrier.  Cm must not alias A or B.
 __device__ void bmm(cd* Cm, int ldc, const cd* A, int lda, int opA, const cd* B, int ldb, int opB, int m, int n, int k,
                     bool accumulate) {
+  if ((m & 15) == 0 && (n & 15) == 0 && (k & 3) == 0 && k > 0 && (blockDim.x & 63) == 0) {
+    bmm_mfma(Cm, ldc, A, lda, opA, B, ldb, opB, m, n, k, accumulate);
+    return;
+  }
   for (int f = threadIdx.x; f < m * n; f += blockDim.x) {
     const int i = f / n, j = f % n;
     cd acc = accumulate ? Cm[(long long)i * ldc + j] : mk<double>(0.0, 0.0);
@@ -644,15 +702,23 @@ __global__ void __launch_bounds__(256)
           fma_acc(acc2, conj(h[((long long)q * p + o) * p + s]), B[(long long)(r * p + o) * n + b * rg + q]);
       tbw[f] = acc2;
     }
+    // h-bar[q][o][s] = sum_{r, b} B[(r, o), (b, q)] conj(T[r, b, s]): the (r, b)
+    // pairs are spread over the block, one partial per (q, o, s) per thread,
+    // then block sums (q < rank; the rest are zero)
     cd* hb = hbar + (long long)blockIdx.x * 2 * hstride + side * hstride;
-    for (int f = threadIdx.x; f < hstride; f += blockDim.x) {
+    for (int f = 0; f < (int)hstride; ++f) {
       const int q = f / (p * p), o = (f / p) % p, s = f % p;
-      cd acc2 = mk<double>(0.0, 0.0);
+      double ar = 0.0, ai = 0.0;
       if (q < rg)
-        for (int r = 0; r < rows; ++r)
-          for (int b = 0; b < chi; ++b)
-            fma_acc(acc2, B[(long long)(r * p + o) * n + b * rg + q], conj(t[elem_off(str, d, slot, chi, r, b, s)]));
-      hb[f] = acc2;
+        for (int rb = threadIdx.x; rb < rows * chi; rb += blockDim.x) {
+          const int r = rb / chi, b = rb % chi;
+          const cd x = B[(long long)(r * p + o) * n + b * rg + q], y = t[elem_off(str, d, slot, chi, r, b, s)];
+          ar += x.re * y.re + x.im * y.im;
+          ai += x.im * y.re - x.re * y.im;
+        }
+      ar = block_sum(ar, red);
+      ai = block_sum(ai, red);
+      if (threadIdx.x == 0) hb[f] = mk<double>(ar, ai);
     }
     __syncthreads();
   }
