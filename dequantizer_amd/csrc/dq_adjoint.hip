@@ -976,6 +976,129 @@ __global__ void __launch_bounds__(64 * fast64::NW)
   }
 }
 
+// The absorb adjoint for degree-3, chi = 16, phys = 2 nodes on FP64 matrix
+// cores (one node per 512-thread block, one physical component of T in LDS
+// at a time).  Pass 1 recomputes Y = T x_0 X_0^T x_1 X_1^T x_2 X_2^T for |Y| and
+// Re<Y, Gamma-bar>; pass 2 forms Ybar = (Gamma-bar - Re<Y, Gamma-bar>/|Y|^2 Y)/|Y|
+// per component (kept in the workspace) and, with products on different axes
+// commuting, Xbar_i^T = conj(Gram_i(Ybar x_{j != i} conj(X_j), T)) and
+// Tbar += Ybar x_0 conj(X_0) x_1 conj(X_1) x_2 conj(X_2).
+__global__ void __launch_bounds__(64 * fast64::NW)
+    k_absorb_vjp_d3c16(DevGraph g, const cd* T, const cd* X, const cd* gbar, cd* Tbar, cd* Xbar, cd* ws) {
+  using namespace fast64;
+  __shared__ cd tl[NPAD];
+  __shared__ cd wk[NPAD];
+  __shared__ double red[NW];
+  const int node = blockIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int base = g.adj_ptr[node];
+  const cd* tg = T + g.toff[node];
+  const cd* gb = gbar + g.toff[node];
+  cd* tb = Tbar + g.toff[node];
+  cd* yb = ws + (long long)node * 8192;  // Ybar, (b0, b1, b2, s) row-major
+  long long slot[3];
+  cd mt[3][4], mc[3][4];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    slot[k] = 2LL * g.adj_edge[base + k] + g.adj_side[base + k];
+    load_operand(X + slot[k] * 256, 2, lane, mt[k]);  // X^T (forward)
+    load_operand(X + slot[k] * 256, 1, lane, mc[k]);  // conj(X) (adjoint)
+  }
+  auto padded = [](int f) { return (f >> 8) * B0P + ((f >> 4) & 15) * RP + (f & 15); };
+  // pass 1: |Y|^2 and Re<Y, Gamma-bar>
+  double nn = 0.0, ip = 0.0;
+  for (int s = 0; s < 2; ++s) {
+    stage_component(tl, tg, s);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      mode_product(tl, tl, mt[j], j, wave, lane);
+      __syncthreads();
+    }
+    for (int f = tid; f < 4096; f += blockDim.x) {
+      const cd y = tl[padded(f)], b = gb[2 * f + s];
+      nn += norm2(y);
+      ip += y.re * b.re + y.im * b.im;
+    }
+    __syncthreads();
+  }
+  nn = sqrt(block_sum(nn, red));
+  ip = block_sum(ip, red) / nn;
+  d4 gr[3], gi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) gr[k] = gi[k] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int s = 0; s < 2; ++s) {
+    stage_component(tl, tg, s);
+    __syncthreads();
+    mode_product(wk, tl, mt[0], 0, wave, lane);
+    __syncthreads();
+    mode_product(wk, wk, mt[1], 1, wave, lane);
+    __syncthreads();
+    mode_product(wk, wk, mt[2], 2, wave, lane);
+    __syncthreads();
+    for (int f = tid; f < 4096; f += blockDim.x) {
+      const int q = padded(f);
+      const cd y = wk[q], b = gb[2 * f + s];
+      const cd v = (1.0 / nn) * (b - (ip / nn) * y);
+      wk[q] = v;
+      yb[2 * f + s] = v;
+    }
+    __syncthreads();
+    // axes 0, 1 -> Gram over axis 2, then axis 2 -> Tbar
+    mode_product(wk, wk, mc[0], 0, wave, lane);
+    __syncthreads();
+    mode_product(wk, wk, mc[1], 1, wave, lane);
+    __syncthreads();
+    gram_partial(wk, tl, 2, wave, lane, gr[2], gi[2]);
+    __syncthreads();
+    mode_product(wk, wk, mc[2], 2, wave, lane);
+    __syncthreads();
+    for (int f = tid; f < 4096; f += blockDim.x) tb[2 * f + s] = tb[2 * f + s] + wk[padded(f)];
+    __syncthreads();
+    // axes 1, 2 -> Gram over axis 0
+    for (int f = tid; f < 4096; f += blockDim.x) wk[padded(f)] = yb[2 * f + s];
+    __syncthreads();
+    mode_product(wk, wk, mc[1], 1, wave, lane);
+    __syncthreads();
+    mode_product(wk, wk, mc[2], 2, wave, lane);
+    __syncthreads();
+    gram_partial(wk, tl, 0, wave, lane, gr[0], gi[0]);
+    __syncthreads();
+    // axes 0, 2 -> Gram over axis 1
+    for (int f = tid; f < 4096; f += blockDim.x) wk[padded(f)] = yb[2 * f + s];
+    __syncthreads();
+    mode_product(wk, wk, mc[0], 0, wave, lane);
+    __syncthreads();
+    mode_product(wk, wk, mc[2], 2, wave, lane);
+    __syncthreads();
+    gram_partial(wk, tl, 1, wave, lane, gr[1], gi[1]);
+    __syncthreads();
+  }
+  // Xbar[slot][b][a] = conj(G[a][b]), G reduced over the waves
+  cd* part = wk;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      part[wave * 256 + ((lane >> 4) + 4 * v) * 16 + (lane & 15)] = mk<double>(gr[j][v], gi[j][v]);
+    __syncthreads();
+    if (tid < 256) {
+      cd sum = mk<double>(0.0, 0.0);
+      for (int w = 0; w < NW; ++w) sum = sum + part[w * 256 + tid];
+      Xbar[slot[j] * 256 + (tid & 15) * 16 + (tid >> 4)] = conj(sum);
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_absorb_vjp_fast(const DevGraph& g, const void* T, const void* X, const void* gbar, void* Tbar,
+                                  void* Xbar, void* ws, hipStream_t s) {
+  if (g.n_nodes <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_absorb_vjp_d3c16, dim3(g.n_nodes), dim3(64 * fast64::NW), 0, s, g, (const cd*)T, (const cd*)X,
+                     (const cd*)gbar, (cd*)Tbar, (cd*)Xbar, (cd*)ws);
+  return hipGetLastError();
+}
+
 // Absorb adjoint: Gamma = Y / |Y|, Y = T x_i X_i^T on every bond
 // (tensor_ops.py:88-93).  Accumulates Tbar, writes Xbar per (node, slot).
 __global__ void __launch_bounds__(256)

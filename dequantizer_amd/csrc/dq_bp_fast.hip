@@ -281,6 +281,53 @@ hipError_t launch_bp_nodes_fast64(const DevGraph& g, const int* nodes, int count
   return hipGetLastError();
 }
 
+// Vidal absorb in complex128 on FP64 matrix cores for the same node shape:
+// per physical component, three in-place products with X_j^T in LDS, the
+// unnormalised result written to Gamma; after both components Gamma is
+// scaled by 1 / |Y| (re-read from global memory by the block that wrote it).
+__global__ void __launch_bounds__(64 * fast64::NW)
+    k_absorb_d3c16_f64(DevGraph g, const cd* T, const cd* X, cd* gam, const int* inst_active) {
+  using namespace fast64;
+  __shared__ cd tl[NPAD];
+  __shared__ double red[NW];
+  const int node = blockIdx.x;
+  if (inst_active && !inst_active[g.node_inst[node]]) return;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int base = g.adj_ptr[node];
+  const cd* tg = T + g.toff[node];
+  cd* out = gam + g.toff[node];
+  cd m[3][4];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    load_operand(X + (2LL * g.adj_edge[base + k] + g.adj_side[base + k]) * 256, 2, lane, m[k]);
+  double nn = 0.0;
+  for (int s = 0; s < 2; ++s) {
+    stage_component(tl, tg, s);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      mode_product(tl, tl, m[j], j, wave, lane);
+      __syncthreads();
+    }
+    for (int f = tid; f < 4096; f += blockDim.x) {
+      const cd v = tl[(f >> 8) * B0P + ((f >> 4) & 15) * RP + (f & 15)];
+      out[2 * f + s] = v;
+      nn += norm2(v);
+    }
+    __syncthreads();
+  }
+  const double inv = 1.0 / sqrt(block_sum(nn, red));
+  for (int f = tid; f < 8192; f += blockDim.x) out[f] = inv * out[f];
+}
+
+hipError_t launch_absorb_fast64(const DevGraph& g, const void* T, const cd* X, void* gam, const int* inst_active,
+                                hipStream_t s) {
+  if (g.n_nodes <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_absorb_d3c16_f64, dim3(g.n_nodes), dim3(64 * fast64::NW), 0, s, g, (const cd*)T, X, (cd*)gam,
+                     inst_active);
+  return hipGetLastError();
+}
+
 // Vidal absorb (tensor_ops.py:88-93) on matrix cores for the same shape:
 // Gamma = T x_0 X_0^T x_1 X_1^T x_2 X_2^T / |.|, X_j the orthogonaliser of the
 // node's side of bond j (complex128, rounded to complex64 operands, FP32

@@ -264,6 +264,11 @@ int dq_bp_set_schedule(dq_graph* g, int n_levels, const int* level_ptr, const in
   return DQ_OK;
 }
 
+// complex128 degree-3 chi=16 phys=2 graphs: the FP64 matrix-core kernels
+static bool fast64_applicable(int rb, const dq_graph* g) {
+  return rb == 8 && g->dev.chi == 16 && g->dev.phys == 2 && g->min_deg == 3 && g->max_deg == 3;
+}
+
 static long long node_ws_bytes(const dq_graph* g, int rb) {
   const long long S = ipow(g->dev.chi, g->max_deg) * g->dev.phys;
   return 2 * S * 2 * rb + (long long)g->dev.chi * g->dev.chi * (long long)sizeof(cd) + 256;
@@ -338,6 +343,8 @@ int dq_vidal_gauge(dq_graph* g, int dtype, const void* T, const void* n2e, doubl
                                  inst_active, S(stream)));
   else if (bp_fast_applicable(rb, g->dev, g->min_deg, g->max_deg))
     RET_LAUNCH(launch_absorb_fast(g->dev, T, g->X, gammas, inst_active, S(stream)));
+  else if (fast64_applicable(rb, g))
+    RET_LAUNCH(launch_absorb_fast64(g->dev, T, g->X, gammas, inst_active, S(stream)));
   else
     RET_LAUNCH(launch_absorb(rb, g->dev, T, g->X, gammas, g->ws, wsn, inst_active, S(stream)));
   return DQ_OK;
@@ -366,6 +373,8 @@ int dq_vidal_gauge_tape(dq_graph* g, int dtype, const void* T, const void* n2e, 
                                  g->ws, nullptr, S(stream)));
   else if (bp_fast_applicable(rb, g->dev, g->min_deg, g->max_deg))
     RET_LAUNCH(launch_absorb_fast(g->dev, T, (const cd*)X, gammas, nullptr, S(stream)));
+  else if (fast64_applicable(rb, g))
+    RET_LAUNCH(launch_absorb_fast64(g->dev, T, (const cd*)X, gammas, nullptr, S(stream)));
   else
     RET_LAUNCH(launch_absorb(rb, g->dev, T, (const cd*)X, gammas, g->ws, wsn, nullptr, S(stream)));
   return DQ_OK;
@@ -441,7 +450,7 @@ int dq_bp_sweep_vjp(dq_graph* g, int dtype, const void* T, const void* e2n_in, c
   int rc;
   if (!g || !T || !e2n_in || !n2e_out_bar || !e2n_in_bar || !T_bar) return fail(DQ_ERR_INVALID, "NULL argument");
   if ((rc = need_c128(dtype)) != DQ_OK) return rc;
-  if (g->dev.chi == 16 && g->dev.phys == 2 && g->min_deg == 3 && g->max_deg == 3) {
+  if (fast64_applicable(8, g)) {
     RET_LAUNCH(launch_bp_node_vjp_fast(g->dev, T, e2n_in, n2e_out_bar, e2n_in_bar, T_bar, S(stream)));
     return DQ_OK;
   }
@@ -461,9 +470,14 @@ int dq_vidal_gauge_vjp(dq_graph* g, int dtype, const void* T, const void* X, con
   if ((rc = need_c128(dtype)) != DQ_OK) return rc;
   const long long wa = (absorb_vjp_ws_bytes(g->dev.chi, g->dev.phys, g->max_deg) + 255) / 256 * 256;
   const long long we = (edge_svd_vjp_ws_bytes(g->dev.chi) + 255) / 256 * 256;
-  const size_t na = (size_t)wa * vjp_blocks(g->dev.n_nodes), ne = (size_t)we * vjp_blocks(g->dev.n_edges);
+  const bool fast = fast64_applicable(8, g);
+  const size_t na = fast ? (size_t)g->dev.n_nodes * 8192 * sizeof(cd) : (size_t)wa * vjp_blocks(g->dev.n_nodes);
+  const size_t ne = (size_t)we * vjp_blocks(g->dev.n_edges);
   if ((rc = ws_ensure(g, na > ne ? na : ne)) != DQ_OK) return rc;
-  RET_LAUNCH(launch_absorb_vjp(g->dev, T, X, gammas_bar, T_bar, X_bar, g->ws, wa, S(stream)));
+  if (fast)
+    RET_LAUNCH(launch_absorb_vjp_fast(g->dev, T, X, gammas_bar, T_bar, X_bar, g->ws, S(stream)));
+  else
+    RET_LAUNCH(launch_absorb_vjp(g->dev, T, X, gammas_bar, T_bar, X_bar, g->ws, wa, S(stream)));
   RET_LAUNCH(launch_edge_svd_vjp(g->dev, tape, eps, cut, X_bar, lam_bar, n2e_bar, g->ws, we, S(stream)));
   return DQ_OK;
 }

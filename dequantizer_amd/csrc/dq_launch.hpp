@@ -40,6 +40,8 @@ hipError_t launch_bp_nodes(int real_bytes, const DevGraph& g, const int* nodes, 
 // Matrix-core node update for graphs whose nodes all have degree 3, chi 16,
 // phys 2, complex64 (dq_bp_fast.hip).
 bool bp_fast_applicable(int real_bytes, const DevGraph& g, int min_deg, int max_deg);
+hipError_t launch_absorb_fast64(const DevGraph& g, const void* T, const cd* X, void* gammas, const int* inst_active,
+                                hipStream_t s);
 hipError_t launch_bp_nodes_fast64(const DevGraph& g, const int* nodes, int count, const void* T, const void* n2e_old,
                                   const void* e2n_old, void* n2e_new, const uint8_t* node_src, const int* inst_active,
                                   hipStream_t s);
@@ -201,6 +203,8 @@ long long su_vjp_ws_bytes_host(int chi, int phys, int max_r, int max_deg);
 
 // Adjoints of the other forward kernels (complex128, dq_adjoint.hip).
 long long bp_vjp_ws_bytes(int chi, int phys, int max_deg);
+hipError_t launch_absorb_vjp_fast(const DevGraph& g, const void* T, const void* X, const void* gbar, void* Tbar,
+                                  void* Xbar, void* ws, hipStream_t s);
 hipError_t launch_bp_node_vjp_fast(const DevGraph& g, const void* T, const void* e2n, const void* obar, void* ebar,
                                    void* Tbar, hipStream_t s);
 long long absorb_vjp_ws_bytes(int chi, int phys, int max_deg);

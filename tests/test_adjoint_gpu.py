@@ -236,12 +236,14 @@ def test_bp_sweep_vjp_matches_central_differences(chi):
     _fd_check(L_of, lambda ds: float(np.real(np.vdot(Tb, ds[0]) + np.vdot(eb, ds[1]))), dirs)
 
 
-def test_vidal_gauge_to_observables_vjp_matches_central_differences():
+@pytest.mark.parametrize("chi", [3, 16])
+def test_vidal_gauge_to_observables_vjp_matches_central_differences(chi):
     """messages -> Vidal gauge (edge SVD + absorb) -> symmetric gauge ->
     one- and two-site density matrices with diag(lambda) messages; the loss
-    sees only gauge-invariant quantities."""
-    topo, h = _graph_handle(8, 3, 2)
-    chi, E, N = 3, topo.n_edges, topo.n_nodes
+    sees only gauge-invariant quantities.  chi=16 on a 3-regular graph runs
+    the matrix-core absorb and its adjoint."""
+    topo, h = _graph_handle(8, chi, 2)
+    E, N = topo.n_edges, topo.n_nodes
     rng = np.random.default_rng(4)
     T0 = _cg(rng, h.tensor_elems)
     n2e0 = _psd_messages(rng, 2 * E, chi)
