@@ -50,6 +50,7 @@ SIGNATURES = {
     "dq_tensor_apply_half": (c_int, [c_int, vp, c_ll, c_int, c_ll, c_int, vp, c_int, vp, vp, vp]),
     "dq_su_tape_bytes": (c_ll, [vp, c_int]),
     "dq_simple_update_tape": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, c_int, c_dbl, c_int, vp, vp, vp, vp]),
+    "dq_bp_sweeps": (c_int, [vp, c_int, vp, vp, vp, vp, vp, c_int, vp, vp]),
     "dq_simple_update_vjp": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, c_int, c_dbl, c_dbl, vp, vp, vp, vp, vp]),
     "dq_edge_svd_tape_bytes": (c_ll, [vp]),
     "dq_vidal_gauge_tape": (c_int, [vp, c_int, vp, vp, c_dbl, vp, vp, vp, vp, vp]),

@@ -52,6 +52,14 @@ struct dq_graph {
   // every node once with all sources = incoming e2n (one synchronous level)
   int* d_all_nodes = nullptr;
   uint8_t* d_zero_src = nullptr;
+  // captured multi-sweep graphs (dq_bp_sweeps), keyed by every pointer and
+  // count they bake in; captured on a private stream, launched on the caller's
+  struct SweepGraph {
+    std::vector<const void*> key;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<SweepGraph> sweep_graphs;
+  hipStream_t capture_stream = nullptr;
   // owned scratch
   void* ws = nullptr;
   size_t ws_bytes = 0;
@@ -210,6 +218,9 @@ int dq_graph_create(dq_graph** out, int n_nodes, int n_edges, int chi, int phys,
 
 int dq_graph_destroy(dq_graph* g) {
   if (!g) return DQ_OK;
+  for (auto& sg : g->sweep_graphs)
+    if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+  if (g->capture_stream) (void)hipStreamDestroy(g->capture_stream);
   void* ptrs[] = {g->d_deg,       g->d_adj_ptr,       g->d_adj_edge,      g->d_adj_side,   g->d_toff,
                   g->d_edge_node, g->d_node_inst,     g->d_edge_inst,     g->d_inst_node_ptr,
                   g->d_inst_edge_ptr, g->d_level_nodes, g->d_node_src, g->d_edge_src, g->ws, g->X, g->node_dist,
@@ -274,6 +285,31 @@ static long long node_ws_bytes(const dq_graph* g, int rb) {
   return 2 * S * 2 * rb + (long long)g->dev.chi * g->dev.chi * (long long)sizeof(cd) + 256;
 }
 
+// Workspace for one sweep of this handle (grown before any launch, so that
+// launches can be captured).
+static int bp_sweep_ws(dq_graph* g, int rb) {
+  if (big_applicable(g->dev, g->max_deg))
+    return ws_ensure(g, (size_t)big_ws_bytes(g->dev, g->max_deg, g->max_level > 0 ? g->max_level : 1, rb));
+  const long long wsb = (node_ws_bytes(g, rb) + 255) / 256 * 256;
+  return ws_ensure(g, (size_t)wsb * (g->max_level > 0 ? g->max_level : 1));
+}
+
+static hipError_t bp_sweep_launch(dq_graph* g, int rb, const void* T, const void* n2e_in, const void* e2n_in,
+                                  void* n2e_out, void* e2n_out, const int* inst_active, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  const bool big = big_applicable(g->dev, g->max_deg);
+  const long long wsb = (node_ws_bytes(g, rb) + 255) / 256 * 256;
+  for (int l = 0; l < g->n_levels && e == hipSuccess; ++l) {
+    const int b = g->level_ptr[l], cnt = g->level_ptr[l + 1] - b;
+    e = big ? launch_bp_nodes_big(rb, g->dev, g->max_deg, g->d_level_nodes + b, cnt, T, n2e_in, e2n_in, n2e_out,
+                                  g->d_node_src, g->ws, inst_active, s)
+            : launch_bp_nodes(rb, g->dev, g->d_level_nodes + b, cnt, T, n2e_in, e2n_in, n2e_out, g->d_node_src, g->ws,
+                              wsb, inst_active, g->min_deg, g->max_deg, s);
+  }
+  if (e == hipSuccess) e = launch_bp_edges(rb, g->dev, n2e_out, n2e_in, e2n_out, g->d_edge_src, inst_active, s);
+  return e;
+}
+
 int dq_bp_sweep(dq_graph* g, int dtype, const void* T, const void* n2e_in, const void* e2n_in, void* n2e_out,
                 void* e2n_out, const int* inst_active, void* stream) {
   int rb, rc;
@@ -281,26 +317,64 @@ int dq_bp_sweep(dq_graph* g, int dtype, const void* T, const void* n2e_in, const
   if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
   if (n2e_out == n2e_in || e2n_out == e2n_in || n2e_out == e2n_in || e2n_out == n2e_in)
     return fail(DQ_ERR_INVALID, "bp sweep outputs must not alias inputs");
-  if (big_applicable(g->dev, g->max_deg)) {
-    if ((rc = ws_ensure(g, (size_t)big_ws_bytes(g->dev, g->max_deg, g->max_level > 0 ? g->max_level : 1, rb))) !=
-        DQ_OK)
-      return rc;
-    for (int l = 0; l < g->n_levels; ++l) {
-      const int b = g->level_ptr[l], cnt = g->level_ptr[l + 1] - b;
-      RET_LAUNCH(launch_bp_nodes_big(rb, g->dev, g->max_deg, g->d_level_nodes + b, cnt, T, n2e_in, e2n_in, n2e_out,
-                                     g->d_node_src, g->ws, inst_active, S(stream)));
-    }
-    RET_LAUNCH(launch_bp_edges(rb, g->dev, n2e_out, n2e_in, e2n_out, g->d_edge_src, inst_active, S(stream)));
+  if ((rc = bp_sweep_ws(g, rb)) != DQ_OK) return rc;
+  RET_LAUNCH(bp_sweep_launch(g, rb, T, n2e_in, e2n_in, n2e_out, e2n_out, inst_active, S(stream)));
+  return DQ_OK;
+}
+
+static hipError_t bp_sweeps_direct(dq_graph* g, int rb, const void* T, void* const* buf, int n_sweeps,
+                                   const int* inst_active, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < n_sweeps && e == hipSuccess; ++i) {
+    const int c = 2 * (i & 1), o = 2 - c;
+    e = bp_sweep_launch(g, rb, T, buf[c], buf[c + 1], buf[o], buf[o + 1], inst_active, s);
+  }
+  return e;
+}
+
+int dq_bp_sweeps(dq_graph* g, int dtype, const void* T, void* n2e_a, void* e2n_a, void* n2e_b, void* e2n_b,
+                 int n_sweeps, const int* inst_active, void* stream) {
+  int rb, rc;
+  if (!g || !T || !n2e_a || !e2n_a || !n2e_b || !e2n_b || n_sweeps < 0) return fail(DQ_ERR_INVALID, "bad argument");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  if (n2e_a == n2e_b || e2n_a == e2n_b || n2e_a == e2n_b || e2n_a == n2e_b || n2e_a == e2n_a || n2e_b == e2n_b)
+    return fail(DQ_ERR_INVALID, "the two message sets must be distinct buffers");
+  if (n_sweeps == 0) return DQ_OK;
+  if ((rc = bp_sweep_ws(g, rb)) != DQ_OK) return rc;
+  void* const buf[4] = {n2e_a, e2n_a, n2e_b, e2n_b};
+  hipStream_t s = S(stream);
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+    // the caller is capturing this stream: the launches join its graph
+    RET_LAUNCH(bp_sweeps_direct(g, rb, T, buf, n_sweeps, inst_active, s));
     return DQ_OK;
   }
-  const long long wsb = (node_ws_bytes(g, rb) + 255) / 256 * 256;
-  if ((rc = ws_ensure(g, (size_t)wsb * (g->max_level > 0 ? g->max_level : 1))) != DQ_OK) return rc;
-  for (int l = 0; l < g->n_levels; ++l) {
-    const int b = g->level_ptr[l], cnt = g->level_ptr[l + 1] - b;
-    RET_LAUNCH(launch_bp_nodes(rb, g->dev, g->d_level_nodes + b, cnt, T, n2e_in, e2n_in, n2e_out, g->d_node_src,
-                               g->ws, wsb, inst_active, g->min_deg, g->max_deg, S(stream)));
+  const std::vector<const void*> key = {T, n2e_a, e2n_a, n2e_b, e2n_b, inst_active, g->ws,
+                                        (const void*)(intptr_t)n_sweeps, (const void*)(intptr_t)rb};
+  hipGraphExec_t exec = nullptr;
+  for (auto& sg : g->sweep_graphs)
+    if (sg.key == key) exec = sg.exec;
+  if (!exec) {
+    if (!g->capture_stream) HIP_TRY(hipStreamCreateWithFlags(&g->capture_stream, hipStreamNonBlocking));
+    (void)device_cu_count();  // device queries before the capture, not inside it
+    HIP_TRY(hipStreamBeginCapture(g->capture_stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t le = bp_sweeps_direct(g, rb, T, buf, n_sweeps, inst_active, g->capture_stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(g->capture_stream, &graph);
+    if (le != hipSuccess || ce != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return fail(DQ_ERR_HIP, "sweep graph capture failed: %s", hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) return fail(DQ_ERR_HIP, "sweep graph instantiation failed: %s", hipGetErrorString(ie));
+    if (g->sweep_graphs.size() >= 4) {
+      (void)hipGraphExecDestroy(g->sweep_graphs.front().exec);
+      g->sweep_graphs.erase(g->sweep_graphs.begin());
+    }
+    g->sweep_graphs.push_back({key, exec});
   }
-  RET_LAUNCH(launch_bp_edges(rb, g->dev, n2e_out, n2e_in, e2n_out, g->d_edge_src, inst_active, S(stream)));
+  RET_LAUNCH(hipGraphLaunch(exec, s));
   return DQ_OK;
 }
 

@@ -81,6 +81,18 @@ int dq_bp_set_schedule(dq_graph* g, int n_levels, const int* level_ptr, const in
 int dq_bp_sweep(dq_graph* g, int dtype, const void* tensors, const void* n2e_in, const void* e2n_in, void* n2e_out,
                 void* e2n_out, const int* inst_active, void* stream);
 
+/* n_sweeps consecutive sweeps (dq_bp_sweep) alternating between message set
+ * A (n2e_a, e2n_a: the input) and B: sweep i reads A and writes B for even i,
+ * the reverse for odd i, so the result is in B for odd n_sweeps and in A for
+ * even.  The launches of the n sweeps are captured once into a HIP graph
+ * (keyed by every pointer and the count) and the graph is replayed on
+ * `stream`; when `stream` is itself being captured the sweeps are launched
+ * directly into the caller's capture.  No counterpart in the reference,
+ * whose BP loop runs one jitted sweep per host iteration
+ * (dequantizer/src/quantum_emulator/bp_quantum_emulator.py:253-266). */
+int dq_bp_sweeps(dq_graph* g, int dtype, const void* tensors, void* n2e_a, void* e2n_a, void* n2e_b, void* e2n_b,
+                 int n_sweeps, const int* inst_active, void* stream);
+
 /* Mean Frobenius distance between two message sets, per instance (device
  * double[n_inst]).  Replaces messages_frob_distance
  * (dequantizer/src/mappings/messages_distance.py:14-22). */
