@@ -167,6 +167,7 @@ def main():
             "sweeps_per_regauge": (args.fixed_sweeps or "to convergence"),
         },
         "su_gates_per_s": n_gates / elapsed,
+        "autotune": [ln for ln in emu.tuning_report.strip().split("\n") if ln],
         "bp_sweeps_timed": int(sweeps),
         "bp_eu_per_s_in_sweeps": (2 * ef.edges_number / sweep_s) if durs else None,
         "setup_and_warmup_s": t_build,

@@ -51,6 +51,8 @@ SIGNATURES = {
     "dq_su_tape_bytes": (c_ll, [vp, c_int]),
     "dq_simple_update_tape": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, c_int, c_dbl, c_int, vp, vp, vp, vp]),
     "dq_bp_sweeps": (c_int, [vp, c_int, vp, vp, vp, vp, vp, c_int, vp, vp]),
+    "dq_autotune": (c_int, [vp, c_int, c_int, c_int, vp, ctypes.c_char_p, c_int]),
+    "dq_tune_override": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int]),
     "dq_simple_update_vjp": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, c_int, c_dbl, c_dbl, vp, vp, vp, vp, vp]),
     "dq_edge_svd_tape_bytes": (c_ll, [vp]),
     "dq_vidal_gauge_tape": (c_int, [vp, c_int, vp, vp, c_dbl, vp, vp, vp, vp, vp]),
@@ -94,6 +96,15 @@ def lib() -> ctypes.CDLL:
                     raise RuntimeError("dequantizer_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
                 _lib = load_library()
     return _lib
+
+
+def autotune(handle_ptr, dtype_code: int, max_gate_rank: int = 2, force: bool = False) -> str:
+    """Run the library's autotuner for a graph handle (kernel variants timed
+    on synthetic inputs of the handle's shape, fastest kept process-wide per
+    shape); returns its timing report."""
+    buf = ctypes.create_string_buffer(4096)
+    check(lib().dq_autotune(handle_ptr, dtype_code, max_gate_rank, int(force), stream_handle(), buf, len(buf)))
+    return buf.value.decode()
 
 
 def check(status: int) -> None:

@@ -32,7 +32,6 @@ constexpr int RP = 33;              // pitch of a (b0, b1) fiber of 32 complex v
 constexpr int B0P = 16 * RP + 1;    // pitch of b0
 constexpr int NPAD = 16 * B0P;      // padded complex elements of one tensor
 constexpr int S = 16 * 16 * 16 * 2;  // unpadded complex elements
-constexpr int NW = 8;                // waves per BP block
 
 __device__ __forceinline__ int axis_stride(int j) { return j == 0 ? B0P : (j == 1 ? RP : 2); }
 
@@ -72,12 +71,13 @@ __device__ __forceinline__ void mode_product(float2* dst, const float2* src, con
   }
 }
 
-// Partial Gram of this wave: sum over its 512/NW columns of conj(T[k]) U[l].
+// Partial Gram of this wave: sum over its 512/W columns of conj(T[k]) U[l].
+template <int W>
 __device__ __forceinline__ void gram_partial(const float2* t, const float2* u, int idx, int wave, int lane,
                                              floatx4& gr, floatx4& gi) {
   const int st = axis_stride(idx);
   const int row = lane & 15, kq = lane >> 4;
-  constexpr int per = 128 / NW;
+  constexpr int per = 128 / W;
   gr = floatx4{0.f, 0.f, 0.f, 0.f};
   gi = floatx4{0.f, 0.f, 0.f, 0.f};
   for (int ks = wave * per; ks < wave * per + per; ++ks) {
@@ -92,7 +92,8 @@ __device__ __forceinline__ void gram_partial(const float2* t, const float2* u, i
 }
 }  // namespace fast3
 
-__global__ void __launch_bounds__(64 * fast3::NW)
+template <int NW>
+__global__ void __launch_bounds__(64 * NW)
     k_bp_nodes_d3c16(DevGraph g, const int* nodes, int count, const float2* T, const float2* n2e_old,
                      const float2* e2n_old, float2* n2e_new, const uint8_t* node_src, const int* inst_active) {
   using namespace fast3;
@@ -160,7 +161,7 @@ __global__ void __launch_bounds__(64 * fast3::NW)
       mode_product<NW>(wk, wk, m[j2], j2, wave, lane);
       __syncthreads();
       floatx4 gr, gi;
-      gram_partial(tl, wk, idx, wave, lane, gr, gi);
+      gram_partial<NW>(tl, wk, idx, wave, lane, gr, gi);
 #pragma unroll
       for (int v = 0; v < 4; ++v) part[wave][(4 * (lane >> 4) + v) * 16 + (lane & 15)] = make_float2(gr[v], gi[v]);
       __syncthreads();
@@ -397,9 +398,18 @@ hipError_t launch_bp_nodes_fast(const DevGraph& g, const int* nodes, int count, 
                                 const void* e2n_old, void* n2e_new, const uint8_t* node_src, const int* inst_active,
                                 hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  const int grid = count < device_cu_count() ? count : device_cu_count();
-  hipLaunchKernelGGL(k_bp_nodes_d3c16, dim3(grid), dim3(64 * fast3::NW), 0, s, g, nodes, count, (const float2*)T,
-                     (const float2*)n2e_old, (const float2*)e2n_old, (float2*)n2e_new, node_src, inst_active);
+  // variant (autotuner): bit 0 = one block per node instead of persistent
+  // blocks, bit 1 = 4 waves per block instead of 8
+  int v = tune_get(kTuneBpD3C16, 0);
+  if (v < 0) v = 0;
+  const int cus = device_cu_count();
+  const int grid = (v & 1) ? count : (count < cus ? count : cus);
+  if (v & 2)
+    hipLaunchKernelGGL(k_bp_nodes_d3c16<4>, dim3(grid), dim3(256), 0, s, g, nodes, count, (const float2*)T,
+                       (const float2*)n2e_old, (const float2*)e2n_old, (float2*)n2e_new, node_src, inst_active);
+  else
+    hipLaunchKernelGGL(k_bp_nodes_d3c16<8>, dim3(grid), dim3(512), 0, s, g, nodes, count, (const float2*)T,
+                       (const float2*)n2e_old, (const float2*)e2n_old, (float2*)n2e_new, node_src, inst_active);
   return hipGetLastError();
 }
 

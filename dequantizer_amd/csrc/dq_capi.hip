@@ -4,7 +4,11 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/dq.h"
@@ -722,3 +726,214 @@ int dq_repad(const dq_graph* old, dq_graph* g, int dtype, const void* T_old, voi
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------- autotuner
+namespace dq {
+static std::mutex g_tune_mutex;
+static std::map<std::pair<int, long long>, int> g_tune;
+
+int tune_get(int kernel, long long key) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  const auto it = g_tune.find({kernel, key});
+  return it == g_tune.end() ? -1 : it->second;
+}
+void tune_set(int kernel, long long key, int variant) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  if (variant < 0)
+    g_tune.erase({kernel, key});
+  else
+    g_tune[{kernel, key}] = variant;
+}
+
+// Deterministic values in [lo, lo + 1) for timing inputs.
+__global__ void k_tune_fill(double* x, long long n, double lo, unsigned seed) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    unsigned h = (unsigned)i * 2654435761u ^ seed;
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    x[i] = lo + (h & 0xffffff) / 16777216.0;
+  }
+}
+__global__ void k_tune_fill_f(float* x, long long n, float lo, unsigned seed) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    unsigned h = (unsigned)i * 2654435761u ^ seed;
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    x[i] = lo + (h & 0xffffff) / 16777216.0f;
+  }
+}
+}  // namespace dq
+
+namespace {
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int fill_real(void* p, long long count, int rb, double lo, unsigned seed, hipStream_t s) {
+  if (rb == 8)
+    hipLaunchKernelGGL(dq::k_tune_fill, dim3(dq::grid_for(count, 256)), dim3(256), 0, s, (double*)p, count, lo, seed);
+  else
+    hipLaunchKernelGGL(dq::k_tune_fill_f, dim3(dq::grid_for(count, 256)), dim3(256), 0, s, (float*)p, count,
+                       (float)lo, seed);
+  return hipGetLastError() == hipSuccess ? DQ_OK : fail(DQ_ERR_HIP, "autotune fill failed");
+}
+
+// Median of `reps` timed runs of `run` (after one untimed run).
+template <typename F>
+int time_median(hipStream_t s, int reps, F run, float* ms) {
+  hipEvent_t a, b;
+  HIP_TRY(hipEventCreate(&a));
+  HIP_TRY(hipEventCreate(&b));
+  std::vector<float> t;
+  int rc = DQ_OK;
+  for (int i = 0; i <= reps && rc == DQ_OK; ++i) {
+    (void)hipEventRecord(a, s);
+    if (run() != hipSuccess) rc = fail(DQ_ERR_HIP, "autotune launch failed");
+    (void)hipEventRecord(b, s);
+    (void)hipEventSynchronize(b);
+    float e = 0.f;
+    (void)hipEventElapsedTime(&e, a, b);
+    if (i > 0) t.push_back(e);
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (rc != DQ_OK) return rc;
+  std::sort(t.begin(), t.end());
+  *ms = t[t.size() / 2];
+  return DQ_OK;
+}
+}  // namespace
+
+int dq_tune_override(int kernel, int dtype, int chi, int phys, int max_gate_rank, int variant) {
+  int rb, rc;
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  if (kernel == 0) {
+    if (variant < -1 || variant > 3) return fail(DQ_ERR_INVALID, "node-update variant %d outside [-1, 3]", variant);
+    tune_set(kTuneBpD3C16, 0, variant);
+  } else if (kernel == 1) {
+    if (variant != -1 && variant != 1 && variant != 2)
+      return fail(DQ_ERR_INVALID, "simple-update mode %d is not -1, 1 (all-LDS) or 2 (compact)", variant);
+    tune_set(kTuneSu, su_tune_key(rb, chi, phys, max_gate_rank), variant);
+  } else {
+    return fail(DQ_ERR_INVALID, "unknown kernel %d", kernel);
+  }
+  return DQ_OK;
+}
+
+int dq_autotune(dq_graph* g, int dtype, int max_gate_rank, int force, void* stream, char* report, int report_len) {
+  int rb, rc;
+  if (!g) return fail(DQ_ERR_INVALID, "NULL graph");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  hipStream_t s = S(stream);
+  std::string rep;
+  char line[256];
+  const int chi = g->dev.chi, p = g->dev.phys;
+  const long long cc = (long long)chi * chi, E = g->dev.n_edges, nm = 2 * E * cc;
+  // BP node kernel variants (complex64, degree 3, chi 16)
+  if (bp_fast_applicable(rb, g->dev, g->min_deg, g->max_deg) && (force || tune_get(kTuneBpD3C16, 0) < 0)) {
+    DevBuf T, m[4];
+    HIP_TRY(hipMalloc(&T.p, (size_t)g->tensor_elems * 2 * rb));
+    for (auto& b : m) HIP_TRY(hipMalloc(&b.p, (size_t)nm * 2 * rb));
+    if ((rc = fill_real(T.p, 2 * g->tensor_elems, rb, -0.5, 1u, s)) != DQ_OK) return rc;
+    for (int i = 0; i < 4; ++i)
+      if ((rc = fill_real(m[i].p, 2 * nm, rb, 0.0, 2u + i, s)) != DQ_OK) return rc;
+    if ((rc = bp_sweep_ws(g, rb)) != DQ_OK) return rc;
+    const char* names[4] = {"8 waves, persistent", "8 waves, block per node", "4 waves, persistent",
+                            "4 waves, block per node"};
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int v = 0; v < 4; ++v) {
+      tune_set(kTuneBpD3C16, 0, v);
+      float ms = 0.f;
+      if ((rc = time_median(s, 5, [&] { return bp_sweep_launch(g, rb, T.p, m[0].p, m[1].p, m[2].p, m[3].p, nullptr, s); },
+                            &ms)) != DQ_OK) {
+        tune_set(kTuneBpD3C16, 0, -1);
+        return rc;
+      }
+      snprintf(line, sizeof line, "bp node update d3 chi16 c64: %s %.1f us\n", names[v], 1e3 * ms);
+      rep += line;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = v;
+      }
+    }
+    tune_set(kTuneBpD3C16, 0, best);
+    snprintf(line, sizeof line, "bp node update d3 chi16 c64: chose %s\n", names[best]);
+    rep += line;
+  }
+  // simple-update storage mode where the compact and all-LDS layouts both apply
+  const long long skey = su_tune_key(rb, chi, p, max_gate_rank);
+  if (max_gate_rank >= 1 && su_mem_mode(chi, p, max_gate_rank) == 2 && (force || tune_get(kTuneSu, skey) < 0) &&
+      E > 0 && g->max_deg <= 4) {
+    // a greedy matching of the edges: gates that share no node, one launch
+    std::vector<int> en(2 * E);
+    HIP_TRY(hipMemcpy(en.data(), g->d_edge_node, en.size() * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<char> used(g->dev.n_nodes, 0);
+    std::vector<SuGate> gates;
+    for (int e = 0; e < E; ++e)
+      if (!used[en[2 * e]] && !used[en[2 * e + 1]]) {
+        used[en[2 * e]] = used[en[2 * e + 1]] = 1;
+        gates.push_back({e, 0, max_gate_rank, 0});
+      }
+    const int cnt = (int)gates.size();
+    const long long hst = (long long)p * p * p * p;
+    DevBuf T0, T, lam0, lam, dg, hv, err, ent;
+    HIP_TRY(hipMalloc(&T0.p, (size_t)g->tensor_elems * 2 * rb));
+    HIP_TRY(hipMalloc(&T.p, (size_t)g->tensor_elems * 2 * rb));
+    HIP_TRY(hipMalloc(&lam0.p, (size_t)E * chi * sizeof(double)));
+    HIP_TRY(hipMalloc(&lam.p, (size_t)E * chi * sizeof(double)));
+    HIP_TRY(hipMalloc(&dg.p, (size_t)cnt * sizeof(SuGate)));
+    HIP_TRY(hipMalloc(&hv.p, (size_t)cnt * 2 * hst * sizeof(cd)));
+    HIP_TRY(hipMalloc(&err.p, (size_t)cnt * sizeof(double)));
+    HIP_TRY(hipMalloc(&ent.p, (size_t)cnt * sizeof(double)));
+    HIP_TRY(hipMemcpy(dg.p, gates.data(), cnt * sizeof(SuGate), hipMemcpyHostToDevice));
+    if ((rc = fill_real(T0.p, 2 * g->tensor_elems, rb, -0.5, 7u, s)) != DQ_OK) return rc;
+    if ((rc = fill_real(lam0.p, E * chi, 8, 0.1, 8u, s)) != DQ_OK) return rc;
+    if ((rc = fill_real(hv.p, (long long)cnt * 2 * hst * 2, 8, -0.5, 9u, s)) != DQ_OK) return rc;
+    const long long wsb = su_ws_bytes(chi, p, max_gate_rank);
+    if ((rc = ws_ensure(g, (size_t)wsb * cnt)) != DQ_OK) return rc;
+    const int modes[2] = {2, 1};
+    const char* names[2] = {"compact (two blocks per CU)", "all-LDS (one block per CU)"};
+    int best = 2;
+    float best_ms = 1e30f;
+    for (int i = 0; i < 2; ++i) {
+      tune_set(kTuneSu, skey, modes[i]);
+      float ms = 0.f;
+      auto run = [&] {
+        hipError_t e = hipMemcpyAsync(T.p, T0.p, (size_t)g->tensor_elems * 2 * rb, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(lam.p, lam0.p, (size_t)E * chi * sizeof(double), hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess)
+          e = launch_su(rb, g->dev, T.p, (double*)lam.p, (const SuGate*)dg.p, cnt, (const cd*)hv.p, max_gate_rank,
+                        1e-10, chi, (double*)err.p, (double*)ent.p, g->ws, wsb, nullptr, s);
+        return e;
+      };
+      if ((rc = time_median(s, 3, run, &ms)) != DQ_OK) {
+        tune_set(kTuneSu, skey, -1);
+        return rc;
+      }
+      snprintf(line, sizeof line, "simple update chi%d p%d r%d %s: %s %.1f us (%d gates, incl. input reset)\n", chi, p,
+               max_gate_rank, rb == 4 ? "c64" : "c128", names[i], 1e3 * ms, cnt);
+      rep += line;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = modes[i];
+      }
+    }
+    tune_set(kTuneSu, skey, best);
+    snprintf(line, sizeof line, "simple update chi%d p%d r%d %s: chose %s\n", chi, p, max_gate_rank,
+             rb == 4 ? "c64" : "c128", names[best == 2 ? 0 : 1]);
+    rep += line;
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  if (report && report_len > 0) {
+    strncpy(report, rep.c_str(), (size_t)report_len - 1);
+    report[report_len - 1] = 0;
+  }
+  return DQ_OK;
+}

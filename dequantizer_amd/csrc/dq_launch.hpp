@@ -23,6 +23,12 @@ inline int device_cu_count() {
   return cache[dev];
 }
 
+// Kernel variants picked by the autotuner, process-wide per (kernel, shape
+// key); -1 = not tuned (the launcher's default).
+enum TuneKernel { kTuneBpD3C16 = 0, kTuneSu = 1 };
+int tune_get(int kernel, long long key);
+void tune_set(int kernel, long long key, int variant);
+
 inline int grid_for(long long total, int block) {
   long long gsz = (total + block - 1) / block;
   if (gsz > 65536) gsz = 65536;
@@ -167,6 +173,8 @@ hipError_t launch_su(int real_bytes, const DevGraph& g, void* T, double* lam, co
 // FP64 matrices fit the LDS variant.
 long long su_ws_bytes(int chi, int phys, int max_r);
 int su_mem_mode(int chi, int phys, int max_r);
+int su_launch_mode(int real_bytes, int chi, int phys, int max_r);
+long long su_tune_key(int real_bytes, int chi, int phys, int max_r);
 
 // One-site density matrices with messages e2n: rho[i] (phys x phys).
 hipError_t launch_rdm1(int real_bytes, const DevGraph& g, const void* T, const void* e2n, const int* nodes, int count,

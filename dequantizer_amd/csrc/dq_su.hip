@@ -690,7 +690,7 @@ hipError_t launch_su(int rb, const DevGraph& g, void* T, double* lam, const SuGa
                      const cd* halves, int max_gate_rank, double eps, int max_rank, double* err, double* entropy,
                      void* ws, long long wsb, void* tape, hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  const int mode = su_mem_mode(g.chi, g.phys, max_gate_rank);
+  const int mode = su_launch_mode(rb, g.chi, g.phys, max_gate_rank);
   const long long tb = su_tape_bytes(g.chi, max_gate_rank);
   char* tp = (char*)tape;
 #define DQ_SU_LAUNCH(RT, MODE)                                                                                   \
@@ -715,8 +715,29 @@ hipError_t launch_su(int rb, const DevGraph& g, void* T, double* lam, const SuGa
   return hipGetLastError();
 }
 
+long long su_tune_key(int rb, int chi, int phys, int max_r) {
+  return ((long long)rb << 24) | ((long long)chi << 12) | ((long long)phys << 6) | max_r;
+}
+
+// The storage mode of a launch: the autotuner's choice when it made one for
+// this shape (compact and all-LDS are both valid for the compact shapes),
+// else the default.
+int su_launch_mode(int rb, int chi, int phys, int max_r) {
+  const int def = su_mem_mode(chi, phys, max_r);
+  const int v = tune_get(kTuneSu, su_tune_key(rb, chi, phys, max_r));
+  if (def == kSuCompact && (v == kSuCompact || v == kSuLds)) return v;
+  return def;
+}
+
 long long su_ws_bytes(int chi, int phys, int max_r) {
-  return (su_ws_cd(su_mem_mode(chi, phys, max_r), chi, phys, max_r) * (long long)sizeof(cd) + 255) / 256 * 256;
+  // sized for the larger of the modes a launch of this shape may take
+  const int def = su_mem_mode(chi, phys, max_r);
+  long long cdn = su_ws_cd(def, chi, phys, max_r);
+  if (def == kSuCompact) {
+    const long long alt = su_ws_cd(kSuLds, chi, phys, max_r);
+    if (alt > cdn) cdn = alt;
+  }
+  return (cdn * (long long)sizeof(cd) + 255) / 256 * 256;
 }
 
 }  // namespace dq

@@ -93,6 +93,24 @@ int dq_bp_sweep(dq_graph* g, int dtype, const void* tensors, const void* n2e_in,
 int dq_bp_sweeps(dq_graph* g, int dtype, const void* tensors, void* n2e_a, void* e2n_a, void* n2e_b, void* e2n_b,
                  int n_sweeps, const int* inst_active, void* stream);
 
+/* Autotuner: times the kernel variants that apply to this handle's shape on
+ * synthetic inputs of that shape (median of several runs, HIP events) and
+ * keeps the fastest for every later launch of the same shape in this
+ * process: the complex64 degree-3 chi=16 node update (4 or 8 waves per
+ * block, persistent blocks or one block per node) and the simple-update
+ * storage mode (compact LDS at two blocks per CU, or all-LDS at one) for
+ * gates up to max_gate_rank.  Shapes already tuned are skipped unless
+ * `force`.  A text report of the timings goes to `report` (may be NULL).
+ * No counterpart in the reference. */
+int dq_autotune(dq_graph* g, int dtype, int max_gate_rank, int force, void* stream, char* report, int report_len);
+
+/* Sets (or with variant -1 clears) a tuned choice by hand, for tests and
+ * experiments: kernel 0 = the complex64 degree-3 chi=16 node update (variant
+ * bit 0: one block per node, bit 1: 4 waves per block); kernel 1 = the
+ * simple-update storage mode for (dtype, chi, phys, max_gate_rank)
+ * (1 = all-LDS, 2 = compact). */
+int dq_tune_override(int kernel, int dtype, int chi, int phys, int max_gate_rank, int variant);
+
 /* Mean Frobenius distance between two message sets, per instance (device
  * double[n_inst]).  Replaces messages_frob_distance
  * (dequantizer/src/mappings/messages_distance.py:14-22). */
