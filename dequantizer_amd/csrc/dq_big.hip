@@ -1,7 +1,7 @@
-// Node updates and the Vidal absorb for large node tensors: S = chi^d p of a
-// million elements and more (degree 6 at chi = 16 is 33.5 M elements, 256 MiB
-// in complex64), where one workgroup per node leaves the chip idle and the
-// tensors are far beyond LDS.  Same arithmetic as the generic kernels
+// Node updates and the Vidal absorb for large node tensors: S = chi^d p of
+// 2^15 elements and more (degree 3 at chi = 32: 64 Ki elements; degree 6 at
+// chi = 16: 33.5 M elements, 256 MiB in complex64), where one workgroup per
+// node leaves the chip idle and the tensors are beyond LDS.  Same arithmetic as the generic kernels
 // (pass_through_node, dequantizer/src/tensor_ops.py:36-60; absorb,
 // tensor_ops.py:88-93), split into HBM-streaming launches over all nodes of a
 // level at once:
@@ -209,7 +209,7 @@ static int big_gram_chunks(long long cols) {
 }
 
 bool big_applicable(const DevGraph& g, int max_deg) {
-  return (g.chi == 16 || g.chi == 32) && ipow(g.chi, max_deg) * g.phys >= (1LL << 20);
+  return (g.chi == 16 || g.chi == 32) && ipow(g.chi, max_deg) * g.phys >= (1LL << 15);
 }
 
 long long big_ws_bytes(const DevGraph& g, int max_deg, int count, int rb) {

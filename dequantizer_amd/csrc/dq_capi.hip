@@ -469,9 +469,15 @@ int dq_vidal_distance(dq_graph* g, int dtype, const void* gammas, const double* 
                                           g->d_inst_edge_ptr, g->n_inst, out, g->ws, inst_active, S(stream)));
     return DQ_OK;
   }
+  if (g->dev.chi == 32 && big_applicable(g->dev, g->max_deg)) {
+    if ((rc = ws_ensure(g, (size_t)vidal_big_ws_bytes(g->dev, g->max_deg, rb))) != DQ_OK) return rc;
+    RET_LAUNCH(launch_vidal_distance_big(rb, g->dev, g->max_deg, gammas, lam, eps, g->d_all_nodes, g->d_zero_src,
+                                         g->d_inst_edge_ptr, g->n_inst, out, g->ws, inst_active, S(stream)));
+    return DQ_OK;
+  }
   const long long cc = (long long)g->dev.chi * g->dev.chi;
   const long long wsv =
-      ((3 * cc * (long long)sizeof(cd) + ((long long)g->max_deg * g->dev.chi + 4 * g->dev.chi + 16) * 8 + 16) + 255) /
+      ((5 * cc * (long long)sizeof(cd) + ((long long)g->max_deg * g->dev.chi + 4 * g->dev.chi + 16) * 8 + 16) + 255) /
       256 * 256;
   if ((rc = ws_ensure(g, (size_t)wsv * (g->dev.n_nodes > 0 ? g->dev.n_nodes : 1))) != DQ_OK) return rc;
   RET_LAUNCH(launch_vidal_distance(rb, g->dev, gammas, lam, eps, g->node_dist, g->d_inst_node_ptr,

@@ -310,7 +310,7 @@ __device__ __forceinline__ bool complex_gt(cd z, double eps) { return z.re == ep
 
 __host__ __device__ inline long long vidal_ws_bytes(int chi, int max_deg) {
   const long long cc = (long long)chi * chi;
-  return 3 * cc * (long long)sizeof(cd) + ((long long)max_deg * chi + 4 * chi + 16) * sizeof(double) + 16;
+  return 5 * cc * (long long)sizeof(cd) + ((long long)max_deg * chi + 4 * chi + 16) * sizeof(double) + 16;
 }
 
 template <typename R>
@@ -325,7 +325,9 @@ __global__ void __launch_bounds__(256)
   const int base = g.adj_ptr[node];
   cd* Rm = (cd*)(ws + (long long)blockIdx.x * wsb);
   cd* A = Rm + cc;
-  double* w2 = (double*)(A + cc);
+  cd* Us = A + cc;  // singular vectors of the fast nuclear norms (chi 16, 32)
+  cd* Vs = Us + cc;
+  double* w2 = (double*)(Vs + cc);
   double* rot = w2 + (long long)d * chi;
   int* flag = (int*)(rot + 4 * chi + 8);
   for (int t = threadIdx.x; t < d * chi; t += blockDim.x) {
@@ -352,8 +354,24 @@ __global__ void __launch_bounds__(256)
       A[q] = h;
     }
     __syncthreads();
-    jacobi_eigh(A, nullptr, chi, rot, flag);
-    for (int k = 0; k < chi; ++k) dist += fabs(A[k * chi + k].re);
+    // sum |eigenvalues| of the Hermitian A = its nuclear norm: one-sided
+    // Jacobi singular values at chi 16 (one wave) and 32 (the block), the
+    // two-sided eigensolver otherwise
+    if (chi == 32 && blockDim.x == 256) {
+      __shared__ double sg[32];
+      __shared__ int pm[32];
+      block_jacobi_svd_n<32>(A, Us, Vs, sg, pm, rot);
+      for (int k = 0; k < 32; ++k) dist += sg[k];
+    } else if (chi == 16) {
+      __shared__ double sg16[16];
+      __shared__ int pm16[16];
+      if (threadIdx.x < 64) wave_jacobi_svd_n<16>(A, Us, Vs, sg16, pm16, rot);
+      __syncthreads();
+      for (int k = 0; k < 16; ++k) dist += sg16[k];
+    } else {
+      jacobi_eigh(A, nullptr, chi, rot, flag);
+      for (int k = 0; k < chi; ++k) dist += fabs(A[k * chi + k].re);
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) node_dist[node] = dist;
@@ -511,6 +529,59 @@ __global__ void __launch_bounds__(256)
   if (lane == 0) es_dist[es] = nuc;
 }
 
+// diag(lambda^2) messages at any chi and precision.
+template <typename R>
+__global__ void __launch_bounds__(256) k_lam2_diag_any(DevGraph g, const double* lam, cx<R>* e2n) {
+  const int chi = g.chi;
+  const long long cc = (long long)chi * chi, total = 2LL * g.n_edges * cc;
+  for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < total;
+       f += (long long)gridDim.x * blockDim.x) {
+    const long long e = f / (2 * cc), q = f % cc;
+    const int k = (int)(q / chi), l = (int)(q % chi);
+    const double v = lam[e * chi + k];
+    e2n[f] = mk<R>(k == l ? (R)(v * v) : (R)0, (R)0);
+  }
+}
+
+// The nuclear-norm term of one (edge, side) at chi = 32, one block each:
+// R Hermitised, the ideal subtracted, singular values by the block-level
+// one-sided Jacobi (sum of |eigenvalues| of a Hermitian matrix).
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_vidal_nuclear32(DevGraph g, const cx<R>* Rm, double eps, double* es_dist, const int* inst_active) {
+  __shared__ cd G[1024], U[1024], V[1024];
+  __shared__ double sg[32], scal[64];
+  __shared__ int perm[32];
+  const long long es = blockIdx.x;
+  const int e = (int)(es >> 1);
+  if (inst_active && !inst_active[g.node_inst[g.edge_node[2 * e]]]) return;
+  const cx<R>* r = Rm + es * 1024;
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x) G[i] = cvt<double>(r[i]);
+  __syncthreads();
+  cd tr = mk<double>(0.0, 0.0);
+  for (int k = 0; k < 32; ++k) tr = tr + G[k * 33];
+  int cnt = 0;
+  for (int k = 0; k < 32; ++k) cnt += complex_gt(cdiv(G[k * 33], tr), eps) ? 1 : 0;
+  cd h[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int i = threadIdx.x + 256 * t, k = i >> 5, l = i & 31;
+    const cd a = cdiv(G[i], tr), at = cdiv(G[l * 32 + k], tr);
+    h[t] = mk<double>(0.5 * (a.re + at.re), 0.5 * (a.im - at.im));
+    if (k == l && complex_gt(a, eps)) h[t].re -= 1.0 / cnt;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < 4; ++t) G[threadIdx.x + 256 * t] = h[t];
+  __syncthreads();
+  block_jacobi_svd_n<32>(G, U, V, sg, perm, scal);
+  if (threadIdx.x == 0) {
+    double nuc = 0.0;
+    for (int k = 0; k < 32; ++k) nuc += sg[k];
+    es_dist[es] = nuc;
+  }
+}
+
 // out[i] = sum over the (edge, side) terms of instance i / (2 |E_i|).
 __global__ void __launch_bounds__(256)
     k_vidal_reduce_es(const double* es_dist, const int* inst_edge_ptr, double* out, const int* inst_active) {
@@ -525,6 +596,45 @@ __global__ void __launch_bounds__(256)
 }
 
 long long vidal_fast_ws_bytes(const DevGraph& g) { return 2LL * g.n_edges * (2 * 256 * 8 + 8) + 256; }
+
+// Vidal distance for chi = 32 on the streaming node update: the lambda^2-
+// weighted bond Grams are node updates from diag(lambda^2) messages.
+long long vidal_big_ws_bytes(const DevGraph& g, int max_deg, int rb) {
+  const long long cc = (long long)g.chi * g.chi;
+  return big_ws_bytes(g, max_deg, g.n_nodes, rb) + 2 * (2LL * g.n_edges * cc * 2 * rb) + 2LL * g.n_edges * 8 + 512;
+}
+
+hipError_t launch_vidal_distance_big(int rb, const DevGraph& g, int max_deg, const void* gam, const double* lam,
+                                     double eps, const int* all_nodes, const uint8_t* zero_src,
+                                     const int* inst_edge_ptr, int n_inst, double* inst_dist, void* ws,
+                                     const int* inst_active, hipStream_t s) {
+  if (g.n_edges <= 0 || g.chi != 32) return hipErrorInvalidValue;
+  const long long cc = (long long)g.chi * g.chi, nm = 2LL * g.n_edges * cc;
+  char* base = (char*)ws + (big_ws_bytes(g, max_deg, g.n_nodes, rb) + 255) / 256 * 256;
+  void* msg = base;
+  void* R = base + nm * 2 * rb;
+  double* es_dist = (double*)((char*)R + nm * 2 * rb);
+  if (rb == 4)
+    hipLaunchKernelGGL(k_lam2_diag_any<float>, dim3(grid_for(nm, 256)), dim3(256), 0, s, g, lam, (cx<float>*)msg);
+  else
+    hipLaunchKernelGGL(k_lam2_diag_any<double>, dim3(grid_for(nm, 256)), dim3(256), 0, s, g, lam, (cx<double>*)msg);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  err = launch_bp_nodes_big(rb, g, max_deg, all_nodes, g.n_nodes, gam, msg, msg, R, zero_src, ws, inst_active, s);
+  if (err != hipSuccess) return err;
+  if (rb == 4)
+    hipLaunchKernelGGL(k_vidal_nuclear32<float>, dim3((unsigned)(2LL * g.n_edges)), dim3(256), 0, s, g,
+                       (const cx<float>*)R, eps, es_dist, inst_active);
+  else
+    hipLaunchKernelGGL(k_vidal_nuclear32<double>, dim3((unsigned)(2LL * g.n_edges)), dim3(256), 0, s, g,
+                       (const cx<double>*)R, eps, es_dist, inst_active);
+  err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  if (n_inst > 0)
+    hipLaunchKernelGGL(k_vidal_reduce_es, dim3(n_inst), dim3(256), 0, s, es_dist, inst_edge_ptr, inst_dist,
+                       inst_active);
+  return hipGetLastError();
+}
 
 hipError_t launch_vidal_distance_fast(const DevGraph& g, const void* gam, const double* lam, double eps,
                                       const int* all_nodes, const uint8_t* zero_src, const int* inst_edge_ptr,

@@ -43,6 +43,7 @@ CASES = {
     "cube2_chi2": lambda: lattice_pair([2, 2, 2], 2, 2, open_boundary=False),
     "rr3_12_chi4": lambda: regular_pair(12, 4, 3),
     "rr3_10_chi16": lambda: regular_pair(10, 16, 3),
+    "rr3_8_chi32": lambda: regular_pair(8, 32, 5),
 }
 
 
