@@ -65,6 +65,9 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=64,
                     help="instances of the batched annealing leg (the whole 200-step run of the random 3-regular "
                          "instance set, N in 100..1000, in one engine); 0 skips it")
+    ap.add_argument("--eagle-steps", type=int, default=20,
+                    help="Trotter steps of the kicked-Ising leg on the 127-qubit heavy-hex lattice (chi 32, "
+                         "complex64, nine RX angles in one batched engine); 0 skips it")
     ap.add_argument("--lattice-size", type=int, default=4,
                     help="L of the periodic L^3 lattice leg (degree 6, chi 16, phys 2, Gaussian tensors): BP to "
                          "1e-8 and the Vidal gauge in complex64 and complex128; 0 skips it")
@@ -192,6 +195,9 @@ def main():
     if args.batch_instances:
         result["batched"] = batched_run(args)
         print(f"bench: batched run done: {result['batched']['seconds']:.1f} s", file=sys.stderr, flush=True)
+    if args.eagle_steps:
+        result["eagle"] = eagle_run(args)
+        print(f"bench: eagle leg done: {result['eagle']['seconds']:.1f} s", file=sys.stderr, flush=True)
     if args.lattice_size:
         result["lattice"] = lattice_run(args)
         print(f"bench: lattice leg done: {result['lattice']['sweep_ms_c64']:.0f} ms per complex64 sweep",
@@ -250,6 +256,26 @@ def batched_run(args):
             "energy_per_spin_bp": [round(float(x), 6) for x in e_bp],
             "energy_per_spin_sign": [round(float(x), 6) for x in e_sign],
             "energy_per_spin_bp_mean": float(e_bp.mean())}
+
+
+def eagle_run(args, chi=32):
+    """Kicked-Ising dynamics on the 127-qubit heavy-hex lattice: RX(theta) on
+    every qubit, then RZZ(-pi/2) in three edge-colour layers, each followed by
+    a regauge (synchronous BP to Vidal distance 1e-5, at most 100 sweeps);
+    chi = 32, complex64; theta in linspace(0, pi/2, 9) as nine instances of
+    one batched engine.  <Z> of qubit 62 after the last step, per angle."""
+    import torch
+    from dequantizer_amd.drivers import run_eagle_simulations
+    thetas = np.linspace(0.0, np.pi / 2, 9)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    zs = run_eagle_simulations(thetas, chi, args.eagle_steps, 1e-5, is_infinite=False, synchronous_update=True,
+                               dtype=torch.complex64, chi=chi)
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t0
+    return {"qubits": 127, "chi": chi, "steps": args.eagle_steps, "angles": 9, "seconds": seconds,
+            "ms_per_step_all_angles": 1e3 * seconds / args.eagle_steps,
+            "z62_final": [round(float(z[-1]), 6) for z in zs], "thetas": [round(float(t), 6) for t in thetas]}
 
 
 def lattice_run(args, chi=16, tol=1e-8, max_sweeps=60):

@@ -4,7 +4,8 @@
   square-root network (examples/3d_ising_canonicalization.py:89-160): Potts
   node tensors per inverse temperature, BP until a convergence criterion,
   Vidal canonicalisation and the mean magnetisation.
-* ``run_eagle_simulation``: kicked-Ising dynamics on the heavy-hex Eagle
+* ``run_eagle_simulation`` / ``run_eagle_simulations`` (several RX angles
+  as instances of one batched engine): kicked-Ising dynamics on the heavy-hex Eagle
   lattice (examples/ibm_eagle_processor.py:56-145): RX(theta) on every qubit,
   then RZZ(-pi/2) on every edge, <Z> of the target qubit after each layer.
   ``colour_layers=True`` applies the RZZ gates by edge-colour class with a
@@ -193,6 +194,47 @@ def run_eagle_simulation(max_chi: int, layers_per_regauging: int, trotter_steps:
         dens = emu.dens_q1(target)
         dynamics.append(float((dens[0, 0] - dens[1, 1]).real))
     return dynamics
+
+
+def run_eagle_simulations(thetas, max_chi: int, trotter_steps: int, accuracy: float, is_infinite: bool = False,
+                          synchronous_update: bool = True, max_belief_propagation_iterations: Optional[int] = 100,
+                          key=PRNGKey(42), traversal_type: str = "dfs", *, dtype=None,
+                          chi: Optional[int] = None) -> List[List[float]]:
+    """The kicked-Ising dynamics of ``run_eagle_simulation`` (three colour
+    layers, a regauge after each) for several RX angles at once: one
+    instance per angle in one batched engine, every launch covering all
+    instances.  Returns <Z> of the target qubit after 0..trotter_steps
+    layers, per angle."""
+    import torch
+    from .emulator import BatchedBPEmulator
+    lattice = get_heavy_hex_ibm_eagle_lattice_infinite() if is_infinite else get_heavy_hex_ibm_eagle_lattice()
+    target = 2 if is_infinite else 62
+    thetas = [float(t) for t in thetas]
+    n = len(thetas)
+    emu = BatchedBPEmulator([lattice] * n, [key] * n, max_chi, 10 ** 9, accuracy, max_belief_propagation_iterations,
+                            synchronous_update, traversal_type, dtype=dtype or torch.complex64, chi=chi)
+    traverser = list(lattice.get_traversal_iterator() or iter([]))
+    edges = [el.id for el in traverser if isinstance(el, Edge)]
+    colours = edge_colouring(lattice)
+    classes = [[e for e in edges if colours[e] == c] for c in range(max(colours.values()) + 1)]
+    insts = list(range(n))
+
+    def z_target():
+        return [float((lambda d: (d[0, 0] - d[1, 1]).real)(emu.rdm1(i, [target])[0])) for i in insts]
+
+    out = [z_target()]
+    mixing = [rx(t) for t in thetas]
+    for _ in range(trotter_steps):
+        for i in insts:
+            for node_id in range(lattice.nodes_number):
+                emu.queue_q1(i, mixing[i], node_id)
+        for cls in classes:
+            for i in insts:
+                for a, b in cls:
+                    emu.apply_q2(i, RZZ, a, b)
+            emu._regauge(insts)
+        out.append(z_target())
+    return [[out[k][i] for k in range(trotter_steps + 1)] for i in insts]
 
 
 def annealing_instances(n_instances: int = 64, base_seed: int = 1000) -> List[EnergyFunction]:

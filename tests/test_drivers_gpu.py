@@ -139,3 +139,17 @@ def test_annealing_instance_set_batched_step():
     for r, ef in zip(res, efs):
         assert r.density_matrices.shape == (ef.nodes_number, 2, 2)
         assert np.isfinite(r.bp_energy)
+
+
+def test_eagle_batched_angles_match_single_runs():
+    """Several RX angles as instances of one engine give the single-angle
+    dynamics (synchronous BP, chi=16, complex128)."""
+    import torch
+    thetas = [0.3, 0.8]
+    zs = drivers.run_eagle_simulations(thetas, 16, 3, 1e-5, is_infinite=False, dtype=torch.complex128,
+                                       synchronous_update=True)
+    for th, z in zip(thetas, zs):
+        z1 = drivers.run_eagle_simulation(16, 1, 3, 1e-5, th, is_infinite=False, synchronous_update=True,
+                                          max_belief_propagation_iterations=100, dtype=torch.complex128,
+                                          colour_layers=True)
+        assert np.abs(np.asarray(z) - np.asarray(z1)).max() < 1e-9, (th, z, z1)
