@@ -12,6 +12,7 @@ Prints one JSON line.  ``--gpus`` above 1 is refused: this build measures one
 GPU.
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -79,8 +80,8 @@ def main():
     args = ap.parse_args()
     args.sync = args.bp_schedule == "sync"
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: every box in this setup has one GPU; "
-              "multi-GPU sharding is described in DESIGN.md, not run", file=sys.stderr)
+        print("bench.py measures one GPU only (--gpus 1); runs over several GPUs are not part of this "
+              "benchmark", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -351,6 +352,8 @@ def adjoint_timing(ef, args):
     import torch
     from dequantizer_amd import PRNGKey
     from dequantizer_amd.adjoint import AnnealingGradient
+    gc.collect()
+    torch.cuda.empty_cache()  # blocks cached by the earlier legs
     t0 = time.perf_counter()
     run = AnnealingGradient(ef, PRNGKey(args.seed), chi=args.chi, n_steps=args.adjoint_steps, dt=args.dt,
                             n_knots=args.knots, fixed_sweeps=args.fixed_sweeps or 10)

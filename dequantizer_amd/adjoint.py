@@ -264,7 +264,9 @@ class AnnealingGradient:
         # forward with one checkpoint per step, and the records that fit
         checkpoints = []
         records = [None] * self.n_steps
-        budget = record_fraction * torch.cuda.mem_get_info()[0]
+        # free device memory plus what torch's allocator holds cached but unused
+        free = torch.cuda.mem_get_info()[0] + torch.cuda.memory_reserved() - torch.cuda.memory_allocated()
+        budget = record_fraction * free
         used, per_step = 0, None
         gam, lam = self.gam0.clone(), self.lam0.clone()
         for k in range(self.n_steps):
