@@ -17,9 +17,12 @@
 //                 complex product = 4 real MFMAs;
 //   Gram          (16 x 512) x (512 x 16), K split over the 8 waves and
 //                 reduced through LDS.
-// Per node: 6 mode products + 3 Grams (U_0 = (T x_2 M2) x_1 M1,
-// U_1 = (T x_2 M2) x_0 M0, U_2 = (T x_0 M0) x_1 M1, the second product of each
-// in place).
+// Per node: 5 mode products + 3 Grams.  X = T x_2 M2 is stored once and
+// serves U_0 = X x_1 M1 and U_1 = X x_0 M0; then X = T x_0 M0 serves
+// U_2 = X x_1 M1.  The second product of each U is fused with its Gram: the
+// 16 x 16 product tile leaves the matrix cores in the layout the Gram's
+// B operand needs (lane l holds row 4(l/16)+v, column l%16), so U never goes
+// through LDS.
 #include "dq_fast64.hpp"
 #include "dq_launch.hpp"
 
@@ -71,6 +74,46 @@ __device__ __forceinline__ void mode_product(float2* dst, const float2* src, con
   }
 }
 
+// U = X x_j M fused with the partial Gram of this wave over axis idx
+// (accumulated into gr, gi): G[k][l] += sum conj(T[k@idx, ...]) U[l@idx, ...].
+// Tiles enumerate (b_tax, s) for the third bond axis tax; within a tile lane
+// l%16 runs along axis idx.  The product's result D[4(l/16)+v][l%16] =
+// U[b_j = 4(l/16)+v][b_idx = l%16] is, for each v, the B operand of a Gram
+// k-step over b_j in {v, 4+v, 8+v, 12+v}; the A operand is conj(T) at the
+// same (b_j, b_tax, s) with b_idx = l%16.
+template <int W>
+__device__ __forceinline__ void mode_gram(const float2* t, const float2* x, const float2 (&m)[4], int j, int idx,
+                                          int tax, int wave, int lane, floatx4& gr, floatx4& gi) {
+  const int sj = axis_stride(j);
+  const int y = lane & 15, kq = lane >> 4;
+  const int lane_base = y * axis_stride(idx);
+#pragma unroll
+  for (int tile = wave; tile < 32; tile += W) {
+    const int cb = lane_base + (tile >> 1) * axis_stride(tax) + (tile & 1);
+    float2 xv[4], tv[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) xv[ks] = x[cb + (4 * ks + kq) * sj];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) tv[v] = t[cb + (4 * kq + v) * sj];
+    floatx4 ur = {0.f, 0.f, 0.f, 0.f}, ui = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      ur = mfma(m[ks].x, xv[ks].x, ur);
+      ur = mfma(-m[ks].y, xv[ks].y, ur);
+      ui = mfma(m[ks].x, xv[ks].y, ui);
+      ui = mfma(m[ks].y, xv[ks].x, ui);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      // conj(a) * u
+      gr = mfma(tv[v].x, ur[v], gr);
+      gr = mfma(tv[v].y, ui[v], gr);
+      gi = mfma(tv[v].x, ui[v], gi);
+      gi = mfma(-tv[v].y, ur[v], gi);
+    }
+  }
+}
+
 // Partial Gram of this wave: sum over its 512/W columns of conj(T[k]) U[l].
 template <int W>
 __device__ __forceinline__ void gram_partial(const float2* t, const float2* u, int idx, int wave, int lane,
@@ -101,8 +144,7 @@ __global__ void __launch_bounds__(64 * NW)
   constexpr int PER = S / NT;   // tensor elements staged per thread
   __shared__ float2 tl[NPAD];
   __shared__ float2 wk[NPAD];
-  __shared__ float2 part[NW][256];
-  __shared__ float2 diag[16];
+  __shared__ float2 diag[3 * 16];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   // Persistent blocks (one per CU: the two tensors take most of the LDS):
   // block b updates nodes[b], nodes[b + gridDim.x], ...  The next node's
@@ -151,45 +193,64 @@ __global__ void __launch_bounds__(64 * NW)
       for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * NT];
     }
     __syncthreads();
-    // (first product, second product, Gram axis) for idx = 0, 1, 2
-    const int plan[3][3] = {{2, 1, 0}, {2, 0, 1}, {0, 1, 2}};
+    floatx4 gr[3], gi[3];
 #pragma unroll
-    for (int step = 0; step < 3; ++step) {
-      const int j1 = plan[step][0], j2 = plan[step][1], idx = plan[step][2];
-      mode_product<NW>(wk, tl, m[j1], j1, wave, lane);
-      __syncthreads();
-      mode_product<NW>(wk, wk, m[j2], j2, wave, lane);
-      __syncthreads();
-      floatx4 gr, gi;
-      gram_partial<NW>(tl, wk, idx, wave, lane, gr, gi);
+    for (int k = 0; k < 3; ++k) gr[k] = gi[k] = floatx4{0.f, 0.f, 0.f, 0.f};
+    mode_product<NW>(wk, tl, m[2], 2, wave, lane);  // X = T x_2 M2
+    __syncthreads();
+    mode_gram<NW>(tl, wk, m[1], 1, 0, 2, wave, lane, gr[0], gi[0]);  // U_0 = X x_1 M1
+    mode_gram<NW>(tl, wk, m[0], 0, 1, 2, wave, lane, gr[1], gi[1]);  // U_1 = X x_0 M0
+    __syncthreads();
+    mode_product<NW>(wk, tl, m[0], 0, wave, lane);  // X = T x_0 M0
+    __syncthreads();
+    mode_gram<NW>(tl, wk, m[1], 1, 2, 0, wave, lane, gr[2], gi[2]);  // U_2 = X x_1 M1
+    __syncthreads();
+    // wave partials of the three Grams into the work tensor, summed over the
+    // waves, divided by their traces
+    float2* part = wk;  // [3][NW][256]
 #pragma unroll
-      for (int v = 0; v < 4; ++v) part[wave][(4 * (lane >> 4) + v) * 16 + (lane & 15)] = make_float2(gr[v], gi[v]);
-      __syncthreads();
-      float2 sum = make_float2(0.f, 0.f);
-      if (tid < 256) {
-        sum = part[0][tid];
-        for (int w = 1; w < NW; ++w) {
-          sum.x += part[w][tid].x;
-          sum.y += part[w][tid].y;
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        part[(k * NW + wave) * 256 + (4 * (lane >> 4) + v) * 16 + (lane & 15)] = make_float2(gr[k][v], gi[k][v]);
+    __syncthreads();
+    constexpr int QN = (768 + NT - 1) / NT;
+    float2 sum[QN];
+#pragma unroll
+    for (int qi = 0; qi < QN; ++qi) {
+      const int q = tid + qi * NT;
+      sum[qi] = make_float2(0.f, 0.f);
+      if (q < 768) {
+        const int k = q >> 8, e = q & 255;
+        for (int w = 0; w < NW; ++w) {
+          sum[qi].x += part[(k * NW + w) * 256 + e].x;
+          sum[qi].y += part[(k * NW + w) * 256 + e].y;
         }
-        if (tid % 17 == 0) diag[tid / 17] = sum;
-      }
-      __syncthreads();
-      if (tid < 256) {
-        float tr_r = 0.f, tr_i = 0.f;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-          tr_r += diag[k].x;
-          tr_i += diag[k].y;
-        }
-        const float den = tr_r * tr_r + tr_i * tr_i;
-        float2 o;
-        o.x = (sum.x * tr_r + sum.y * tr_i) / den;
-        o.y = (sum.y * tr_r - sum.x * tr_i) / den;
-        n2e_new[(2LL * oe[idx] + os[idx]) * 256 + tid] = o;
+        if (e % 17 == 0) diag[k * 16 + e / 17] = sum[qi];
       }
     }
     __syncthreads();
+#pragma unroll
+    for (int qi = 0; qi < QN; ++qi) {
+      const int q = tid + qi * NT;
+      if (q < 768) {
+        const int k = q >> 8, e = q & 255;
+        float tr_r = 0.f, tr_i = 0.f;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+          tr_r += diag[k * 16 + c].x;
+          tr_i += diag[k * 16 + c].y;
+        }
+        const float den = tr_r * tr_r + tr_i * tr_i;
+        float2 o;
+        o.x = (sum[qi].x * tr_r + sum[qi].y * tr_i) / den;
+        o.y = (sum[qi].y * tr_r - sum[qi].x * tr_i) / den;
+        n2e_new[(2LL * oe[k] + os[k]) * 256 + e] = o;
+      }
+    }
+    // the next node's staging writes only tl, last read before the barrier
+    // that precedes the partial writes; its first write of wk follows the
+    // staging barrier, which every thread reaches after its reads of part
     if (next >= count) break;
     it = next;
   }
