@@ -62,7 +62,27 @@ __device__ void bmm_mfma(cd* Cm, int ldc, const cd* A, int lda, int opA, const c
     const int tile = it / split, ks = it % split;
     const int i0 = (tile / tn) * 16, j0 = (tile % tn) * 16;
     d4 cr = {0.0, 0.0, 0.0, 0.0}, ci = cr;
-    for (int l0 = ks * kc; l0 < (ks + 1) * kc; l0 += 4) {
+    const int le = (ks + 1) * kc;
+    int l0 = ks * kc;
+    // four k-steps per group with all their operands loaded first, so that
+    // the operand latency (the operands often live in global memory) is paid
+    // once per group; the accumulation order is the plain k order
+    for (; l0 + 16 <= le; l0 += 16) {
+      cd a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = opget(A, lda, opA, i0 + r, l0 + 4 * u + kq);
+        b[u] = opget(B, ldb, opB, l0 + 4 * u + kq, j0 + r);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].re, b[u].re, cr, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[u].im, b[u].im, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].re, b[u].im, ci, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].im, b[u].re, ci, 0, 0, 0);
+      }
+    }
+    for (; l0 < le; l0 += 4) {
       const cd a = opget(A, lda, opA, i0 + r, l0 + kq);
       const cd b = opget(B, ldb, opB, l0 + kq, j0 + r);
       cr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.re, cr, 0, 0, 0);
@@ -382,7 +402,7 @@ __device__ __forceinline__ int row_bond(int d, int slot, int chi, int row, int j
   return 0;
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 2)
     k_su_vjp(DevGraph g, const cd* Tpre, const double* lam, const SuGate* gates, const cd* halves, int max_r,
              double eps, double cut, const char* tape, long long tape_bytes, cd* Tbar, double* lambar, cd* hbar,
              char* ws, long long wsb) {
@@ -483,34 +503,77 @@ __global__ void __launch_bounds__(256)
     const cd* K = side == 0 ? Ka : Kb;
     cd* Kbar = side == 0 ? Kba : Kbb;
     const double* w = wl[side];
-    for (int f = threadIdx.x; f < m * n; f += blockDim.x) {
-      const int R = f / n, c = f % n;
-      const int r = R / p, o = R % p, b = c / rg, q = c % rg;
-      cd acc = mk<double>(0.0, 0.0);
-      for (int s = 0; s < p; ++s) fma_acc(acc, h[((long long)q * p + o) * p + s], t[elem_off(str, d, slot, chi, r, b, s)]);
-      A[f] = acc;
+    // the gathers below are batched kU elements per thread, loads first, so
+    // that the global-memory latency is paid once per batch
+    constexpr int kU = 8;
+    if (p == 2) {
+      constexpr int kA = 4;
+      for (int f0 = threadIdx.x; f0 < m * n; f0 += kA * blockDim.x) {
+        cd tv[kA][2], hv[kA][2];
+#pragma unroll
+        for (int u = 0; u < kA; ++u) {
+          const int f = f0 + u * blockDim.x;
+          if (f < m * n) {
+            const int R = f / n, c = f - (f / n) * n;
+            const int r = R >> 1, o = R & 1, b = c / rg, q = c - (c / rg) * rg;
+            const long long to = elem_off(str, d, slot, chi, r, b, 0);
+            tv[u][0] = t[to];
+            tv[u][1] = t[to + 1];
+            hv[u][0] = h[(q * 2 + o) * 2];
+            hv[u][1] = h[(q * 2 + o) * 2 + 1];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kA; ++u) {
+          const int f = f0 + u * blockDim.x;
+          if (f < m * n) A[f] = hv[u][0] * tv[u][0] + hv[u][1] * tv[u][1];
+        }
+      }
+    } else {
+      for (int f = threadIdx.x; f < m * n; f += blockDim.x) {
+        const int R = f / n, c = f % n;
+        const int r = R / p, o = R % p, b = c / rg, q = c % rg;
+        cd acc = mk<double>(0.0, 0.0);
+        for (int s = 0; s < p; ++s) fma_acc(acc, h[((long long)q * p + o) * p + s], t[elem_off(str, d, slot, chi, r, b, s)]);
+        A[f] = acc;
+      }
     }
-    for (int f = threadIdx.x; f < m * chi; f += blockDim.x) {
-      const int R = f / chi, j = f % chi;
-      const int r = R / p, o = R % p;
-      const double mk_r = row_weight(w, d, slot, chi, r, -1, true, eps);
-      Y[f] = j < kr ? mk_r * tb[elem_off(str, d, slot, chi, r, j, o)] : mk<double>(0.0, 0.0);
+    // Y = masked output gradient / nu (the 1 / nu of Kbar and B folded in)
+    const double inv_nu = 1.0 / nu[side];
+    for (int f0 = threadIdx.x; f0 < m * chi; f0 += kU * blockDim.x) {
+      cd yv[kU];
+      double wv[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int f = f0 + u * blockDim.x;
+        yv[u] = mk<double>(0.0, 0.0);
+        wv[u] = 0.0;
+        if (f < m * chi) {
+          const int R = f / chi, j = f - (f / chi) * chi;
+          const int r = R / p, o = R - (R / p) * p;
+          if (j < kr) {
+            wv[u] = inv_nu * row_weight(w, d, slot, chi, r, -1, true, eps);
+            yv[u] = tb[elem_off(str, d, slot, chi, r, j, o)];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int f = f0 + u * blockDim.x;
+        if (f < m * chi) Y[f] = wv[u] * yv[u];
+      }
     }
     __syncthreads();
-    // Kbar = Atil^H Y / nu  (n x chi)
+    // Kbar = Atil^H Y  (n x chi)
     adj::bmm(Kbar, chi, A, n, adj::H, Y, chi, adj::N, n, chi, m, false);
-    for (int f = threadIdx.x; f < n * chi; f += blockDim.x) Kbar[f] = (1.0 / nu[side]) * Kbar[f];
-    __syncthreads();
     // nubar = -Re<Kbar, K> / nu
     double acc = 0.0;
     for (int f = threadIdx.x; f < n * chi; f += blockDim.x)
       acc += Kbar[f].re * K[f].re + Kbar[f].im * K[f].im;
     acc = block_sum(acc, red);
     nubar[side] = -acc / nu[side];
-    // write-path gradient of Atil: B = Y K^H / nu
+    // write-path gradient of Atil: B = Y K^H
     adj::bmm(B, n, Y, chi, adj::N, K, chi, adj::H, m, n, chi, false);
-    for (int f = threadIdx.x; f < m * n; f += blockDim.x) B[f] = (1.0 / nu[side]) * B[f];
-    __syncthreads();
   }
 
   // ---------------- core adjoint
@@ -647,31 +710,68 @@ __global__ void __launch_bounds__(256)
     // AE = Atil Ebar;  B += 2 D^2 AE (Gram adjoint, D the row weight)
     cd* AE = W.AE();
     adj::bmm(AE, n, A, n, adj::N, w2, n, adj::N, m, n, n, false);
-    for (int f = threadIdx.x; f < m * n; f += blockDim.x) {
-      const double D = row_weight(w, d, slot, chi, (f / n) / p, -1, false, eps);
-      B[f] = B[f] + (2.0 * D * D) * AE[f];
-    }
-    __syncthreads();
-    // row weight gradient: rho_r = 2 D_r Re sum_{o,c} conj(AE[(r,o),c]) Atil[(r,o),c];
-    // lambda_j[b_j(r)] += rho_r * prod of the other weights of row r
-    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
-      double rho = 0.0;
-      for (int o = 0; o < p; ++o) {
-        const long long Rr = (long long)r * p + o;
-        for (int c = 0; c < n; ++c) {
-          const cd ae = AE[Rr * n + c], a = A[Rr * n + c];
-          rho += ae.re * a.re + ae.im * a.im;
+    const int L = p * n;  // elements of one row r: (o, c), contiguous
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63, nwv = blockDim.x >> 6;
+    if (p == 2 && L <= 64) {
+      // one wave per row, four rows per batch (loads first): B += 2 D^2 AE and
+      // rho_r = 2 D_r Re sum conj(AE) Atil over the row (wave sum);
+      // lambda_j[b_j(r)] += rho_r * prod of the other weights of row r
+      for (int r0 = 4 * wv; r0 < rows; r0 += 4 * nwv) {
+        cd aev[4], av[4], bv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const long long f = (long long)(r0 + u) * L + ln;
+          if (r0 + u < rows && ln < L) {
+            aev[u] = AE[f];
+            av[u] = A[f];
+            bv[u] = B[f];
+          } else {
+            aev[u] = av[u] = bv[u] = mk<double>(0.0, 0.0);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int r = r0 + u;
+          if (r >= rows) break;
+          const double D = row_weight(w, d, slot, chi, r, -1, false, eps);
+          if (ln < L) B[(long long)r * L + ln] = bv[u] + (2.0 * D * D) * aev[u];
+          const double rho = 2.0 * D * wave_sum(aev[u].re * av[u].re + aev[u].im * av[u].im);
+          if (rho != 0.0 && ln < d && ln != slot) {
+            const int j = ln;
+            const int bj = row_bond(d, slot, chi, r, j);
+            const double wj = row_weight(w, d, slot, chi, r, j, false, eps);
+            const int ej = g.adj_edge[g.adj_ptr[nod] + j];
+            atomicAdd(&lambar[(long long)ej * chi + bj], rho * wj);
+          }
         }
       }
-      const double D = row_weight(w, d, slot, chi, r, -1, false, eps);
-      rho *= 2.0 * D;
-      if (rho != 0.0) {
-        for (int j = 0; j < d; ++j) {
-          if (j == slot) continue;
-          const int bj = row_bond(d, slot, chi, r, j);
-          const double wj = row_weight(w, d, slot, chi, r, j, false, eps);
-          const int ej = g.adj_edge[g.adj_ptr[nod] + j];
-          atomicAdd(&lambar[(long long)ej * chi + bj], rho * wj);
+    } else {
+      for (int f = threadIdx.x; f < m * n; f += blockDim.x) {
+        const double D = row_weight(w, d, slot, chi, (f / n) / p, -1, false, eps);
+        B[f] = B[f] + (2.0 * D * D) * AE[f];
+      }
+      __syncthreads();
+      // row weight gradient: rho_r = 2 D_r Re sum_{o,c} conj(AE[(r,o),c]) Atil[(r,o),c];
+      // lambda_j[b_j(r)] += rho_r * prod of the other weights of row r
+      for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+        double rho = 0.0;
+        for (int o = 0; o < p; ++o) {
+          const long long Rr = (long long)r * p + o;
+          for (int c = 0; c < n; ++c) {
+            const cd ae = AE[Rr * n + c], a = A[Rr * n + c];
+            rho += ae.re * a.re + ae.im * a.im;
+          }
+        }
+        const double D = row_weight(w, d, slot, chi, r, -1, false, eps);
+        rho *= 2.0 * D;
+        if (rho != 0.0) {
+          for (int j = 0; j < d; ++j) {
+            if (j == slot) continue;
+            const int bj = row_bond(d, slot, chi, r, j);
+            const double wj = row_weight(w, d, slot, chi, r, j, false, eps);
+            const int ej = g.adj_edge[g.adj_ptr[nod] + j];
+            atomicAdd(&lambar[(long long)ej * chi + bj], rho * wj);
+          }
         }
       }
     }
@@ -680,6 +780,89 @@ __global__ void __launch_bounds__(256)
     const cd* t = Tpre + g.toff[nod];
     cd* tbw = Tbar + g.toff[nod];
     const cd* h = hh[side];
+    cd* hb = hbar + (long long)blockIdx.x * 2 * hstride + side * hstride;
+    if (p == 2 && rg <= 2) {
+      // one pass over (r, b), two pairs per batch (loads first):
+      //   Gamma-bar[r, b, s] = sum_{o, q} conj(h[q][o][s]) B[(r, o), (b, q)]
+      //   h-bar[q][o][s]     = sum_{r, b} B[(r, o), (b, q)] conj(T[r, b, s])
+      // with the 16 h-bar partials of each thread reduced once at the end
+      __shared__ cd hs[16];
+      if (threadIdx.x < 16) hs[threadIdx.x] = h[threadIdx.x];
+      __syncthreads();
+      double ar[16], ai[16];
+#pragma unroll
+      for (int f = 0; f < 16; ++f) ar[f] = ai[f] = 0.0;
+      constexpr int kB = 2;
+      for (int rb0 = threadIdx.x; rb0 < rows * chi; rb0 += kB * blockDim.x) {
+        cd y[kB][2], x[kB][2][2];
+        long long to[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+          const int rb = rb0 + u * blockDim.x;
+          to[u] = -1;
+          y[u][0] = y[u][1] = mk<double>(0.0, 0.0);
+#pragma unroll
+          for (int o = 0; o < 2; ++o) x[u][o][0] = x[u][o][1] = mk<double>(0.0, 0.0);
+          if (rb < rows * chi) {
+            const int r = rb / chi, b = rb - (rb / chi) * chi;
+            to[u] = elem_off(str, d, slot, chi, r, b, 0);
+            y[u][0] = t[to[u]];
+            y[u][1] = t[to[u] + 1];
+#pragma unroll
+            for (int o = 0; o < 2; ++o)
+#pragma unroll
+              for (int q = 0; q < 2; ++q)
+                if (q < rg) x[u][o][q] = B[(long long)(r * 2 + o) * n + b * rg + q];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+          if (to[u] < 0) continue;
+#pragma unroll
+          for (int sp = 0; sp < 2; ++sp) {
+            cd acc2 = mk<double>(0.0, 0.0);
+#pragma unroll
+            for (int o = 0; o < 2; ++o)
+#pragma unroll
+              for (int q = 0; q < 2; ++q)
+                if (q < rg) fma_acc(acc2, conj(hs[(q * 2 + o) * 2 + sp]), x[u][o][q]);
+            tbw[to[u] + sp] = acc2;
+          }
+#pragma unroll
+          for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int o = 0; o < 2; ++o) {
+              const cd xx = x[u][o][q];
+              const int f = (q * 2 + o) * 2;
+              ar[f] += xx.re * y[u][0].re + xx.im * y[u][0].im;
+              ai[f] += xx.im * y[u][0].re - xx.re * y[u][0].im;
+              ar[f + 1] += xx.re * y[u][1].re + xx.im * y[u][1].im;
+              ai[f + 1] += xx.im * y[u][1].re - xx.re * y[u][1].im;
+            }
+        }
+      }
+      __shared__ double hred[32][8];
+#pragma unroll
+      for (int f = 0; f < 16; ++f) {
+        const double sr = wave_sum(ar[f]), si = wave_sum(ai[f]);
+        if (ln == 0) {
+          hred[2 * f][wv] = sr;
+          hred[2 * f + 1][wv] = si;
+        }
+      }
+      __syncthreads();
+      if (threadIdx.x < 16) {
+        const int f = threadIdx.x;
+        double sr = 0.0, si = 0.0;
+        for (int w2i = 0; w2i < nwv; ++w2i) {
+          sr += hred[2 * f][w2i];
+          si += hred[2 * f + 1][w2i];
+        }
+        hb[f] = mk<double>(sr, si);
+      }
+      __syncthreads();
+      continue;
+    }
     const long long S = ipow(chi, d) * p;
     for (long long f = threadIdx.x; f < S; f += blockDim.x) {
       // decode f -> (row r, bond b at slot, phys s)
@@ -705,7 +888,6 @@ __global__ void __launch_bounds__(256)
     // h-bar[q][o][s] = sum_{r, b} B[(r, o), (b, q)] conj(T[r, b, s]): the (r, b)
     // pairs are spread over the block, one partial per (q, o, s) per thread,
     // then block sums (q < rank; the rest are zero)
-    cd* hb = hbar + (long long)blockIdx.x * 2 * hstride + side * hstride;
     for (int f = 0; f < (int)hstride; ++f) {
       const int q = f / (p * p), o = (f / p) % p, s = f % p;
       double ar = 0.0, ai = 0.0;

@@ -145,12 +145,37 @@ __global__ void __launch_bounds__(64 * NW)
   __shared__ float2 tl[NPAD];
   __shared__ float2 wk[NPAD];
   __shared__ float2 diag[3 * 16];
+  __shared__ float2 mb[3 * 16 * 17];  // incoming messages, row pitch 17
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   // Persistent blocks (one per CU: the two tensors take most of the LDS):
   // block b updates nodes[b], nodes[b + gridDim.x], ...  The next node's
-  // tensor is loaded into registers while the current one is processed, so
-  // its HBM latency is hidden behind the matrix-core work.
+  // tensor is loaded into registers while the current one is processed, and
+  // its three incoming messages while the current outputs are reduced, so
+  // their HBM latency is hidden behind the matrix-core work.
   auto active = [&](int it) { return !inst_active || inst_active[g.node_inst[nodes[it]]]; };
+  constexpr int QM = (768 + NT - 1) / NT;  // message elements per thread
+  auto fetch_msgs = [&](int nit, float2 (&r)[QM]) {
+    const int base = g.adj_ptr[nodes[nit]];
+#pragma unroll
+    for (int qi = 0; qi < QM; ++qi) {
+      const int q = tid + qi * NT;
+      if (q < 768) {
+        const int k = q >> 8;
+        const int e = g.adj_edge[base + k], sd = g.adj_side[base + k];
+        const int code = node_src ? node_src[base + k] : 0;
+        const float2* src = code == 0 ? e2n_old + (2LL * e + sd) * 256
+                                      : (code == 1 ? n2e_new : n2e_old) + (2LL * e + 1 - sd) * 256;
+        r[qi] = src[q & 255];
+      }
+    }
+  };
+  auto store_msgs = [&](const float2 (&r)[QM]) {
+#pragma unroll
+    for (int qi = 0; qi < QM; ++qi) {
+      const int q = tid + qi * NT;
+      if (q < 768) mb[(q >> 8) * 272 + ((q >> 4) & 15) * 17 + (q & 15)] = r[qi];
+    }
+  };
   int it = blockIdx.x;
   while (it < count && !active(it)) it += gridDim.x;
   if (it >= count) return;
@@ -159,31 +184,26 @@ __global__ void __launch_bounds__(64 * NW)
     const float2* tg = T + g.toff[nodes[it]];
 #pragma unroll
     for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * NT];
+    float2 mr[QM];
+    fetch_msgs(it, mr);
+    store_msgs(mr);
   }
   while (true) {
     const int node = nodes[it];
     const int base = g.adj_ptr[node];
-    // stage T into LDS (padded); the previous node's last reads are behind
-    // the barrier that ended its loop iteration
+    // stage T into LDS (padded); the previous node's last reads of tl are
+    // behind the barrier that precedes its partial writes
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int f = tid + k * NT;
       const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
       tl[b0 * B0P + b1 * RP + r] = pre[k];
     }
-    // message operands in registers: m[j][ks] = M_j[lane%16][4 ks + lane/16]
-    float2 m[3][4];
     int oe[3], os[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const int e = g.adj_edge[base + k], sd = g.adj_side[base + k];
-      oe[k] = e;
-      os[k] = sd;
-      const int code = node_src ? node_src[base + k] : 0;
-      const float2* src = code == 0 ? e2n_old + (2LL * e + sd) * 256
-                                    : (code == 1 ? n2e_new : n2e_old) + (2LL * e + 1 - sd) * 256;
-#pragma unroll
-      for (int ks = 0; ks < 4; ++ks) m[k][ks] = src[(lane & 15) * 16 + 4 * ks + (lane >> 4)];
+      oe[k] = g.adj_edge[base + k];
+      os[k] = g.adj_side[base + k];
     }
     int next = it + gridDim.x;
     while (next < count && !active(next)) next += gridDim.x;
@@ -193,6 +213,12 @@ __global__ void __launch_bounds__(64 * NW)
       for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * NT];
     }
     __syncthreads();
+    // message operands: m[j][ks] = M_j[lane%16][4 ks + lane/16]
+    float2 m[3][4];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) m[k][ks] = mb[k * 272 + (lane & 15) * 17 + 4 * ks + (lane >> 4)];
     floatx4 gr[3], gi[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) gr[k] = gi[k] = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -204,6 +230,8 @@ __global__ void __launch_bounds__(64 * NW)
     mode_product<NW>(wk, tl, m[0], 0, wave, lane);  // X = T x_0 M0
     __syncthreads();
     mode_gram<NW>(tl, wk, m[1], 1, 2, 0, wave, lane, gr[2], gi[2]);  // U_2 = X x_1 M1
+    float2 mr[QM];
+    if (next < count) fetch_msgs(next, mr);
     __syncthreads();
     // wave partials of the three Grams into the work tensor, summed over the
     // waves, divided by their traces
@@ -250,8 +278,10 @@ __global__ void __launch_bounds__(64 * NW)
     }
     // the next node's staging writes only tl, last read before the barrier
     // that precedes the partial writes; its first write of wk follows the
-    // staging barrier, which every thread reaches after its reads of part
+    // staging barrier, which every thread reaches after its reads of part;
+    // mb was last read right after the previous staging barrier
     if (next >= count) break;
+    store_msgs(mr);
     it = next;
   }
 }
@@ -291,23 +321,23 @@ __global__ void __launch_bounds__(64 * fast64::NW)
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) m[k][ks] = src[(lane & 15) * 16 + 4 * ks + (lane >> 4)];
   }
-  const int plan[3][3] = {{2, 1, 0}, {2, 0, 1}, {0, 1, 2}};
   d4 gr[3], gi[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) gr[k] = gi[k] = d4{0.0, 0.0, 0.0, 0.0};
   for (int s = 0; s < 2; ++s) {
     for (int f = tid; f < 4096; f += 64 * NW) tl[(f >> 8) * B0P + ((f >> 4) & 15) * RP + (f & 15)] = tg[2 * f + s];
     __syncthreads();
-#pragma unroll
-    for (int step = 0; step < 3; ++step) {
-      const int j1 = plan[step][0], j2 = plan[step][1], idx = plan[step][2];
-      mode_product(wk, tl, m[j1], j1, wave, lane);
-      __syncthreads();
-      mode_product(wk, wk, m[j2], j2, wave, lane);
-      __syncthreads();
-      gram_partial(tl, wk, idx, wave, lane, gr[idx], gi[idx]);
-      __syncthreads();
-    }
+    // X = T x_2 M2 serves U_0 = X x_1 M1 and U_1 = X x_0 M0; X = T x_0 M0
+    // serves U_2 = X x_1 M1; each U's second product is fused with its Gram
+    mode_product(wk, tl, m[2], 2, wave, lane);
+    __syncthreads();
+    mode_gram(tl, wk, m[1], 1, 0, 2, wave, lane, gr[0], gi[0]);
+    mode_gram(tl, wk, m[0], 0, 1, 2, wave, lane, gr[1], gi[1]);
+    __syncthreads();
+    mode_product(wk, tl, m[0], 0, wave, lane);
+    __syncthreads();
+    mode_gram(tl, wk, m[1], 1, 2, 0, wave, lane, gr[2], gi[2]);
+    __syncthreads();
   }
   // reduce the wave partials through LDS (the work tensor is free), one
   // output at a time, then divide by the trace

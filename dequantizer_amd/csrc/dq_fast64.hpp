@@ -64,6 +64,41 @@ __device__ __forceinline__ void mode_product(cd* dst, const cd* src, const cd (&
     for (int v = 0; v < 4; ++v) dst[cb + (kq + 4 * v) * st] = mk<double>(ar[v], ai[v]);
   }
 }
+// U = X x_j M fused with this wave's share of the Gram over axis idx,
+// G[k][l] += sum conj(T[k@idx, ...]) U[l@idx, ...]: tiles enumerate the third
+// axis tax, lane l%16 runs along idx; the product's D[l/16 + 4v][l%16] =
+// U[b_j = l/16 + 4v][b_idx = l%16] is, for each v, the B operand of a Gram
+// k-step over b_j in 4v..4v+3, with conj(T) at the same (b_j, b_tax) as the
+// A operand.
+__device__ __forceinline__ void mode_gram(const cd* t, const cd* x, const cd (&m)[4], int j, int idx, int tax,
+                                          int wave, int lane, d4& gr, d4& gi) {
+  const int sj = axis_stride(j);
+  const int y = lane & 15, kq = lane >> 4;
+  const int lane_base = y * axis_stride(idx);
+  for (int tile = wave; tile < 16; tile += NW) {
+    const int cb = lane_base + tile * axis_stride(tax);
+    cd xv[4], tv[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) xv[ks] = x[cb + (4 * ks + kq) * sj];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) tv[v] = t[cb + (kq + 4 * v) * sj];
+    d4 ur = {0.0, 0.0, 0.0, 0.0}, ui = ur;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      ur = mfma(m[ks].re, xv[ks].re, ur);
+      ur = mfma(-m[ks].im, xv[ks].im, ur);
+      ui = mfma(m[ks].re, xv[ks].im, ui);
+      ui = mfma(m[ks].im, xv[ks].re, ui);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      gr = mfma(tv[v].re, ur[v], gr);
+      gr = mfma(tv[v].im, ui[v], gr);
+      gi = mfma(tv[v].re, ui[v], gi);
+      gi = mfma(-tv[v].im, ur[v], gi);
+    }
+  }
+}
 // this wave's share of sum_r conj(T[k][r]) U[l][r] along axis idx
 __device__ __forceinline__ void gram_partial(const cd* t, const cd* u, int idx, int wave, int lane, d4& gr, d4& gi) {
   const int st = axis_stride(idx);
