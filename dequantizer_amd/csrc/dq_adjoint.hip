@@ -1036,20 +1036,23 @@ __global__ void __launch_bounds__(64 * fast64::NW)
   d4 gr[3], gi[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) gr[k] = gi[k] = d4{0.0, 0.0, 0.0, 0.0};
-  for (int s = 0; s < 2; ++s) {
-    stage_component(tl, tg, s);
-    __syncthreads();
+  {
+    cd mm[3][4];
 #pragma unroll
-    for (int idx = 0; idx < 3; ++idx) {
-      int j1, j2;
-      others(idx, j1, j2);
-      load_operand(Mg[j1], 0, lane, m1);
-      load_operand(Mg[j2], 0, lane, m2);
-      mode_product(wk, tl, m1, j1, wave, lane);
+    for (int k = 0; k < 3; ++k) load_operand(Mg[k], 0, lane, mm[k]);
+    for (int s = 0; s < 2; ++s) {
+      stage_component(tl, tg, s);
       __syncthreads();
-      mode_product(wk, wk, m2, j2, wave, lane);
+      // as the forward kernel: X = T x_2 M2 serves U_0 and U_1, X = T x_0 M0
+      // serves U_2, each U's second product fused with its Gram
+      mode_product(wk, tl, mm[2], 2, wave, lane);
       __syncthreads();
-      gram_partial(tl, wk, idx, wave, lane, gr[idx], gi[idx]);
+      mode_gram(tl, wk, mm[1], 1, 0, 2, wave, lane, gr[0], gi[0]);
+      mode_gram(tl, wk, mm[0], 0, 1, 2, wave, lane, gr[1], gi[1]);
+      __syncthreads();
+      mode_product(wk, tl, mm[0], 0, wave, lane);
+      __syncthreads();
+      mode_gram(tl, wk, mm[1], 1, 2, 0, wave, lane, gr[2], gi[2]);
       __syncthreads();
     }
   }
@@ -1088,10 +1091,25 @@ __global__ void __launch_bounds__(64 * fast64::NW)
     }
     __syncthreads();
   }
-  // pass 2: starts with component 1, still in LDS
-  d4 mr[3], mi[3];
+  // pass 2: starts with component 1, still in LDS.  H_j accumulates
+  // sum conj(T[a@j]) Z[b@j] over the Z of bond j (Ubar1 or X); Mbar_j[b][a] = H_j[a][b].
+  d4 hr[3], hi[3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) mr[k] = mi[k] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < 3; ++k) hr[k] = hi[k] = d4{0.0, 0.0, 0.0, 0.0};
+  constexpr int TPER = 4096 / (64 * NW);
+  // Gamma-bar += the work tensor (this thread's elements f = tid + 512 k of
+  // component s; all loads first)
+  auto add_tbar = [&](int s) {
+    cd v[TPER], o[TPER];
+#pragma unroll
+    for (int k = 0; k < TPER; ++k) {
+      const int f = tid + k * 64 * NW;
+      v[k] = wk[(f >> 8) * B0P + ((f >> 4) & 15) * RP + (f & 15)];
+      o[k] = tb[2 * f + s];
+    }
+#pragma unroll
+    for (int k = 0; k < TPER; ++k) tb[2 * (tid + k * 64 * NW) + s] = o[k] + v[k];
+  };
   for (int s = 1; s >= 0; --s) {
     if (s == 0) {
       stage_component(tl, tg, 0);
@@ -1104,28 +1122,21 @@ __global__ void __launch_bounds__(64 * fast64::NW)
       load_operand(Gb[idx], 2, lane, op);  // Gbar^T
       load_operand(Mg[j1], 3, lane, m1);   // M1^H
       load_operand(Mg[j2], 3, lane, m2);   // M2^H
-      // Ubar1 = (T x_idx Gbar^T) x_j2 M2^H
+      // Y = T x_idx Gbar^T
       mode_product(wk, tl, op, idx, wave, lane);
       __syncthreads();
-      mode_product(wk, wk, m2, j2, wave, lane);
+      // X = Y x_j1 M1^H (not stored) with its Gram over j2
+      mode_gram(tl, wk, m1, j1, j2, idx, wave, lane, hr[j2], hi[j2]);
       __syncthreads();
-      gram_partial(wk, tl, j1, wave, lane, mr[j1], mi[j1]);
+      // Ubar1 = Y x_j2 M2^H (stored in place) with its Gram over j1
+      mode_gram<true>(tl, wk, m2, j2, j1, idx, wave, lane, hr[j1], hi[j1]);
       __syncthreads();
+      // Tbar += Ubar1 x_j1 M1^H
       mode_product(wk, wk, m1, j1, wave, lane);
       __syncthreads();
-      for (int f = tid; f < 4096; f += blockDim.x) {
-        const int q = (f >> 8) * B0P + ((f >> 4) & 15) * RP + (f & 15);
-        tb[2 * f + s] = tb[2 * f + s] + wk[q];
-      }
+      add_tbar(s);
       __syncthreads();
-      // X = (T x_idx Gbar^T) x_j1 M1^H
-      mode_product(wk, tl, op, idx, wave, lane);
-      __syncthreads();
-      mode_product(wk, wk, m1, j1, wave, lane);
-      __syncthreads();
-      gram_partial(wk, tl, j2, wave, lane, mr[j2], mi[j2]);
-      __syncthreads();
-      // bra side: U x_idx conj(Gbar)
+      // bra side: Tbar += U x_idx conj(Gbar), U = (T x_j1 M1) x_j2 M2
       load_operand(Mg[j1], 0, lane, m1);
       load_operand(Mg[j2], 0, lane, m2);
       load_operand(Gb[idx], 1, lane, op);
@@ -1135,24 +1146,21 @@ __global__ void __launch_bounds__(64 * fast64::NW)
       __syncthreads();
       mode_product(wk, wk, op, idx, wave, lane);
       __syncthreads();
-      for (int f = tid; f < 4096; f += blockDim.x) {
-        const int q = (f >> 8) * B0P + ((f >> 4) & 15) * RP + (f & 15);
-        tb[2 * f + s] = tb[2 * f + s] + wk[q];
-      }
+      add_tbar(s);
       __syncthreads();
     }
   }
-  // Mbar_j = conj(sum of the Gram partials), reduced over the waves
+  // Mbar_j[b][a] = H_j[a][b], summed over the waves
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
 #pragma unroll
     for (int v = 0; v < 4; ++v)
-      part[wave * 256 + ((lane >> 4) + 4 * v) * 16 + (lane & 15)] = mk<double>(mr[j][v], mi[j][v]);
+      part[wave * 256 + (lane & 15) * 16 + (lane >> 4) + 4 * v] = mk<double>(hr[j][v], hi[j][v]);
     __syncthreads();
     if (tid < 256) {
       cd sum = mk<double>(0.0, 0.0);
       for (int w = 0; w < NW; ++w) sum = sum + part[w * 256 + tid];
-      ebar[slot[j] * 256 + tid] = conj(sum);
+      ebar[slot[j] * 256 + tid] = sum;
     }
     __syncthreads();
   }

@@ -70,8 +70,11 @@ __device__ __forceinline__ void mode_product(cd* dst, const cd* src, const cd (&
 // U[b_j = l/16 + 4v][b_idx = l%16] is, for each v, the B operand of a Gram
 // k-step over b_j in 4v..4v+3, with conj(T) at the same (b_j, b_tax) as the
 // A operand.
-__device__ __forceinline__ void mode_gram(const cd* t, const cd* x, const cd (&m)[4], int j, int idx, int tax,
-                                          int wave, int lane, d4& gr, d4& gi) {
+// With kStore the product tile is also written back over its input (in
+// place: each wave reads its whole tile before writing it).
+template <bool kStore = false>
+__device__ __forceinline__ void mode_gram(const cd* t, cd* x, const cd (&m)[4], int j, int idx, int tax, int wave,
+                                          int lane, d4& gr, d4& gi) {
   const int sj = axis_stride(j);
   const int y = lane & 15, kq = lane >> 4;
   const int lane_base = y * axis_stride(idx);
@@ -96,6 +99,10 @@ __device__ __forceinline__ void mode_gram(const cd* t, const cd* x, const cd (&m
       gr = mfma(tv[v].im, ui[v], gr);
       gi = mfma(tv[v].re, ui[v], gi);
       gi = mfma(-tv[v].im, ur[v], gi);
+    }
+    if (kStore) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) x[cb + (kq + 4 * v) * sj] = mk<double>(ur[v], ui[v]);
     }
   }
 }
