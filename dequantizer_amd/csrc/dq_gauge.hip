@@ -401,6 +401,39 @@ __global__ void __launch_bounds__(256)
   const cx<R>* t = gam + g.toff[node];
   cx<R>* o = out + g.toff[node];
   double acc = 0.0;
+  constexpr int KE = 32;  // elements per thread kept in registers (fast path)
+  if (S <= (long long)KE * blockDim.x && d <= kMaxDeg && chi <= 64) {
+    // bond factors once per node, 32-bit index arithmetic, one read and one
+    // write of the tensor (same factor order and summation order as below)
+    __shared__ double sq[kMaxDeg * 64];
+    for (int i = threadIdx.x; i < d * chi; i += blockDim.x)
+      sq[i] = sqrt(safe_truncate(lam[(long long)g.adj_edge[base + i / chi] * chi + i % chi], eps));
+    __syncthreads();
+    cx<R> v[KE];
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int f = threadIdx.x + k * blockDim.x;
+      if (f < S) {
+        int r = f / phys;
+        double fac = 1.0;
+        for (int j = d - 1; j >= 0; --j) {
+          const int q = r / chi;
+          fac *= sq[j * chi + (r - q * chi)];
+          r = q;
+        }
+        v[k] = (R)fac * t[f];
+        acc += (double)norm2(v[k]);
+      }
+    }
+    const double nrm = sqrt(block_sum(acc, red));
+    const R inv = (R)(1.0 / nrm);
+#pragma unroll
+    for (int k = 0; k < KE; ++k) {
+      const int f = threadIdx.x + k * blockDim.x;
+      if (f < S) o[f] = inv * v[k];
+    }
+    return;
+  }
   for (long long f = threadIdx.x; f < S; f += blockDim.x) {
     long long r = f / phys;
     double fac = 1.0;

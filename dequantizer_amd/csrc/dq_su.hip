@@ -24,6 +24,39 @@ __global__ void __launch_bounds__(256)
   cx<R>* t = T + g.toff[node];
   const cd* G = gates + (long long)blockIdx.x * p * p;
   double acc = 0.0;
+  constexpr int KR = 16;  // rows per thread kept in registers (qubit fast path)
+  if (p == 2 && rows <= (long long)KR * blockDim.x) {
+    // one read and one write of the tensor: the gated rows stay in registers
+    // until the norm is known (same arithmetic and summation order as below)
+    const cx<R> g00 = cvt<R>(G[0]), g01 = cvt<R>(G[1]), g10 = cvt<R>(G[2]), g11 = cvt<R>(G[3]);
+    cx<R> v0[KR], v1[KR];
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      const long long r = threadIdx.x + (long long)k * blockDim.x;
+      if (r < rows) {
+        const cx<R> a = t[2 * r], b = t[2 * r + 1];
+        v0[k] = mk<R>(0, 0);
+        fma_acc(v0[k], g00, a);
+        fma_acc(v0[k], g01, b);
+        v1[k] = mk<R>(0, 0);
+        fma_acc(v1[k], g10, a);
+        fma_acc(v1[k], g11, b);
+        acc += (double)norm2(v0[k]);
+        acc += (double)norm2(v1[k]);
+      }
+    }
+    const double nrm = sqrt(block_sum(acc, red));
+    const R inv = (R)(1.0 / nrm);
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      const long long r = threadIdx.x + (long long)k * blockDim.x;
+      if (r < rows) {
+        t[2 * r] = inv * v0[k];
+        t[2 * r + 1] = inv * v1[k];
+      }
+    }
+    return;
+  }
   cx<R> in[16];
   for (long long r = threadIdx.x; r < rows; r += blockDim.x) {
     for (int i = 0; i < p; ++i) in[i] = t[r * p + i];
