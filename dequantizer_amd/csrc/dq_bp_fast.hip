@@ -485,6 +485,136 @@ bool bp_fast_applicable(int rb, const DevGraph& g, int min_deg, int max_deg) {
   return rb == 4 && g.chi == 16 && g.phys == 2 && min_deg == 3 && max_deg == 3;
 }
 
+// The same node update with diagonal messages diag(lambda_e^2) on every bond
+// (the lambda^2-weighted bond Grams of the Vidal distance): a mode product
+// with a diagonal matrix scales the tensor along that axis, so each output is
+// one Gram, out_idx[k][l] = sum conj(T[k@idx, c]) w(c) T[l@idx, c] with
+// w(c) the product of the two other bonds' lambda^2 at column c, and the
+// matrix cores do only the three Grams.  Weights are rounded to float as the
+// diagonal messages of the general path are.  Persistent blocks as
+// k_bp_nodes_d3c16.
+__global__ void __launch_bounds__(512)
+    k_bp_nodes_lam2_d3c16(DevGraph g, const int* nodes, int count, const float2* T, const double* lam,
+                          float2* n2e_new, const int* inst_active) {
+  using namespace fast3;
+  constexpr int NW = 8, NT = 512, PER = S / NT;
+  __shared__ float2 tl[NPAD];
+  __shared__ float2 part[3 * NW * 256];
+  __shared__ float wl[3][16];
+  __shared__ float2 diag[3 * 16];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  auto active = [&](int it) { return !inst_active || inst_active[g.node_inst[nodes[it]]]; };
+  int it = blockIdx.x;
+  while (it < count && !active(it)) it += gridDim.x;
+  if (it >= count) return;
+  float2 pre[PER];
+  {
+    const float2* tg = T + g.toff[nodes[it]];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * NT];
+  }
+  while (true) {
+    const int node = nodes[it];
+    const int base = g.adj_ptr[node];
+    // tl and wl were last read before the barrier that precedes the partial
+    // writes of the previous node
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int f = tid + k * NT;
+      const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
+      tl[b0 * B0P + b1 * RP + r] = pre[k];
+    }
+    if (tid < 48) {
+      const double v = lam[(long long)g.adj_edge[base + (tid >> 4)] * 16 + (tid & 15)];
+      wl[tid >> 4][tid & 15] = (float)(v * v);
+    }
+    int oe[3], os[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      oe[k] = g.adj_edge[base + k];
+      os[k] = g.adj_side[base + k];
+    }
+    int next = it + gridDim.x;
+    while (next < count && !active(next)) next += gridDim.x;
+    if (next < count) {
+      const float2* tg = T + g.toff[nodes[next]];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * NT];
+    }
+    __syncthreads();
+    const int row = lane & 15, kq = lane >> 4;
+    constexpr int per = 128 / NW;
+#pragma unroll
+    for (int idx = 0; idx < 3; ++idx) {
+      // columns c enumerate the other two bonds (lower axis first) and s
+      const int jlo = idx == 0 ? 1 : 0, jhi = idx == 2 ? 1 : 2;
+      const int st = axis_stride(idx);
+      floatx4 gr = {0.f, 0.f, 0.f, 0.f}, gi = {0.f, 0.f, 0.f, 0.f};
+      for (int ks = wave * per; ks < wave * per + per; ++ks) {
+        const int c = 4 * ks + kq;
+        const float2 a = tl[col_base(idx, c) + row * st];
+        const float w = wl[jlo][c >> 5] * wl[jhi][(c >> 1) & 15];
+        const float2 b = make_float2(a.x * w, a.y * w);
+        gr = mfma(a.x, b.x, gr);
+        gr = mfma(a.y, b.y, gr);
+        gi = mfma(a.x, b.y, gi);
+        gi = mfma(-a.y, b.x, gi);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        part[(idx * NW + wave) * 256 + (4 * (lane >> 4) + v) * 16 + (lane & 15)] = make_float2(gr[v], gi[v]);
+    }
+    __syncthreads();
+    constexpr int QN = (768 + NT - 1) / NT;
+    float2 sum[QN];
+#pragma unroll
+    for (int qi = 0; qi < QN; ++qi) {
+      const int q = tid + qi * NT;
+      sum[qi] = make_float2(0.f, 0.f);
+      if (q < 768) {
+        const int k = q >> 8, e = q & 255;
+        for (int w = 0; w < NW; ++w) {
+          sum[qi].x += part[(k * NW + w) * 256 + e].x;
+          sum[qi].y += part[(k * NW + w) * 256 + e].y;
+        }
+        if (e % 17 == 0) diag[k * 16 + e / 17] = sum[qi];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int qi = 0; qi < QN; ++qi) {
+      const int q = tid + qi * NT;
+      if (q < 768) {
+        const int k = q >> 8, e = q & 255;
+        float tr_r = 0.f, tr_i = 0.f;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+          tr_r += diag[k * 16 + c].x;
+          tr_i += diag[k * 16 + c].y;
+        }
+        const float den = tr_r * tr_r + tr_i * tr_i;
+        float2 o;
+        o.x = (sum[qi].x * tr_r + sum[qi].y * tr_i) / den;
+        o.y = (sum[qi].y * tr_r - sum[qi].x * tr_i) / den;
+        n2e_new[(2LL * oe[k] + os[k]) * 256 + e] = o;
+      }
+    }
+    // part and diag are next written after the next node's staging barrier
+    if (next >= count) break;
+    it = next;
+  }
+}
+
+hipError_t launch_bp_nodes_lam2_fast(const DevGraph& g, const int* nodes, int count, const void* T,
+                                     const double* lam, void* n2e_new, const int* inst_active, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  const int cus = device_cu_count();
+  const int grid = count < cus ? count : cus;
+  hipLaunchKernelGGL(k_bp_nodes_lam2_d3c16, dim3(grid), dim3(512), 0, s, g, nodes, count, (const float2*)T, lam,
+                     (float2*)n2e_new, inst_active);
+  return hipGetLastError();
+}
+
 hipError_t launch_bp_nodes_fast(const DevGraph& g, const int* nodes, int count, const void* T, const void* n2e_old,
                                 const void* e2n_old, void* n2e_new, const uint8_t* node_src, const int* inst_active,
                                 hipStream_t s) {

@@ -511,19 +511,9 @@ hipError_t launch_vidal_distance(int rb, const DevGraph& g, const void* gam, con
 // complex64).  The trace-normalised R of vidal_dist (tensor_ops.py:143-175),
 // R = Gram over the other bonds of conj(Gamma) and Gamma * prod lambda_i^2,
 // is the BP node update (tensor_ops.py:36-60) with the diagonal messages
-// diag(lambda^2) on every bond: the MFMA node kernel computes all of them,
-// one per (edge, side).  One wave then forms R - ideal and its nuclear norm.
-
-// e2n[e][s] = diag(lambda_e^2).
-__global__ void __launch_bounds__(256) k_lam2_diag(DevGraph g, const double* lam, cx<float>* e2n) {
-  const long long total = 2LL * g.n_edges * 256;
-  for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < total;
-       f += (long long)gridDim.x * blockDim.x) {
-    const int e = (int)(f >> 9), k = (int)((f >> 4) & 15), l = (int)(f & 15);
-    const double v = lam[(long long)e * 16 + k];
-    e2n[f] = mk<float>(k == l ? (float)(v * v) : 0.0f, 0.0f);
-  }
-}
+// diag(lambda^2) on every bond: the diagonal-message MFMA node kernel
+// (k_bp_nodes_lam2_d3c16) computes all of them, one per (edge, side).  One
+// wave then forms R - ideal and its nuclear norm.
 
 // Per (edge, side): |R/tr(R) - ideal|_* with ideal = diag(R_kk/tr > eps)/count,
 // R Hermitised (it is a Gram matrix up to rounding).  4 waves per block.
@@ -628,7 +618,7 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) out[inst] = s / (2.0 * (e1 - e0));
 }
 
-long long vidal_fast_ws_bytes(const DevGraph& g) { return 2LL * g.n_edges * (2 * 256 * 8 + 8) + 256; }
+long long vidal_fast_ws_bytes(const DevGraph& g) { return 2LL * g.n_edges * (256 * 8 + 8) + 256; }
 
 // Vidal distance for chi = 32 on the streaming node update: the lambda^2-
 // weighted bond Grams are node updates from diag(lambda^2) messages.
@@ -674,14 +664,12 @@ hipError_t launch_vidal_distance_fast(const DevGraph& g, const void* gam, const 
                                       int n_inst, double* inst_dist, void* ws, const int* inst_active,
                                       hipStream_t s) {
   if (g.n_edges <= 0) return hipSuccess;
-  cx<float>* msg = (cx<float>*)ws;
-  cx<float>* R = msg + 2LL * g.n_edges * 256;
+  // the lambda^2-weighted bond Grams R (node updates from diag(lambda^2)
+  // messages), then per (edge, side) the nuclear distance
+  cx<float>* R = (cx<float>*)ws;
   double* es_dist = (double*)(R + 2LL * g.n_edges * 256);
-  const long long total = 2LL * g.n_edges * 256;
-  hipLaunchKernelGGL(k_lam2_diag, dim3(grid_for(total, 256)), dim3(256), 0, s, g, lam, msg);
-  hipError_t err = hipGetLastError();
-  if (err != hipSuccess) return err;
-  err = launch_bp_nodes_fast(g, all_nodes, g.n_nodes, gam, msg, msg, R, zero_src, inst_active, s);
+  (void)zero_src;
+  hipError_t err = launch_bp_nodes_lam2_fast(g, all_nodes, g.n_nodes, gam, lam, R, inst_active, s);
   if (err != hipSuccess) return err;
   hipLaunchKernelGGL(k_vidal_nuclear16, dim3((unsigned)((2LL * g.n_edges + 3) / 4)), dim3(256), 0, s, g, R, eps,
                      es_dist, inst_active);

@@ -71,6 +71,9 @@ hipError_t launch_absorb_big(int real_bytes, const DevGraph& g, int max_deg, con
 hipError_t launch_bp_nodes_fast(const DevGraph& g, const int* nodes, int count, const void* T, const void* n2e_old,
                                 const void* e2n_old, void* n2e_new, const uint8_t* node_src, const int* inst_active,
                                 hipStream_t s);
+// The same node update with diag(lam_e^2) messages on every bond (Vidal distance).
+hipError_t launch_bp_nodes_lam2_fast(const DevGraph& g, const int* nodes, int count, const void* T,
+                                     const double* lam, void* n2e_new, const int* inst_active, hipStream_t s);
 
 // e2n_new[e][s] = (edge_src[2e+s] == 0 ? n2e_new : n2e_old)[e][1-s].
 hipError_t launch_bp_edges(int real_bytes, const DevGraph& g, const void* n2e_new, const void* n2e_old, void* e2n_new,
