@@ -153,7 +153,9 @@ template <typename R>
 __global__ void __launch_bounds__(128)
     k_edge_svd16(DevGraph g, const cx<R>* n2e, double eps, cd* X, double* lam, const int* inst_active, char* tape) {
   constexpr int n = 16, cc = 256;
-  __shared__ cd Mf[cc], Mb[cc], Qf[cc], Qb[cc], If[cc], Ib[cc], G[cc];
+  // six 16 x 16 matrices (G reuses Qf once the square roots are formed), so
+  // that six blocks fit a CU's LDS and 1536 edges run in one round
+  __shared__ cd Mf[cc], Mb[cc], Qf[cc], Qb[cc], If[cc], Ib[cc];
   __shared__ double wf[n], wb[n], sigma[n], scal[2 * n];
   __shared__ int perm[n];
   const int e = blockIdx.x;
@@ -180,6 +182,17 @@ __global__ void __launch_bounds__(128)
     wb[tid] = Mb[tid * (n + 1)].re;
   }
   __syncthreads();
+  if (tape) {
+    EdgeSvdTape tp(tape + (long long)e * edge_svd_tape_bytes(n), n);
+    for (int q = tid; q < cc; q += blockDim.x) {
+      tp.Qf()[q] = Qf[q];
+      tp.Qb()[q] = Qb[q];
+    }
+    for (int j = tid; j < n; j += blockDim.x) {
+      tp.wf()[j] = wf[j];
+      tp.wb()[j] = wb[j];
+    }
+  }
   // m^{1/2} and m^{-1/2} (safe) from the eigendecompositions
   cd* Sf = Mf;
   cd* Sb = Mb;
@@ -204,6 +217,7 @@ __global__ void __launch_bounds__(128)
   }
   __syncthreads();
   // G (column-major) = Sf Sb^T
+  cd* G = Qf;
   for (int q = tid; q < cc; q += blockDim.x) {
     const int c = q / n, r = q % n;
     cd acc = mk<double>(0, 0);
@@ -240,16 +254,10 @@ __global__ void __launch_bounds__(128)
     EdgeSvdTape tp(tape + (long long)e * edge_svd_tape_bytes(n), n);
     for (int q = tid; q < cc; q += blockDim.x) {
       const int i = q / n, j = q % n;
-      tp.Qf()[q] = Qf[q];
-      tp.Qb()[q] = Qb[q];
       tp.U()[q] = U[(long long)j * n + i];
       tp.V()[q] = V[(long long)perm[j] * n + i];
     }
-    for (int j = tid; j < n; j += blockDim.x) {
-      tp.wf()[j] = wf[j];
-      tp.wb()[j] = wb[j];
-      tp.sigma()[j] = sigma[j];
-    }
+    for (int j = tid; j < n; j += blockDim.x) tp.sigma()[j] = sigma[j];
     if (tid == 0) {
       tp.scal()[0] = nrm;
       tp.scal()[1] = nu;
