@@ -423,61 +423,86 @@ hipError_t launch_absorb_fast64(const DevGraph& g, const void* T, const cd* X, v
 // Vidal absorb (tensor_ops.py:88-93) on matrix cores for the same shape:
 // Gamma = T x_0 X_0^T x_1 X_1^T x_2 X_2^T / |.|, X_j the orthogonaliser of the
 // node's side of bond j (complex128, rounded to complex64 operands, FP32
-// accumulation like the generic complex64 kernel).  One 256-thread block per
-// node with a single padded tensor in LDS (68 KiB, two blocks per CU): the
-// three mode products run in place (each column tile reads and rewrites only
-// its own columns).
-__global__ void __launch_bounds__(256)
+// accumulation like the generic complex64 kernel).  256-thread blocks with a
+// single padded tensor in LDS (68 KiB, two blocks per CU), persistent over
+// the nodes with the next node's tensor and orthogonalisers loaded into
+// registers while the current node is processed: the three mode products run
+// in place (each column tile reads and rewrites only its own columns).
+__global__ void __launch_bounds__(256, 2)
     k_absorb_d3c16(DevGraph g, const float2* T, const cd* X, float2* gam, const int* inst_active) {
   using namespace fast3;
+  constexpr int PER = S / 256;
   __shared__ float2 tl[NPAD];
   __shared__ double red[8];
-  const int node = blockIdx.x;
-  if (inst_active && !inst_active[g.node_inst[node]]) return;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int base = g.adj_ptr[node];
-  const float2* tg = T + g.toff[node];
-  for (int f = tid; f < S; f += 256) {
-    const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
-    tl[b0 * B0P + b1 * RP + r] = tg[f];
-  }
+  auto active = [&](int node) { return !inst_active || inst_active[g.node_inst[node]]; };
   // m[j][ks] = X_j^T[lane%16][4 ks + lane/16] = X_j[4 ks + lane/16][lane%16]
-  float2 m[3][4];
+  auto load = [&](int node, float2 (&pre)[PER], float2 (&m)[3][4]) {
+    const float2* tg = T + g.toff[node];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const cd* x = X + (2LL * g.adj_edge[base + k] + g.adj_side[base + k]) * 256;
+    for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * 256];
+    const int base = g.adj_ptr[node];
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const cd v = x[(4 * ks + (lane >> 4)) * 16 + (lane & 15)];
-      m[k][ks] = make_float2((float)v.re, (float)v.im);
+    for (int k = 0; k < 3; ++k) {
+      const cd* x = X + (2LL * g.adj_edge[base + k] + g.adj_side[base + k]) * 256;
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const cd v = x[(4 * ks + (lane >> 4)) * 16 + (lane & 15)];
+        m[k][ks] = make_float2((float)v.re, (float)v.im);
+      }
     }
-  }
-  __syncthreads();
+  };
+  int node = blockIdx.x;
+  while (node < g.n_nodes && !active(node)) node += gridDim.x;
+  if (node >= g.n_nodes) return;
+  float2 pre[PER], mn[3][4];
+  load(node, pre, mn);
+  while (true) {
 #pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    mode_product<4>(tl, tl, m[j], j, wave, lane);
+    for (int k = 0; k < PER; ++k) {
+      const int f = tid + k * 256;
+      const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
+      tl[b0 * B0P + b1 * RP + r] = pre[k];
+    }
+    float2 m[3][4];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) m[k][ks] = mn[k][ks];
+    int next = node + gridDim.x;
+    while (next < g.n_nodes && !active(next)) next += gridDim.x;
+    if (next < g.n_nodes) load(next, pre, mn);
     __syncthreads();
-  }
-  double acc = 0.0;
-  for (int f = tid; f < S; f += 256) {
-    const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
-    const float2 v = tl[b0 * B0P + b1 * RP + r];
-    acc += (double)(v.x * v.x + v.y * v.y);
-  }
-  const float inv = (float)(1.0 / sqrt(block_sum(acc, red)));
-  float2* out = gam + g.toff[node];
-  for (int f = tid; f < S; f += 256) {
-    const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
-    const float2 v = tl[b0 * B0P + b1 * RP + r];
-    out[f] = make_float2(inv * v.x, inv * v.y);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      mode_product<4>(tl, tl, m[j], j, wave, lane);
+      __syncthreads();
+    }
+    double acc = 0.0;
+    for (int f = tid; f < S; f += 256) {
+      const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
+      const float2 v = tl[b0 * B0P + b1 * RP + r];
+      acc += (double)(v.x * v.x + v.y * v.y);
+    }
+    const float inv = (float)(1.0 / sqrt(block_sum(acc, red)));
+    float2* out = gam + g.toff[node];
+    for (int f = tid; f < S; f += 256) {
+      const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
+      const float2 v = tl[b0 * B0P + b1 * RP + r];
+      out[f] = make_float2(inv * v.x, inv * v.y);
+    }
+    if (next >= g.n_nodes) break;
+    __syncthreads();  // the next staging overwrites tl
+    node = next;
   }
 }
 
 hipError_t launch_absorb_fast(const DevGraph& g, const void* T, const cd* X, void* gam, const int* inst_active,
                               hipStream_t s) {
   if (g.n_nodes <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_absorb_d3c16, dim3(g.n_nodes), dim3(256), 0, s, g, (const float2*)T, X, (float2*)gam,
-                     inst_active);
+  const int slots = 2 * device_cu_count();
+  hipLaunchKernelGGL(k_absorb_d3c16, dim3(g.n_nodes < slots ? g.n_nodes : slots), dim3(256), 0, s, g,
+                     (const float2*)T, X, (float2*)gam, inst_active);
   return hipGetLastError();
 }
 
