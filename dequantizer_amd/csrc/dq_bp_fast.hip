@@ -152,20 +152,24 @@ __global__ void __launch_bounds__(64 * NW)
   // tensor is loaded into registers while the current one is processed, and
   // its three incoming messages while the current outputs are reduced, so
   // their HBM latency is hidden behind the matrix-core work.
-  auto active = [&](int it) { return !inst_active || inst_active[g.node_inst[nodes[it]]]; };
+  // graph metadata through scalar loads (ldc); the current node's adjacency
+  // base is carried over from the previous iteration
+  auto active = [&](int it) { return !inst_active || ldc(inst_active, ldc(g.node_inst, ldc(nodes, it))); };
   constexpr int QM = (768 + NT - 1) / NT;  // message elements per thread
-  auto fetch_msgs = [&](int nit, float2 (&r)[QM]) {
-    const int base = g.adj_ptr[nodes[nit]];
+  auto fetch_msgs = [&](int base, float2 (&r)[QM]) {
+    const float2* src[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {  // uniform indices: scalar loads
+      const int e = ldc(g.adj_edge, base + k), sd = ldc(g.adj_side, base + k);
+      const int code = node_src ? ldc(node_src, base + k) : 0;
+      src[k] = code == 0 ? e2n_old + (2LL * e + sd) * 256 : (code == 1 ? n2e_new : n2e_old) + (2LL * e + 1 - sd) * 256;
+    }
 #pragma unroll
     for (int qi = 0; qi < QM; ++qi) {
       const int q = tid + qi * NT;
       if (q < 768) {
         const int k = q >> 8;
-        const int e = g.adj_edge[base + k], sd = g.adj_side[base + k];
-        const int code = node_src ? node_src[base + k] : 0;
-        const float2* src = code == 0 ? e2n_old + (2LL * e + sd) * 256
-                                      : (code == 1 ? n2e_new : n2e_old) + (2LL * e + 1 - sd) * 256;
-        r[qi] = src[q & 255];
+        r[qi] = (k == 0 ? src[0] : (k == 1 ? src[1] : src[2]))[q & 255];
       }
     }
   };
@@ -180,17 +184,18 @@ __global__ void __launch_bounds__(64 * NW)
   while (it < count && !active(it)) it += gridDim.x;
   if (it >= count) return;
   float2 pre[PER];
+  int base;
   {
-    const float2* tg = T + g.toff[nodes[it]];
+    const int node = ldc(nodes, it);
+    base = ldc(g.adj_ptr, node);
+    const float2* tg = T + ldc(g.toff, node);
 #pragma unroll
     for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * NT];
     float2 mr[QM];
-    fetch_msgs(it, mr);
+    fetch_msgs(base, mr);
     store_msgs(mr);
   }
   while (true) {
-    const int node = nodes[it];
-    const int base = g.adj_ptr[node];
     // stage T into LDS (padded); the previous node's last reads of tl are
     // behind the barrier that precedes its partial writes
 #pragma unroll
@@ -202,13 +207,16 @@ __global__ void __launch_bounds__(64 * NW)
     int oe[3], os[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      oe[k] = g.adj_edge[base + k];
-      os[k] = g.adj_side[base + k];
+      oe[k] = ldc(g.adj_edge, base + k);
+      os[k] = ldc(g.adj_side, base + k);
     }
     int next = it + gridDim.x;
     while (next < count && !active(next)) next += gridDim.x;
+    int base_n = 0;
     if (next < count) {
-      const float2* tg = T + g.toff[nodes[next]];
+      const int node_n = ldc(nodes, next);
+      base_n = ldc(g.adj_ptr, node_n);
+      const float2* tg = T + ldc(g.toff, node_n);
 #pragma unroll
       for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * NT];
     }
@@ -231,7 +239,7 @@ __global__ void __launch_bounds__(64 * NW)
     __syncthreads();
     mode_gram<NW>(tl, wk, m[1], 1, 2, 0, wave, lane, gr[2], gi[2]);  // U_2 = X x_1 M1
     float2 mr[QM];
-    if (next < count) fetch_msgs(next, mr);
+    if (next < count) fetch_msgs(base_n, mr);
     __syncthreads();
     // wave partials of the three Grams into the work tensor, summed over the
     // waves, divided by their traces
@@ -283,6 +291,7 @@ __global__ void __launch_bounds__(64 * NW)
     if (next >= count) break;
     store_msgs(mr);
     it = next;
+    base = base_n;
   }
 }
 

@@ -90,6 +90,16 @@ __host__ __device__ inline long long ipow(long long b, int e) {
   return r;
 }
 
+// Read of graph metadata that no kernel writes while it runs (adjacency,
+// offsets, node lists, schedule codes) through the constant address space:
+// with a wave-uniform index it becomes a scalar load, whose completion is
+// tracked apart from the vector loads in flight (a chain of such reads does
+// not wait for a prefetched tensor).
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p, long long i) {
+  return ((const T __attribute__((address_space(4)))*)p)[i];
+}
+
 // ---------------------------------------------------------------- reductions
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
