@@ -411,7 +411,14 @@ __global__ void __launch_bounds__(256, 2)
   __shared__ double red[8];
   __shared__ double scal[8];
   __shared__ int kc_sh;
-  const SuGate G = gates[blockIdx.x];
+  SuGate G;
+  {
+    const int* gi = (const int*)(gates + blockIdx.x);
+    G.edge = ldc(gi, 0);
+    G.first = ldc(gi, 1);
+    G.rank = ldc(gi, 2);
+    G.pad = 0;
+  }
   if (G.rank < 1 || G.rank > max_r) return;
   const int e = G.edge, rg = G.rank, chi = g.chi, p = g.phys;
   const int n = chi * rg;

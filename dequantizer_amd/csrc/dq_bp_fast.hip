@@ -312,19 +312,19 @@ __global__ void __launch_bounds__(64 * fast64::NW)
   __shared__ cd tl[NPAD];
   __shared__ cd wk[NPAD];
   __shared__ cd diag[16];
-  const int node = nodes[blockIdx.x];
-  if (inst_active && !inst_active[g.node_inst[node]]) return;
+  const int node = ldc(nodes, blockIdx.x);
+  if (inst_active && !ldc(inst_active, ldc(g.node_inst, node))) return;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int base = g.adj_ptr[node];
-  const cd* tg = T + g.toff[node];
+  const int base = ldc(g.adj_ptr, node);
+  const cd* tg = T + ldc(g.toff, node);
   cd m[3][4];
   int oe[3], os[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const int e = g.adj_edge[base + k], sd = g.adj_side[base + k];
+    const int e = ldc(g.adj_edge, base + k), sd = ldc(g.adj_side, base + k);
     oe[k] = e;
     os[k] = sd;
-    const int code = node_src ? node_src[base + k] : 0;
+    const int code = node_src ? ldc(node_src, base + k) : 0;
     const cd* src = code == 0 ? e2n_old + (2LL * e + sd) * 256
                               : (code == 1 ? n2e_new : n2e_old) + (2LL * e + 1 - sd) * 256;
 #pragma unroll
@@ -392,15 +392,15 @@ __global__ void __launch_bounds__(64 * fast64::NW)
   __shared__ cd tl[NPAD];
   __shared__ double red[NW];
   const int node = blockIdx.x;
-  if (inst_active && !inst_active[g.node_inst[node]]) return;
+  if (inst_active && !ldc(inst_active, ldc(g.node_inst, node))) return;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int base = g.adj_ptr[node];
-  const cd* tg = T + g.toff[node];
-  cd* out = gam + g.toff[node];
+  const int base = ldc(g.adj_ptr, node);
+  const cd* tg = T + ldc(g.toff, node);
+  cd* out = gam + ldc(g.toff, node);
   cd m[3][4];
 #pragma unroll
   for (int k = 0; k < 3; ++k)
-    load_operand(X + (2LL * g.adj_edge[base + k] + g.adj_side[base + k]) * 256, 2, lane, m[k]);
+    load_operand(X + (2LL * ldc(g.adj_edge, base + k) + ldc(g.adj_side, base + k)) * 256, 2, lane, m[k]);
   double nn = 0.0;
   for (int s = 0; s < 2; ++s) {
     stage_component(tl, tg, s);
@@ -444,16 +444,16 @@ __global__ void __launch_bounds__(256, 2)
   __shared__ float2 tl[NPAD];
   __shared__ double red[8];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  auto active = [&](int node) { return !inst_active || inst_active[g.node_inst[node]]; };
+  auto active = [&](int node) { return !inst_active || ldc(inst_active, ldc(g.node_inst, node)); };
   // m[j][ks] = X_j^T[lane%16][4 ks + lane/16] = X_j[4 ks + lane/16][lane%16]
   auto load = [&](int node, float2 (&pre)[PER], float2 (&m)[3][4]) {
-    const float2* tg = T + g.toff[node];
+    const float2* tg = T + ldc(g.toff, node);
 #pragma unroll
     for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * 256];
-    const int base = g.adj_ptr[node];
+    const int base = ldc(g.adj_ptr, node);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const cd* x = X + (2LL * g.adj_edge[base + k] + g.adj_side[base + k]) * 256;
+      const cd* x = X + (2LL * ldc(g.adj_edge, base + k) + ldc(g.adj_side, base + k)) * 256;
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
         const cd v = x[(4 * ks + (lane >> 4)) * 16 + (lane & 15)];
@@ -494,7 +494,7 @@ __global__ void __launch_bounds__(256, 2)
       acc += (double)(v.x * v.x + v.y * v.y);
     }
     const float inv = (float)(1.0 / sqrt(block_sum(acc, red)));
-    float2* out = gam + g.toff[node];
+    float2* out = gam + ldc(g.toff, node);
     for (int f = tid; f < S; f += 256) {
       const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
       const float2 v = tl[b0 * B0P + b1 * RP + r];
@@ -537,19 +537,19 @@ __global__ void __launch_bounds__(512)
   __shared__ float wl[3][16];
   __shared__ float2 diag[3 * 16];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  auto active = [&](int it) { return !inst_active || inst_active[g.node_inst[nodes[it]]]; };
+  auto active = [&](int it) { return !inst_active || ldc(inst_active, ldc(g.node_inst, ldc(nodes, it))); };
   int it = blockIdx.x;
   while (it < count && !active(it)) it += gridDim.x;
   if (it >= count) return;
   float2 pre[PER];
   {
-    const float2* tg = T + g.toff[nodes[it]];
+    const float2* tg = T + ldc(g.toff, ldc(nodes, it));
 #pragma unroll
     for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * NT];
   }
   while (true) {
-    const int node = nodes[it];
-    const int base = g.adj_ptr[node];
+    const int node = ldc(nodes, it);
+    const int base = ldc(g.adj_ptr, node);
     // tl and wl were last read before the barrier that precedes the partial
     // writes of the previous node
 #pragma unroll
@@ -558,20 +558,21 @@ __global__ void __launch_bounds__(512)
       const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
       tl[b0 * B0P + b1 * RP + r] = pre[k];
     }
-    if (tid < 48) {
-      const double v = lam[(long long)g.adj_edge[base + (tid >> 4)] * 16 + (tid & 15)];
-      wl[tid >> 4][tid & 15] = (float)(v * v);
-    }
     int oe[3], os[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      oe[k] = g.adj_edge[base + k];
-      os[k] = g.adj_side[base + k];
+      oe[k] = ldc(g.adj_edge, base + k);
+      os[k] = ldc(g.adj_side, base + k);
+    }
+    if (tid < 48) {
+      const int k = tid >> 4;
+      const double v = lam[(long long)(k == 0 ? oe[0] : (k == 1 ? oe[1] : oe[2])) * 16 + (tid & 15)];
+      wl[k][tid & 15] = (float)(v * v);
     }
     int next = it + gridDim.x;
     while (next < count && !active(next)) next += gridDim.x;
     if (next < count) {
-      const float2* tg = T + g.toff[nodes[next]];
+      const float2* tg = T + ldc(g.toff, ldc(nodes, next));
 #pragma unroll
       for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * NT];
     }

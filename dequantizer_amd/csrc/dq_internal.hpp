@@ -150,9 +150,9 @@ __device__ inline double block_sum(double v, double* red) {
 
 // Adjacency slot of edge e (node on side `side`) in `node`'s bond order.
 __device__ inline int find_slot(const DevGraph& g, int node, int e, int side) {
-  const int b = g.adj_ptr[node], d = g.deg[node];
+  const int b = ldc(g.adj_ptr, node), d = ldc(g.deg, node);
   for (int k = 0; k < d; ++k)
-    if (g.adj_edge[b + k] == e && g.adj_side[b + k] == side) return k;
+    if (ldc(g.adj_edge, b + k) == e && ldc(g.adj_side, b + k) == side) return k;
   return -1;
 }
 

@@ -530,7 +530,14 @@ __global__ void __launch_bounds__(256, kMem == kSuCompact ? 2 : 1)
   __shared__ int flag[2];
   __shared__ int perm[128];
   __shared__ int sh_rank;
-  const SuGate G = gates[blockIdx.x];
+  SuGate G;
+  {
+    const int* gi = (const int*)(gates + blockIdx.x);
+    G.edge = ldc(gi, 0);
+    G.first = ldc(gi, 1);
+    G.rank = ldc(gi, 2);
+    G.pad = 0;
+  }
   if (G.rank < 1 || G.rank > max_r) {
     // rank outside what this launch's buffers were sized for: the gate is
     // skipped and flagged with NaN error and entropy (uniform per block)
@@ -542,7 +549,7 @@ __global__ void __launch_bounds__(256, kMem == kSuCompact ? 2 : 1)
   }
   const int e = G.edge, r = G.rank, chi = g.chi, p = g.phys;
   const int n = chi * r, cp = chi * p;
-  const int na = g.edge_node[2 * e + G.first], nb = g.edge_node[2 * e + 1 - G.first];
+  const int na = ldc(g.edge_node, 2 * e + G.first), nb = ldc(g.edge_node, 2 * e + 1 - G.first);
   const int ia = find_slot(g, na, e, G.first), ib = find_slot(g, nb, e, 1 - G.first);
   const long long hstride = (long long)p * p * p * p;
   const cd* h1 = halves + (long long)blockIdx.x * 2 * hstride;
