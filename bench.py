@@ -177,7 +177,7 @@ def main():
         "setup_and_warmup_s": t_build,
         "roofline": {
             "bound": "mfma",
-            "kernel": "BP sweep (k_bp_nodes + k_bp_edges)",
+            "kernel": "BP sweep (k_bp_nodes_d3c16, the synchronous edge update fused in)",
             "achieved": achieved_tflops,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",

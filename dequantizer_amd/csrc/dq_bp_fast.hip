@@ -138,7 +138,8 @@ __device__ __forceinline__ void gram_partial(const float2* t, const float2* u, i
 template <int NW>
 __global__ void __launch_bounds__(64 * NW)
     k_bp_nodes_d3c16(DevGraph g, const int* nodes, int count, const float2* T, const float2* n2e_old,
-                     const float2* e2n_old, float2* n2e_new, const uint8_t* node_src, const int* inst_active) {
+                     const float2* e2n_old, float2* n2e_new, const uint8_t* node_src, const int* inst_active,
+                     float2* e2n_copy) {
   using namespace fast3;
   constexpr int NT = 64 * NW;   // threads
   constexpr int PER = S / NT;   // tensor elements staged per thread
@@ -221,6 +222,21 @@ __global__ void __launch_bounds__(64 * NW)
       for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * NT];
     }
     __syncthreads();
+    // synchronous edge update fused in (e2n_copy != NULL): this node's three
+    // edge->node messages of the next sweep are the opposite sides' inputs
+    // of this one, e2n_copy[e][s] = n2e_old[e][1 - s]
+    float2 cp[QM];
+    if (e2n_copy) {
+#pragma unroll
+      for (int qi = 0; qi < QM; ++qi) {
+        const int q = tid + qi * NT;
+        if (q < 768) {
+          const int k = q >> 8;
+          const int e = k == 0 ? oe[0] : (k == 1 ? oe[1] : oe[2]), sd = k == 0 ? os[0] : (k == 1 ? os[1] : os[2]);
+          cp[qi] = n2e_old[(2LL * e + 1 - sd) * 256 + (q & 255)];
+        }
+      }
+    }
     // message operands: m[j][ks] = M_j[lane%16][4 ks + lane/16]
     float2 m[3][4];
 #pragma unroll
@@ -282,6 +298,17 @@ __global__ void __launch_bounds__(64 * NW)
         o.x = (sum[qi].x * tr_r + sum[qi].y * tr_i) / den;
         o.y = (sum[qi].y * tr_r - sum[qi].x * tr_i) / den;
         n2e_new[(2LL * oe[k] + os[k]) * 256 + e] = o;
+      }
+    }
+    if (e2n_copy) {
+#pragma unroll
+      for (int qi = 0; qi < QM; ++qi) {
+        const int q = tid + qi * NT;
+        if (q < 768) {
+          const int k = q >> 8;
+          const int e = k == 0 ? oe[0] : (k == 1 ? oe[1] : oe[2]), sd = k == 0 ? os[0] : (k == 1 ? os[1] : os[2]);
+          e2n_copy[(2LL * e + sd) * 256 + (q & 255)] = cp[qi];
+        }
       }
     }
     // the next node's staging writes only tl, last read before the barrier
@@ -652,7 +679,7 @@ hipError_t launch_bp_nodes_lam2_fast(const DevGraph& g, const int* nodes, int co
 
 hipError_t launch_bp_nodes_fast(const DevGraph& g, const int* nodes, int count, const void* T, const void* n2e_old,
                                 const void* e2n_old, void* n2e_new, const uint8_t* node_src, const int* inst_active,
-                                hipStream_t s) {
+                                hipStream_t s, void* e2n_copy) {
   if (count <= 0) return hipSuccess;
   // variant (autotuner): bit 0 = one block per node instead of persistent
   // blocks, bit 1 = 4 waves per block instead of 8
@@ -662,10 +689,12 @@ hipError_t launch_bp_nodes_fast(const DevGraph& g, const int* nodes, int count, 
   const int grid = (v & 1) ? count : (count < cus ? count : cus);
   if (v & 2)
     hipLaunchKernelGGL(k_bp_nodes_d3c16<4>, dim3(grid), dim3(256), 0, s, g, nodes, count, (const float2*)T,
-                       (const float2*)n2e_old, (const float2*)e2n_old, (float2*)n2e_new, node_src, inst_active);
+                       (const float2*)n2e_old, (const float2*)e2n_old, (float2*)n2e_new, node_src, inst_active,
+                       (float2*)e2n_copy);
   else
     hipLaunchKernelGGL(k_bp_nodes_d3c16<8>, dim3(grid), dim3(512), 0, s, g, nodes, count, (const float2*)T,
-                       (const float2*)n2e_old, (const float2*)e2n_old, (float2*)n2e_new, node_src, inst_active);
+                       (const float2*)n2e_old, (const float2*)e2n_old, (float2*)n2e_new, node_src, inst_active,
+                       (float2*)e2n_copy);
   return hipGetLastError();
 }
 

@@ -53,6 +53,9 @@ struct dq_graph {
   int* d_level_nodes = nullptr;
   uint8_t *d_node_src = nullptr, *d_edge_src = nullptr;
   int max_level = 0;
+  // one level, every node reading the incoming e2n and every edge passing
+  // the previous n2e (the plain synchronous sweep)
+  bool plain_sync = false;
   // every node once with all sources = incoming e2n (one synchronous level)
   int* d_all_nodes = nullptr;
   uint8_t* d_zero_src = nullptr;
@@ -276,6 +279,9 @@ int dq_bp_set_schedule(dq_graph* g, int n_levels, const int* level_ptr, const in
   g->n_levels = n_levels;
   g->level_ptr.assign(level_ptr, level_ptr + n_levels + 1);
   g->max_level = max_level;
+  bool plain = n_levels == 1;
+  for (int i = 0; i < nadj && plain; ++i) plain = node_src[i] == 0 && edge_src[i] == 1;
+  g->plain_sync = plain;
   return DQ_OK;
 }
 
@@ -302,6 +308,10 @@ static hipError_t bp_sweep_launch(dq_graph* g, int rb, const void* T, const void
                                   void* n2e_out, void* e2n_out, const int* inst_active, hipStream_t s) {
   hipError_t e = hipSuccess;
   const bool big = big_applicable(g->dev, g->max_deg);
+  if (!big && g->plain_sync && bp_fast_applicable(rb, g->dev, g->min_deg, g->max_deg))
+    // the matrix-core node kernel also does the synchronous edge update
+    return launch_bp_nodes_fast(g->dev, g->d_level_nodes, g->level_ptr[1], T, n2e_in, e2n_in, n2e_out,
+                                g->d_node_src, inst_active, s, e2n_out);
   const long long wsb = (node_ws_bytes(g, rb) + 255) / 256 * 256;
   for (int l = 0; l < g->n_levels && e == hipSuccess; ++l) {
     const int b = g->level_ptr[l], cnt = g->level_ptr[l + 1] - b;

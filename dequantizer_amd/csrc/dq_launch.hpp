@@ -68,9 +68,11 @@ hipError_t launch_bp_nodes_big(int real_bytes, const DevGraph& g, int max_deg, c
 hipError_t launch_absorb_big(int real_bytes, const DevGraph& g, int max_deg, const int* nodes, int count,
                              const void* T, const cd* X, void* gammas, void* ws, const int* inst_active,
                              hipStream_t s);
+// e2n_copy != NULL (plain synchronous schedule only): the edge update
+// e2n_copy[e][s] = n2e_old[e][1 - s] is done by the node kernel as well.
 hipError_t launch_bp_nodes_fast(const DevGraph& g, const int* nodes, int count, const void* T, const void* n2e_old,
                                 const void* e2n_old, void* n2e_new, const uint8_t* node_src, const int* inst_active,
-                                hipStream_t s);
+                                hipStream_t s, void* e2n_copy = nullptr);
 // The same node update with diag(lam_e^2) messages on every bond (Vidal distance).
 hipError_t launch_bp_nodes_lam2_fast(const DevGraph& g, const int* nodes, int count, const void* T,
                                      const double* lam, void* n2e_new, const int* inst_active, hipStream_t s);
