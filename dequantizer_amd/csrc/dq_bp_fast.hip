@@ -153,18 +153,66 @@ __global__ void __launch_bounds__(64 * NW)
   // tensor is loaded into registers while the current one is processed, and
   // its three incoming messages while the current outputs are reduced, so
   // their HBM latency is hidden behind the matrix-core work.
-  // graph metadata through scalar loads (ldc); the current node's adjacency
-  // base is carried over from the previous iteration
-  auto active = [&](int it) { return !inst_active || ldc(inst_active, ldc(g.node_inst, ldc(nodes, it))); };
+  // Graph metadata of this block's first MT nodes (node j is list entry
+  // blockIdx.x + j * gridDim.x) is gathered once into LDS at the start, so
+  // that the per-node chain of dependent metadata reads (list entry, node
+  // offset, adjacency) leaves the critical path; later nodes read it through
+  // scalar loads.
+  constexpr int MT = 64;
+  __shared__ long long mt_toff[MT];
+  __shared__ int mt_valid[MT], mt_e[MT][3], mt_s[MT][3], mt_code[MT][3];
+  const int nmine = count > (int)blockIdx.x ? (count - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x : 0;
+  if (tid < MT) {
+    const int j = tid;
+    int valid = 0;
+    if (j < nmine) {
+      const int node = nodes[blockIdx.x + (long long)j * gridDim.x];
+      valid = !inst_active || inst_active[g.node_inst[node]];
+      const int base = g.adj_ptr[node];
+      mt_toff[j] = g.toff[node];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        mt_e[j][k] = g.adj_edge[base + k];
+        mt_s[j][k] = g.adj_side[base + k];
+        mt_code[j][k] = node_src ? node_src[base + k] : 0;
+      }
+    }
+    mt_valid[j] = valid;
+  }
+  __syncthreads();
+  auto node_of = [&](int j) { return ldc(nodes, blockIdx.x + (long long)j * gridDim.x); };
+  auto valid_of = [&](int j) {
+    if (j < MT) return mt_valid[j] != 0;
+    return !inst_active || ldc(inst_active, ldc(g.node_inst, node_of(j))) != 0;
+  };
+  auto toff_of = [&](int j) { return j < MT ? mt_toff[j] : ldc(g.toff, node_of(j)); };
+  auto adj_of = [&](int j, int (&e)[3], int (&sd)[3], int (&code)[3]) {
+    if (j < MT) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        e[k] = mt_e[j][k];
+        sd[k] = mt_s[j][k];
+        code[k] = mt_code[j][k];
+      }
+    } else {
+      const int base = ldc(g.adj_ptr, node_of(j));
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        e[k] = ldc(g.adj_edge, base + k);
+        sd[k] = ldc(g.adj_side, base + k);
+        code[k] = node_src ? ldc(node_src, base + k) : 0;
+      }
+    }
+  };
   constexpr int QM = (768 + NT - 1) / NT;  // message elements per thread
-  auto fetch_msgs = [&](int base, float2 (&r)[QM]) {
+  auto fetch_msgs = [&](int j, float2 (&r)[QM]) {
+    int e[3], sd[3], code[3];
+    adj_of(j, e, sd, code);
     const float2* src[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {  // uniform indices: scalar loads
-      const int e = ldc(g.adj_edge, base + k), sd = ldc(g.adj_side, base + k);
-      const int code = node_src ? ldc(node_src, base + k) : 0;
-      src[k] = code == 0 ? e2n_old + (2LL * e + sd) * 256 : (code == 1 ? n2e_new : n2e_old) + (2LL * e + 1 - sd) * 256;
-    }
+    for (int k = 0; k < 3; ++k)
+      src[k] = code[k] == 0 ? e2n_old + (2LL * e[k] + sd[k]) * 256
+                            : (code[k] == 1 ? n2e_new : n2e_old) + (2LL * e[k] + 1 - sd[k]) * 256;
 #pragma unroll
     for (int qi = 0; qi < QM; ++qi) {
       const int q = tid + qi * NT;
@@ -181,19 +229,16 @@ __global__ void __launch_bounds__(64 * NW)
       if (q < 768) mb[(q >> 8) * 272 + ((q >> 4) & 15) * 17 + (q & 15)] = r[qi];
     }
   };
-  int it = blockIdx.x;
-  while (it < count && !active(it)) it += gridDim.x;
-  if (it >= count) return;
+  int j = 0;
+  while (j < nmine && !valid_of(j)) ++j;
+  if (j >= nmine) return;
   float2 pre[PER];
-  int base;
   {
-    const int node = ldc(nodes, it);
-    base = ldc(g.adj_ptr, node);
-    const float2* tg = T + ldc(g.toff, node);
+    const float2* tg = T + toff_of(j);
 #pragma unroll
     for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * NT];
     float2 mr[QM];
-    fetch_msgs(base, mr);
+    fetch_msgs(j, mr);
     store_msgs(mr);
   }
   while (true) {
@@ -206,18 +251,14 @@ __global__ void __launch_bounds__(64 * NW)
       tl[b0 * B0P + b1 * RP + r] = pre[k];
     }
     int oe[3], os[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      oe[k] = ldc(g.adj_edge, base + k);
-      os[k] = ldc(g.adj_side, base + k);
+    {
+      int code[3];
+      adj_of(j, oe, os, code);
     }
-    int next = it + gridDim.x;
-    while (next < count && !active(next)) next += gridDim.x;
-    int base_n = 0;
-    if (next < count) {
-      const int node_n = ldc(nodes, next);
-      base_n = ldc(g.adj_ptr, node_n);
-      const float2* tg = T + ldc(g.toff, node_n);
+    int next = j + 1;
+    while (next < nmine && !valid_of(next)) ++next;
+    if (next < nmine) {
+      const float2* tg = T + toff_of(next);
 #pragma unroll
       for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * NT];
     }
@@ -255,7 +296,7 @@ __global__ void __launch_bounds__(64 * NW)
     __syncthreads();
     mode_gram<NW>(tl, wk, m[1], 1, 2, 0, wave, lane, gr[2], gi[2]);  // U_2 = X x_1 M1
     float2 mr[QM];
-    if (next < count) fetch_msgs(base_n, mr);
+    if (next < nmine) fetch_msgs(next, mr);
     __syncthreads();
     // wave partials of the three Grams into the work tensor, summed over the
     // waves, divided by their traces
@@ -315,10 +356,9 @@ __global__ void __launch_bounds__(64 * NW)
     // that precedes the partial writes; its first write of wk follows the
     // staging barrier, which every thread reaches after its reads of part;
     // mb was last read right after the previous staging barrier
-    if (next >= count) break;
+    if (next >= nmine) break;
     store_msgs(mr);
-    it = next;
-    base = base_n;
+    j = next;
   }
 }
 
