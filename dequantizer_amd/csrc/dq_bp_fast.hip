@@ -232,11 +232,12 @@ __global__ void __launch_bounds__(64 * NW)
   int j = 0;
   while (j < nmine && !valid_of(j)) ++j;
   if (j >= nmine) return;
-  float2 pre[PER];
+  // the node tensor moves as 16-byte pairs (elements 2i, 2i+1: same b0, b1)
+  float4 pre[PER / 2];
   {
-    const float2* tg = T + toff_of(j);
+    const float4* tg = (const float4*)(T + toff_of(j));
 #pragma unroll
-    for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * NT];
+    for (int k = 0; k < PER / 2; ++k) pre[k] = tg[tid + k * NT];
     float2 mr[QM];
     fetch_msgs(j, mr);
     store_msgs(mr);
@@ -245,10 +246,12 @@ __global__ void __launch_bounds__(64 * NW)
     // stage T into LDS (padded); the previous node's last reads of tl are
     // behind the barrier that precedes its partial writes
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int f = tid + k * NT;
+    for (int k = 0; k < PER / 2; ++k) {
+      const int f = 2 * (tid + k * NT);
       const int r = f & 31, b1 = (f >> 5) & 15, b0 = f >> 9;
-      tl[b0 * B0P + b1 * RP + r] = pre[k];
+      float2* d = tl + b0 * B0P + b1 * RP + r;
+      d[0] = make_float2(pre[k].x, pre[k].y);
+      d[1] = make_float2(pre[k].z, pre[k].w);
     }
     int oe[3], os[3];
     {
@@ -258,9 +261,9 @@ __global__ void __launch_bounds__(64 * NW)
     int next = j + 1;
     while (next < nmine && !valid_of(next)) ++next;
     if (next < nmine) {
-      const float2* tg = T + toff_of(next);
+      const float4* tg = (const float4*)(T + toff_of(next));
 #pragma unroll
-      for (int k = 0; k < PER; ++k) pre[k] = tg[tid + k * NT];
+      for (int k = 0; k < PER / 2; ++k) pre[k] = tg[tid + k * NT];
     }
     __syncthreads();
     // synchronous edge update fused in (e2n_copy != NULL): this node's three
