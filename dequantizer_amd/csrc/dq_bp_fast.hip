@@ -260,12 +260,14 @@ __global__ void __launch_bounds__(64 * NW)
     }
     int next = j + 1;
     while (next < nmine && !valid_of(next)) ++next;
-    if (next < nmine) {
-      const float4* tg = (const float4*)(T + toff_of(next));
-#pragma unroll
-      for (int k = 0; k < PER / 2; ++k) pre[k] = tg[tid + k * NT];
-    }
+    const float4* tg_n = next < nmine ? (const float4*)(T + toff_of(next)) : nullptr;
     __syncthreads();
+    // the next node's tensor is requested after the staging barrier, so that
+    // the issue of its loads overlaps the other wave's matrix-core work
+    if (tg_n) {
+#pragma unroll
+      for (int k = 0; k < PER / 2; ++k) pre[k] = tg_n[tid + k * NT];
+    }
     // synchronous edge update fused in (e2n_copy != NULL): this node's three
     // edge->node messages of the next sweep are the opposite sides' inputs
     // of this one, e2n_copy[e][s] = n2e_old[e][1 - s]
