@@ -23,11 +23,12 @@ import numpy as np
 METRIC = "edge-updates/s & SU gates/s at χ=16 vs plain GPU and CPU; fwd+adjoint time"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-# HBM bytes of one BP sweep (node kernel + edge kernel) of the default
-# workload (N=1000, chi=16, complex64), from FETCH_SIZE / WRITE_SIZE counter
-# passes scaled by a 256 MiB copy (FETCH_SIZE reads half its bytes, WRITE_SIZE
-# all): 2 x (35368.6 + 3055.5) KiB read + (6000 + 6000) KiB written.
-TRAFFIC_BP_SWEEP_DEFAULT = 2 * (35368.6 + 3055.5) * 1024 + (6000.0 + 6000.0) * 1024
+# HBM bytes of one BP sweep (one node-kernel launch with the synchronous edge
+# update fused in) of the default workload (N=1000, chi=16, complex64), from
+# FETCH_SIZE / WRITE_SIZE counter passes scaled by a 256 MiB copy (FETCH_SIZE
+# reads half its bytes, WRITE_SIZE all): 2 x 38304.5 KiB read + 12000 KiB
+# written.
+TRAFFIC_BP_SWEEP_DEFAULT = 2 * 38304.5 * 1024 + 12000.0 * 1024
 
 
 def eu_flops(d: int, chi: int, p: int) -> int:
