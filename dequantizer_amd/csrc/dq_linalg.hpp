@@ -31,6 +31,21 @@ __device__ __forceinline__ void rr_pair(int r, int k, int n2, int& p, int& q) {
   }
 }
 
+// 1/sqrt(x) and 1/x for normal positive x: the hardware estimates
+// (V_RSQ_F64, V_RCP_F64) refined by one Newton step, y <- y (3 - x y^2) / 2
+// and y <- y (2 - x y), which takes them to about an ulp.  The rotation
+// parameters below only need that accuracy (c^2 + s^2 = 1 to rounding), and
+// the estimates avoid the long dependent sequences of correctly rounded
+// division and square root on the critical path of every Jacobi round.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  return y * fma(-0.5 * x * y, y, 1.5);
+}
+__device__ __forceinline__ double rcp_nr(double x) {
+  const double y = __builtin_amdgcn_rcp(x);
+  return y * fma(-x, y, 2.0);
+}
+
 // Jacobi rotation for the Hermitian 2x2 block [[app, apq], [conj(apq), aqq]]:
 // J = [[c, s], [-s e, c e]] with e = exp(-i arg apq) makes J^H B J diagonal.
 // Returns false when the block is already diagonal to working precision.
@@ -41,14 +56,23 @@ __device__ __forceinline__ bool jacobi_params(double app, double aqq, cd apq, do
   // orthogonality FP64 can reach for n <= 64 (~n eps), so converged pairs
   // stop rotating; convergence is quadratic past it
   if (!(n2 > 1e-26 * fabs(app * aqq)) || !(n2 > abs_tol * abs_tol)) return false;
-  const double mag = sqrt(n2);
-  if (mag < 1e-300) return false;
-  const double inv = 1.0 / mag;
+  if (!(n2 > 1e-300)) return false;  // |apq| below 1e-150: nothing to rotate
+  const double inv = rsqrt_nr(n2);  // 1 / |apq|
   er = apq.re * inv;
   ei = -apq.im * inv;
   const double tau = 0.5 * (aqq - app) * inv;
-  const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-  c = 1.0 / sqrt(1.0 + t * t);
+  const double at = fabs(tau);
+  // t = sign(tau) / (|tau| + sqrt(1 + tau^2)), the smaller root of
+  // t^2 + 2 tau t - 1 = 0; for |tau| > 1e150 (tau^2 overflows) t = 1/(2 tau)
+  double t;
+  if (at > 1e150) {
+    t = 0.5 * rcp_nr(at);
+  } else {
+    const double q = 1.0 + tau * tau;
+    t = rcp_nr(at + q * rsqrt_nr(q));
+  }
+  if (tau < 0.0) t = -t;
+  c = rsqrt_nr(1.0 + t * t);
   s = t * c;
   return true;
 }
