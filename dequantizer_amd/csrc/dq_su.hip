@@ -608,11 +608,36 @@ __global__ void __launch_bounds__(256, kMem == kSuCompact ? 2 : 1)
   cd* RbT = E;
   for (int f = threadIdx.x; f < n * n; f += blockDim.x) RbT[(f % n) * n + f / n] = Rb[f];
   __syncthreads();
-  for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
-    const int i = f / n, j = f % n;
-    cd acc = mk<double>(0, 0);
-    for (int c = 0; c < n; ++c) fma_acc(acc, le[c / r] * Ra[i * n + c], RbT[c * n + j]);
-    C[j * n + i] = acc;
+  if (n == 32 && blockDim.x == 256) {
+    // on FP64 matrix cores: wave w computes the 16 x 16 tile (w / 2, w % 2)
+    // of C = (R_a diag(lambda_rep)) R_b^T, 8 k-steps of 4 real MFMAs each,
+    // all operand loads of the tile issued before the products
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int i0 = (w >> 1) * 16, j0 = (w & 1) * 16, rr = l & 15, kq = l >> 4;
+    cd a[8], b[8];
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      const int c = 4 * ks + kq;
+      a[ks] = le[c / r] * Ra[(i0 + rr) * n + c];
+      b[ks] = RbT[c * n + j0 + rr];
+    }
+    su_d4 cr = {0.0, 0.0, 0.0, 0.0}, ci = cr;
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      cr = su_mfma(a[ks].re, b[ks].re, cr);
+      cr = su_mfma(-a[ks].im, b[ks].im, cr);
+      ci = su_mfma(a[ks].re, b[ks].im, ci);
+      ci = su_mfma(a[ks].im, b[ks].re, ci);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) C[(j0 + rr) * n + i0 + kq + 4 * v] = mk<double>(cr[v], ci[v]);
+  } else {
+    for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
+      const int i = f / n, j = f % n;
+      cd acc = mk<double>(0, 0);
+      for (int c = 0; c < n; ++c) fma_acc(acc, le[c / r] * Ra[i * n + c], RbT[c * n + j]);
+      C[j * n + i] = acc;
+    }
   }
   __syncthreads();
   if (n == 32 && blockDim.x == 256) {
