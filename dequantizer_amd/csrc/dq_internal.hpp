@@ -118,6 +118,11 @@ __device__ __forceinline__ double dpp_move(double v) {
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+template <int kCtrl>
+__device__ __forceinline__ int dpp_move_int(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xF, 0xF, false);
+}
+
 // Sum over aligned groups of L lanes (L = 1, 2, 4, 8, 16) by DPP moves inside
 // a row: quad_perm(1,0,3,2) pairs lanes i, i^1; quad_perm(2,3,0,1) pairs
 // i, i^2; row_half_mirror pairs i, 7-i within 8 lanes; row_mirror pairs

@@ -705,7 +705,81 @@ __device__ inline void permute_cols(const cd* V, cd* out, int n, const int* perm
 // R = L^H P^T.  Pivots below n * eps_64 * max diag end the factorisation
 // (the remaining rows of R are zero).  R: n x n row-major.  piv: n ints;
 // scal: 2 doubles of scratch.
+__device__ inline void pivoted_cholesky_swap(cd* G, int n, cd* R, int* piv, double* scal);
+
+// The same factorisation for n <= 64 without moving rows and columns: the
+// chosen pivots are tracked in a bit mask (identical in every thread) and
+// every wave finds the next pivot itself, so a step costs two block barriers
+// (after the column scaling, after the trailing update) instead of up to
+// five.  Per element the same operations in the same order as the swapping
+// version; ties between equal pivots go to the smaller original index.
 __device__ inline void pivoted_cholesky(cd* G, int n, cd* R, int* piv, double* scal) {
+  if (n > 64) {
+    pivoted_cholesky_swap(G, n, R, piv, scal);
+    return;
+  }
+  const int lane = threadIdx.x & 63;
+  for (int i = threadIdx.x; i < n * n; i += blockDim.x) R[i] = mk<double>(0.0, 0.0);
+  double mx = 0.0;
+  for (int i = 0; i < n; ++i) mx = fmax(mx, G[i * n + i].re);
+  const double tol = n * 2.3e-16 * mx;
+  unsigned long long done = 0;
+  int k = 0;
+  auto take = [](double& best, int& j, double ob, int oj) {
+    if (ob > best || (ob == best && oj < j)) {
+      best = ob;
+      j = oj;
+    }
+  };
+  for (; k < n; ++k) {
+    double best = -1.0;
+    int j = n;
+    if (lane < n && !((done >> lane) & 1ull)) {
+      best = G[lane * n + lane].re;
+      j = lane;
+    }
+    // argmax over the wave: within each 16-lane row by DPP moves (every lane
+    // of the row ends with the row's result), then across rows
+    take(best, j, dpp_move<0xB1>(best), dpp_move_int<0xB1>(j));
+    take(best, j, dpp_move<0x4E>(best), dpp_move_int<0x4E>(j));
+    take(best, j, dpp_move<0x141>(best), dpp_move_int<0x141>(j));
+    take(best, j, dpp_move<0x140>(best), dpp_move_int<0x140>(j));
+    take(best, j, __shfl_xor(best, 16, 64), __shfl_xor(j, 16, 64));
+    take(best, j, __shfl_xor(best, 32, 64), __shfl_xor(j, 32, 64));
+    if (!(best > tol)) break;
+    if (threadIdx.x == 0) piv[k] = j;
+    const double lkk = sqrt(best);
+    // column j of L below the pivot: the rows not chosen yet
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+      if (i != j && !((done >> i) & 1ull)) G[i * n + j] = (1.0 / lkk) * G[i * n + j];
+    done |= 1ull << j;
+    __syncthreads();
+    // the pivot's diagonal is written only now: before the barrier other
+    // waves may still be reading the diagonal for their own pivot search
+    if (threadIdx.x == 0) G[j * n + j] = mk<double>(lkk, 0.0);
+    for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
+      const int i = f / n, c = f % n;
+      if (!((done >> i) & 1ull) && !((done >> c) & 1ull)) G[i * n + c] = G[i * n + c] - G[i * n + j] * conj(G[c * n + j]);
+    }
+    __syncthreads();
+  }
+  // the indices never chosen follow the pivots, in increasing order
+  if (threadIdx.x == 0) {
+    int m = k;
+    for (int i = 0; i < n; ++i)
+      if (!((done >> i) & 1ull)) piv[m++] = i;
+  }
+  __syncthreads();
+  // R[i][piv[m]] = conj(L[m][i]) for m >= i, rows i < k; L[m][i] sits at
+  // G[piv[m]][piv[i]]
+  for (int f = threadIdx.x; f < k * n; f += blockDim.x) {
+    const int i = f / n, m = f % n;
+    if (m >= i) R[i * n + piv[m]] = m == i ? conj(G[piv[i] * n + piv[i]]) : conj(G[piv[m] * n + piv[i]]);
+  }
+  __syncthreads();
+}
+
+__device__ inline void pivoted_cholesky_swap(cd* G, int n, cd* R, int* piv, double* scal) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) piv[i] = i;
   for (int i = threadIdx.x; i < n * n; i += blockDim.x) R[i] = mk<double>(0.0, 0.0);
   if (threadIdx.x == 0) {
