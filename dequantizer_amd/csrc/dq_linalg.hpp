@@ -711,15 +711,19 @@ __device__ inline void pivoted_cholesky_swap(cd* G, int n, cd* R, int* piv, doub
 // chosen pivots are tracked in a bit mask (identical in every thread) and
 // every wave finds the next pivot itself, so a step costs two block barriers
 // (after the column scaling, after the trailing update) instead of up to
-// five.  Per element the same operations in the same order as the swapping
-// version; ties between equal pivots go to the smaller original index.
+// five.  The pivot's square root and its reciprocal come from the refined
+// reciprocal square root (rsqrt_nr); ties between equal pivots go to the
+// smaller original index.
 __device__ inline void pivoted_cholesky(cd* G, int n, cd* R, int* piv, double* scal) {
   if (n > 64) {
     pivoted_cholesky_swap(G, n, R, piv, scal);
     return;
   }
   const int lane = threadIdx.x & 63;
-  for (int i = threadIdx.x; i < n * n; i += blockDim.x) R[i] = mk<double>(0.0, 0.0);
+  // the block size once, in a scalar register: read inside the loops it is
+  // a load from the dispatch packet after every barrier
+  const int bd = __builtin_amdgcn_readfirstlane(blockDim.x);
+  for (int i = threadIdx.x; i < n * n; i += bd) R[i] = mk<double>(0.0, 0.0);
   double mx = 0.0;
   for (int i = 0; i < n; ++i) mx = fmax(mx, G[i * n + i].re);
   const double tol = n * 2.3e-16 * mx;
@@ -748,17 +752,19 @@ __device__ inline void pivoted_cholesky(cd* G, int n, cd* R, int* piv, double* s
     take(best, j, __shfl_xor(best, 32, 64), __shfl_xor(j, 32, 64));
     if (!(best > tol)) break;
     if (threadIdx.x == 0) piv[k] = j;
-    const double lkk = sqrt(best);
+    const double inv = rsqrt_nr(best);  // best > tol > 0
+    const double lkk = best * inv;
     // column j of L below the pivot: the rows not chosen yet
-    for (int i = threadIdx.x; i < n; i += blockDim.x)
-      if (i != j && !((done >> i) & 1ull)) G[i * n + j] = (1.0 / lkk) * G[i * n + j];
+    for (int i = threadIdx.x; i < n; i += bd)
+      if (i != j && !((done >> i) & 1ull)) G[i * n + j] = inv * G[i * n + j];
     done |= 1ull << j;
     __syncthreads();
     // the pivot's diagonal is written only now: before the barrier other
     // waves may still be reading the diagonal for their own pivot search
     if (threadIdx.x == 0) G[j * n + j] = mk<double>(lkk, 0.0);
-    for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
-      const int i = f / n, c = f % n;
+    const int sh = n == 32 ? 5 : (n == 16 ? 4 : 0);  // row/column by shifts for n = 16, 32
+    for (int f = threadIdx.x; f < n * n; f += bd) {
+      const int i = sh ? f >> sh : f / n, c = sh ? f & (n - 1) : f % n;
       if (!((done >> i) & 1ull) && !((done >> c) & 1ull)) G[i * n + c] = G[i * n + c] - G[i * n + j] * conj(G[c * n + j]);
     }
     __syncthreads();
@@ -772,7 +778,7 @@ __device__ inline void pivoted_cholesky(cd* G, int n, cd* R, int* piv, double* s
   __syncthreads();
   // R[i][piv[m]] = conj(L[m][i]) for m >= i, rows i < k; L[m][i] sits at
   // G[piv[m]][piv[i]]
-  for (int f = threadIdx.x; f < k * n; f += blockDim.x) {
+  for (int f = threadIdx.x; f < k * n; f += bd) {
     const int i = f / n, m = f % n;
     if (m >= i) R[i * n + piv[m]] = m == i ? conj(G[piv[i] * n + piv[i]]) : conj(G[piv[m] * n + piv[i]]);
   }
