@@ -6,12 +6,14 @@
 // U = T x_{j != idx} M_j (message j applied on the ket side), out[k,l] =
 // sum conj(T)[..k..] U[..l..] over the other axes, out /= trace(out).
 //
-// One 512-thread block (8 waves, two per SIMD so that one wave's LDS
-// fetches overlap the other's MFMAs) per node.  T (16x16x16x2 complex) and
-// one work tensor live in LDS with a padded layout: element (b0, b1, b2, s)
-// at b0*529 + b1*33 + b2*2 + s (complex), so that the 16 lanes of an MFMA
-// operand fetch that walk along any bond axis hit distinct banks.  The two
-// tensors take 135 KiB of the CU's 160 KiB, so one block runs per CU.
+// Persistent 512-thread blocks (8 waves, two per SIMD so that one wave's LDS
+// fetches overlap the other's MFMAs), one per CU, each looping over its
+// nodes.  T (16x16x16x2 complex) and one work tensor live in LDS with a
+// padded layout: element (b0, b1, b2, s) at b0*529 + b1*33 + b2*2 + s
+// (complex), so that the 16 lanes of an MFMA operand fetch that walk along
+// any bond axis hit distinct banks.  The two tensors take 135 KiB of the
+// CU's 160 KiB (with the incoming messages and the block's node metadata,
+// 142 KiB).
 //   mode product  X x_j M: a GEMM (16 x 16) x (16 x 512) split in 32 column
 //                 tiles; per tile 4 k-steps of V_MFMA_F32_16X16X4_F32, each
 //                 complex product = 4 real MFMAs;
@@ -114,25 +116,6 @@ __device__ __forceinline__ void mode_gram(const float2* t, const float2* x, cons
   }
 }
 
-// Partial Gram of this wave: sum over its 512/W columns of conj(T[k]) U[l].
-template <int W>
-__device__ __forceinline__ void gram_partial(const float2* t, const float2* u, int idx, int wave, int lane,
-                                             floatx4& gr, floatx4& gi) {
-  const int st = axis_stride(idx);
-  const int row = lane & 15, kq = lane >> 4;
-  constexpr int per = 128 / W;
-  gr = floatx4{0.f, 0.f, 0.f, 0.f};
-  gi = floatx4{0.f, 0.f, 0.f, 0.f};
-  for (int ks = wave * per; ks < wave * per + per; ++ks) {
-    const int off = col_base(idx, 4 * ks + kq) + row * st;
-    const float2 a = t[off], b = u[off];
-    // conj(a) * b
-    gr = mfma(a.x, b.x, gr);
-    gr = mfma(a.y, b.y, gr);
-    gi = mfma(a.x, b.y, gi);
-    gi = mfma(-a.y, b.x, gi);
-  }
-}
 }  // namespace fast3
 
 template <int NW>
