@@ -86,19 +86,19 @@ __global__ void __launch_bounds__(256)
   double nrm = 0.0;
   for (int j = 0; j < n; ++j) nrm += sigma[j] * sigma[j];
   nrm = sqrt(nrm);
-  // masked Frobenius norms of U and Vh
-  double nu = 0.0, nv = 0.0;
-  for (int j = 0; j < n; ++j) {
+  // masked Frobenius norms of U and Vh, the (j, i) terms spread over the
+  // block and summed once
+  double pu = 0.0, pv = 0.0;
+  for (int q = threadIdx.x; q < cc; q += blockDim.x) {
+    const int j = q / n, i = q % n;
     const double lj = nrm > 0.0 ? sigma[j] / nrm : 0.0;
     if (safe_truncate(lj, eps) > eps) {
-      for (int i = 0; i < n; ++i) {
-        nu += norm2(U[j * n + i]);
-        nv += norm2(V[perm[j] * n + i]);
-      }
+      pu += norm2(U[j * n + i]);
+      pv += norm2(V[perm[j] * n + i]);
     }
   }
-  nu = sqrt(nu);
-  nv = sqrt(nv);
+  const double nu = sqrt(block_sum(pu, red));
+  const double nv = sqrt(block_sum(pv, red));
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
     const double lj = nrm > 0.0 ? safe_truncate(sigma[j] / nrm, eps) : 0.0;
     lam[(long long)e * n + j] = lj;
@@ -193,7 +193,15 @@ __global__ void __launch_bounds__(128)
       tp.wb()[j] = wb[j];
     }
   }
-  // m^{1/2} and m^{-1/2} (safe) from the eigendecompositions
+  // m^{1/2} and m^{-1/2} (safe) from the eigendecompositions; the square
+  // roots of the clipped and of the inverted eigenvalues once per edge
+  __shared__ double rt[4][n];
+  if (tid < 4 * n) {
+    const int which = tid / n, i = tid % n;
+    const double w = (which & 1) ? wb[i] : wf[i];
+    rt[which][i] = which < 2 ? sqrt(safe_truncate(w, eps)) : sqrt(safe_inverse(w, eps));
+  }
+  __syncthreads();
   cd* Sf = Mf;
   cd* Sb = Mb;
   for (int q = tid; q < cc; q += blockDim.x) {
@@ -203,8 +211,7 @@ __global__ void __launch_bounds__(128)
     for (int i = 0; i < n; ++i) {
       const cd pf = Qf[r * n + i] * conj(Qf[c * n + i]);
       const cd pb = Qb[r * n + i] * conj(Qb[c * n + i]);
-      const double a = sqrt(safe_truncate(wf[i], eps)), b = sqrt(safe_truncate(wb[i], eps));
-      const double ai = sqrt(safe_inverse(wf[i], eps)), bi = sqrt(safe_inverse(wb[i], eps));
+      const double a = rt[0][i], b = rt[1][i], ai = rt[2][i], bi = rt[3][i];
       sf = sf + a * pf;
       sb = sb + b * pb;
       inf = inf + ai * pf;
@@ -233,19 +240,20 @@ __global__ void __launch_bounds__(128)
   double nrm = 0.0;
   for (int j = 0; j < n; ++j) nrm += sigma[j] * sigma[j];
   nrm = sqrt(nrm);
-  // masked Frobenius norms of U and Vh
-  double nu = 0.0, nv = 0.0;
-  for (int j = 0; j < n; ++j) {
+  // masked Frobenius norms of U and Vh, the (j, i) terms spread over the
+  // block and summed once
+  __shared__ double red[8];
+  double pu = 0.0, pv = 0.0;
+  for (int q = tid; q < cc; q += blockDim.x) {
+    const int j = q / n, i = q % n;
     const double lj = nrm > 0.0 ? sigma[j] / nrm : 0.0;
     if (safe_truncate(lj, eps) > eps) {
-      for (int i = 0; i < n; ++i) {
-        nu += norm2(U[j * n + i]);
-        nv += norm2(V[perm[j] * n + i]);
-      }
+      pu += norm2(U[j * n + i]);
+      pv += norm2(V[perm[j] * n + i]);
     }
   }
-  nu = sqrt(nu);
-  nv = sqrt(nv);
+  const double nu = sqrt(block_sum(pu, red));
+  const double nv = sqrt(block_sum(pv, red));
   for (int j = tid; j < n; j += blockDim.x) {
     const double lj = nrm > 0.0 ? safe_truncate(sigma[j] / nrm, eps) : 0.0;
     lam[(long long)e * n + j] = lj;
