@@ -651,24 +651,44 @@ __global__ void __launch_bounds__(256, kMem == kSuCompact ? 2 : 1)
   } else {
     jacobi_svd(C, n, n, n, U, n, Vs, sigma, rot, flag, perm);
   }
-  if (threadIdx.x == 0) {
+  int keep = n < max_rank ? n : max_rank;
+  if (keep > chi) keep = chi;
+  if (n <= 64) {
+    // wave 0, one singular value per lane: threshold, norm, entropy terms
+    // and discarded weight summed across the wave (the logarithms in parallel)
+    if (threadIdx.x < 64) {
+      const int j = threadIdx.x;
+      double sj = 0.0;
+      if (j < n) {
+        sj = sigma[j] > eps ? sigma[j] : 0.0;
+        sigma[j] = sj;
+      }
+      const double nrm = sqrt(wave_sum(sj * sj));
+      const double x = nrm > 0.0 ? (sj / nrm) * (sj / nrm) : 0.0;
+      const double ent = -wave_sum(x > 0.0 ? x * log(x) : 0.0);
+      const double err = wave_sum(j < n && j >= keep ? x : 0.0);
+      if (j == 0) {
+        err_out[blockIdx.x] = sqrt(err);
+        ent_out[blockIdx.x] = ent;
+        sh_rank = keep;
+      }
+    }
+  } else if (threadIdx.x == 0) {
     double nrm = 0.0;
     for (int j = 0; j < n; ++j) {
       if (!(sigma[j] > eps)) sigma[j] = 0.0;
       nrm += sigma[j] * sigma[j];
     }
     nrm = sqrt(nrm);
-    int rank = n < max_rank ? n : max_rank;
-    if (rank > chi) rank = chi;
     double ent = 0.0, err = 0.0;
     for (int j = 0; j < n; ++j) {
       const double x = nrm > 0.0 ? (sigma[j] / nrm) * (sigma[j] / nrm) : 0.0;
       if (x > 0.0) ent -= x * log(x);
-      if (j >= rank) err += x;
+      if (j >= keep) err += x;
     }
     err_out[blockIdx.x] = sqrt(err);
     ent_out[blockIdx.x] = ent;
-    sh_rank = rank;
+    sh_rank = keep;
   }
   __syncthreads();
   const int rank = sh_rank;
