@@ -138,7 +138,7 @@ def main():
     # one captured graph): the per-sweep duration is the total over the sweeps
     total_s, n_sweeps_timed = 0.0, 0
     for ev0, ev1, act, n_sw in emu.sweep_events:
-        if int(act.sum().item()) > 0:
+        if (act if isinstance(act, bool) else int(act.sum().item()) > 0):
             total_s += ev0.elapsed_time(ev1) * 1e-3
             n_sweeps_timed += n_sw
     durs = [total_s / n_sweeps_timed] if n_sweeps_timed else []
