@@ -245,12 +245,18 @@ __global__ void __launch_bounds__(64 * NW)
     while (next < nmine && !valid_of(next)) ++next;
     const float4* tg_n = next < nmine ? (const float4*)(T + toff_of(next)) : nullptr;
     __syncthreads();
-    // the next node's tensor is requested after the staging barrier, so that
-    // the issue of its loads overlaps the other wave's matrix-core work
-    if (tg_n) {
+    // the next node's tensor is requested in four parts, one after each of
+    // the next four barriers, so that the issue of its loads (64 KiB through
+    // the CU's address path) overlaps matrix-core work instead of stalling
+    // one phase
+    constexpr int PQ = PER / 8;  // float4 loads per part
+    auto prefetch = [&](int part) {
+      if (tg_n) {
 #pragma unroll
-      for (int k = 0; k < PER / 2; ++k) pre[k] = tg_n[tid + k * NT];
-    }
+        for (int k = part * PQ; k < (part + 1) * PQ; ++k) pre[k] = tg_n[tid + k * NT];
+      }
+    };
+    prefetch(0);
     // synchronous edge update fused in (e2n_copy != NULL): this node's three
     // edge->node messages of the next sweep are the opposite sides' inputs
     // of this one, e2n_copy[e][s] = n2e_old[e][1 - s]
@@ -277,11 +283,14 @@ __global__ void __launch_bounds__(64 * NW)
     for (int k = 0; k < 3; ++k) gr[k] = gi[k] = floatx4{0.f, 0.f, 0.f, 0.f};
     mode_product<NW>(wk, tl, m[2], 2, wave, lane);  // X = T x_2 M2
     __syncthreads();
+    prefetch(1);
     mode_gram<NW>(tl, wk, m[1], 1, 0, 2, wave, lane, gr[0], gi[0]);  // U_0 = X x_1 M1
     mode_gram<NW>(tl, wk, m[0], 0, 1, 2, wave, lane, gr[1], gi[1]);  // U_1 = X x_0 M0
     __syncthreads();
+    prefetch(2);
     mode_product<NW>(wk, tl, m[0], 0, wave, lane);  // X = T x_0 M0
     __syncthreads();
+    prefetch(3);
     mode_gram<NW>(tl, wk, m[1], 1, 2, 0, wave, lane, gr[2], gi[2]);  // U_2 = X x_1 M1
     float2 mr[QM];
     if (next < nmine) fetch_msgs(next, mr);
