@@ -76,6 +76,10 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="before the timed steps, one device-to-device copy of this many MiB (a kernel of known "
                          "traffic against which hardware byte counters are scaled); 0 skips it")
+    ap.add_argument("--exact-qubits", type=int, default=30,
+                    help="exact state-vector leg: qubits of a random 3-regular instance (16 * 2^n bytes of HBM); "
+                         "0 skips it")
+    ap.add_argument("--exact-layers", type=int, default=10, help="timed annealing layers of the exact leg")
     ap.add_argument("--gate-order", default="colored", choices=["colored", "pairs"],
                     help="coupling-gate order: pair list sorted by edge colour, or the generator's pair order")
     args = ap.parse_args()
@@ -200,6 +204,10 @@ def main():
     if args.eagle_steps:
         result["eagle"] = eagle_run(args)
         print(f"bench: eagle leg done: {result['eagle']['seconds']:.1f} s", file=sys.stderr, flush=True)
+    if args.exact_qubits:
+        result["exact_sv"] = exact_run(args)
+        print(f"bench: exact state-vector leg done: {result['exact_sv']['ms_per_layer']:.1f} ms per layer",
+              file=sys.stderr, flush=True)
     if args.lattice_size:
         result["lattice"] = lattice_run(args)
         print(f"bench: lattice leg done: {result['lattice']['sweep_ms_c64']:.0f} ms per complex64 sweep",
@@ -258,6 +266,65 @@ def batched_run(args):
             "energy_per_spin_bp": [round(float(x), 6) for x in e_bp],
             "energy_per_spin_sign": [round(float(x), 6) for x in e_sign],
             "energy_per_spin_bp_mean": float(e_bp.mean())}
+
+
+def exact_run(args):
+    """Exact state-vector annealing (exact_quantum_annealer.py:84-122) of a
+    random 3-regular instance: per layer one fused diagonal pass for all
+    coupling gates and the single-qubit mixing layer (12 qubits per pass);
+    HIP events on the launch stream time each kernel group.  Algorithmic
+    bytes of one pass: the 2^n complex128 amplitudes read and written once."""
+    import torch
+
+    from dequantizer_amd.annealing import HADAMARD, coupling_gates, mixing_gate, random_regular
+    from dequantizer_amd.exact import StateVector
+
+    n = args.exact_qubits
+    ef = random_regular(n, 3, seed=args.seed)
+    pairs = np.asarray(ef.coupled_spin_pairs, dtype=np.int64)
+    sv = StateVector(n)
+    sv.apply1_layer(HADAMARD)
+    sched = [(args.dt * (1 - i / max(1, args.exact_layers)), args.dt * i / max(1, args.exact_layers))
+             for i in range(args.exact_layers + 1)]
+    stream = torch.cuda.current_stream()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in sched]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, (mix, cpl) in enumerate(sched):
+        diag = np.diagonal(coupling_gates(ef, cpl).reshape(-1, 4, 4), axis1=1, axis2=2)
+        mg = mixing_gate(mix)
+        ev[i][0].record(stream)
+        sv.apply_diagonal(pairs, diag)
+        ev[i][1].record(stream)
+        sv.apply1_layer(mg)
+        ev[i][2].record(stream)
+        if i == 0:  # the first layer is the warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t0
+    r0 = torch.cuda.Event(enable_timing=True)
+    r1 = torch.cuda.Event(enable_timing=True)
+    r0.record(stream)
+    rho = sv.rdm1()
+    r1.record(stream)
+    torch.cuda.synchronize()
+    trace_err = float((rho[:, 0, 0] + rho[:, 1, 1] - 1).abs().max())
+    diag_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev[1:]]))
+    mix_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev[1:]]))
+    del sv, rho
+    torch.cuda.empty_cache()
+    passes = 1 + max(0, -(-(n - 12) // 8))
+    pass_bytes = 2 * 16 * float(2 ** n)
+    return {"qubits": n, "couplings": int(pairs.shape[0]), "layers": args.exact_layers,
+            "state_gib": 16 * 2 ** n / 2 ** 30, "seconds": seconds, "ms_per_layer": 1e3 * seconds / args.exact_layers,
+            "diagonal_pass_ms": diag_ms, "diagonal_pass_gbs": pass_bytes / diag_ms / 1e6,
+            "diagonal_pass_frac": pass_bytes / diag_ms / 1e6 / PEAK_HBM_GBS,
+            "mixing_layer_ms": mix_ms, "mixing_passes": passes,
+            "mixing_pass_gbs": passes * pass_bytes / mix_ms / 1e6,
+            "mixing_pass_frac": passes * pass_bytes / mix_ms / 1e6 / PEAK_HBM_GBS,
+            "rdm1_all_ms": r0.elapsed_time(r1), "rdm1_passes": passes,
+            "rdm1_pass_gbs": passes * pass_bytes / 2 / r0.elapsed_time(r1) / 1e6, "trace_err": trace_err}
 
 
 def eagle_run(args, chi=32):

@@ -36,6 +36,7 @@ from .mappings import (
 )
 from ._lib import DQError
 from .emulator import BatchedBPEmulator, BPQuantumEmulator
+from .exact import ExactQuantumEmulator
 from .random import PRNGKey, split
 
 __all__ = [
@@ -46,5 +47,5 @@ __all__ = [
     "get_heavy_hex_ibm_eagle_lattice_infinite", "get_belief_propagation_map", "get_vidal_gauge_fixing_map",
     "get_vidal_gauge_distance_map", "get_symmetric_gauge_fixing_map", "lambdas2messages",
     "messages_frob_distance", "get_one_side_density_matrix", "get_two_side_density_matrix",
-    "PRNGKey", "split", "BPQuantumEmulator", "BatchedBPEmulator", "DQError",
+    "PRNGKey", "split", "BPQuantumEmulator", "BatchedBPEmulator", "ExactQuantumEmulator", "DQError",
 ]

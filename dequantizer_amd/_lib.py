@@ -67,6 +67,13 @@ SIGNATURES = {
     "dq_rdm2": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, vp]),
     "dq_update_active": (c_int, [vp, c_dbl, vp, c_int, vp, c_int, vp]),
     "dq_repad": (c_int, [vp, vp, c_int, vp, vp, vp]),
+    "dq_sv_work_bytes": (c_ll, []),
+    "dq_sv_set_basis": (c_int, [vp, c_int, c_ll, vp]),
+    "dq_sv_apply1": (c_int, [vp, c_int, c_int, vp, vp]),
+    "dq_sv_apply2": (c_int, [vp, c_int, c_int, c_int, vp, vp]),
+    "dq_sv_apply_diagonal": (c_int, [vp, c_int, c_int, vp, vp, vp, vp]),
+    "dq_sv_apply1_layer": (c_int, [vp, c_int, c_ll, vp, vp]),
+    "dq_sv_rdm1": (c_int, [vp, c_int, c_ll, vp, vp, vp]),
 }
 
 

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libdq.so")
-SOURCES = ["dq_capi.hip", "dq_bp.hip", "dq_bp_fast.hip", "dq_gauge.hip", "dq_su.hip", "dq_obs.hip", "dq_tensor.hip", "dq_gates.hip", "dq_adjoint.hip", "dq_big.hip"]
+SOURCES = ["dq_capi.hip", "dq_bp.hip", "dq_bp_fast.hip", "dq_gauge.hip", "dq_su.hip", "dq_obs.hip", "dq_tensor.hip", "dq_gates.hip", "dq_adjoint.hip", "dq_big.hip", "dq_sv.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",

@@ -731,6 +731,83 @@ int dq_update_active(const double* dist, double acc, int* iters, int max_iters, 
   return DQ_OK;
 }
 
+static int sv_check(const void* psi, int n, const char* what) {
+  if (!psi) return fail(DQ_ERR_INVALID, "%s: psi is NULL", what);
+  if (n < 1 || n > 40) return fail(DQ_ERR_INVALID, "%s: n_qubits %d outside [1, 40]", what, n);
+  return DQ_OK;
+}
+
+long long dq_sv_work_bytes(void) {
+  const long long dlow = 4096LL * 16;
+  const long long part = sv_rdm_part_bytes();
+  return part > dlow ? part : dlow;
+}
+
+int dq_sv_set_basis(void* psi, int n_qubits, long long index, void* stream) {
+  int rc = sv_check(psi, n_qubits, "dq_sv_set_basis");
+  if (rc) return rc;
+  if (index < 0 || (unsigned long long)index >= (1ULL << n_qubits))
+    return fail(DQ_ERR_INVALID, "dq_sv_set_basis: index %lld outside [0, 2^%d)", index, n_qubits);
+  RET_LAUNCH(launch_sv_basis(psi, n_qubits, (unsigned long long)index, S(stream)));
+  return DQ_OK;
+}
+
+int dq_sv_apply1(void* psi, int n_qubits, int bit, const void* gate, void* stream) {
+  int rc = sv_check(psi, n_qubits, "dq_sv_apply1");
+  if (rc) return rc;
+  if (!gate) return fail(DQ_ERR_INVALID, "dq_sv_apply1: gate is NULL");
+  if (bit < 0 || bit >= n_qubits) return fail(DQ_ERR_INVALID, "dq_sv_apply1: bit %d outside [0, %d)", bit, n_qubits);
+  RET_LAUNCH(launch_sv_apply1(psi, n_qubits, bit, gate, S(stream)));
+  return DQ_OK;
+}
+
+int dq_sv_apply2(void* psi, int n_qubits, int bit1, int bit2, const void* gate, void* stream) {
+  int rc = sv_check(psi, n_qubits, "dq_sv_apply2");
+  if (rc) return rc;
+  if (!gate) return fail(DQ_ERR_INVALID, "dq_sv_apply2: gate is NULL");
+  if (bit1 < 0 || bit1 >= n_qubits || bit2 < 0 || bit2 >= n_qubits || bit1 == bit2)
+    return fail(DQ_ERR_INVALID, "dq_sv_apply2: bits (%d, %d) must be distinct and in [0, %d)", bit1, bit2, n_qubits);
+  RET_LAUNCH(launch_sv_apply2(psi, n_qubits, bit1, bit2, gate, S(stream)));
+  return DQ_OK;
+}
+
+int dq_sv_apply_diagonal(void* psi, int n_qubits, int n_terms, const int* term_bits, const void* term_diag,
+                         void* work, void* stream) {
+  int rc = sv_check(psi, n_qubits, "dq_sv_apply_diagonal");
+  if (rc) return rc;
+  if (n_terms < 0) return fail(DQ_ERR_INVALID, "dq_sv_apply_diagonal: n_terms %d < 0", n_terms);
+  if (n_terms == 0) return DQ_OK;
+  if (!term_bits || !term_diag || !work) return fail(DQ_ERR_INVALID, "dq_sv_apply_diagonal: NULL argument");
+  RET_LAUNCH(launch_sv_diag(psi, n_qubits, n_terms, term_bits, term_diag, work, S(stream)));
+  return DQ_OK;
+}
+
+static int sv_mask_check(int n, long long mask, const char* what) {
+  if (n < 64 && ((unsigned long long)mask >> n) != 0ULL)
+    return fail(DQ_ERR_INVALID, "%s: target_mask has bits at or above n_qubits = %d", what, n);
+  return DQ_OK;
+}
+
+int dq_sv_apply1_layer(void* psi, int n_qubits, long long target_mask, const void* gates, void* stream) {
+  int rc = sv_check(psi, n_qubits, "dq_sv_apply1_layer");
+  if (rc) return rc;
+  if ((rc = sv_mask_check(n_qubits, target_mask, "dq_sv_apply1_layer"))) return rc;
+  if (target_mask == 0) return DQ_OK;
+  if (!gates) return fail(DQ_ERR_INVALID, "dq_sv_apply1_layer: gates is NULL");
+  RET_LAUNCH(launch_sv_layer1(psi, n_qubits, (unsigned long long)target_mask, gates, S(stream)));
+  return DQ_OK;
+}
+
+int dq_sv_rdm1(const void* psi, int n_qubits, long long target_mask, void* rho, void* work, void* stream) {
+  int rc = sv_check(psi, n_qubits, "dq_sv_rdm1");
+  if (rc) return rc;
+  if ((rc = sv_mask_check(n_qubits, target_mask, "dq_sv_rdm1"))) return rc;
+  if (target_mask == 0) return DQ_OK;
+  if (!rho || !work) return fail(DQ_ERR_INVALID, "dq_sv_rdm1: NULL argument");
+  RET_LAUNCH(launch_sv_rdm1(psi, n_qubits, (unsigned long long)target_mask, rho, work, S(stream)));
+  return DQ_OK;
+}
+
 int dq_repad(const dq_graph* old, dq_graph* g, int dtype, const void* T_old, void* T_new, void* stream) {
   int rb, rc;
   if (!old || !g) return fail(DQ_ERR_INVALID, "NULL graph");

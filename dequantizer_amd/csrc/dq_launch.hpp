@@ -246,6 +246,15 @@ hipError_t launch_rdm2_vjp(const DevGraph& g, const void* T, const double* lam, 
 hipError_t launch_apply1_vjp(const DevGraph& g, const void* Tpre, const int* nodes, int count, const void* gates,
                              void* Tbar, void* gbar, hipStream_t s);
 
+// Exact state vector (complex128, dq_sv.hip); bits are amplitude-index bits.
+hipError_t launch_sv_basis(void* psi, int n, unsigned long long idx, hipStream_t st);
+hipError_t launch_sv_apply1(void* psi, int n, int b, const void* g, hipStream_t st);
+hipError_t launch_sv_apply2(void* psi, int n, int b1, int b2, const void* g, hipStream_t st);
+hipError_t launch_sv_diag(void* psi, int n, int K, const int* tb, const void* td, void* work, hipStream_t st);
+hipError_t launch_sv_layer1(void* psi, int n, unsigned long long targets, const void* gates, hipStream_t st);
+hipError_t launch_sv_rdm1(const void* psi, int n, unsigned long long targets, void* rho, void* work, hipStream_t st);
+long long sv_rdm_part_bytes();
+
 // Per-instance convergence bookkeeping of the BP-to-Vidal loop.
 hipError_t launch_update_active(const double* dist, double acc, int* iters, int max_iters, int* active, int n_inst,
                                 hipStream_t s);

@@ -286,6 +286,47 @@ int dq_update_active(const double* dist, double acc, int* iters, int max_iters, 
  * `g` (same topology, chi >= old chi); new slots are zero. */
 int dq_repad(const dq_graph* old, dq_graph* g, int dtype, const void* tensors_old, void* tensors_new, void* stream);
 
+/* Exact state vector: device complex128 psi[2^n_qubits], 1 <= n_qubits <= 40
+ * (the caller allocates it).  Bit b of an amplitude's index is one qubit; the
+ * reference's qubit at position pos (its row-major reshape to n axes of 2,
+ * dequantizer/src/quantum_emulator/exact_quantum_emulator.py:67-76) is bit
+ * n_qubits - 1 - pos.  Gates are device complex128, row-major [out][in].
+ * Replaces ExactQuantumEmulator (exact_quantum_emulator.py:11-147) and the
+ * state object of the external `qem` package that the exact annealer drives
+ * (examples/quantum_annealing/quantum_annealing/src/exact_quantum_annealer.py:84-122). */
+
+/* Bytes of the `work` buffer of dq_sv_apply_diagonal and dq_sv_rdm1. */
+long long dq_sv_work_bytes(void);
+
+/* psi = the basis state `index` (set_to_product_state,
+ * exact_quantum_emulator.py:37-49). */
+int dq_sv_set_basis(void* psi, int n_qubits, long long index, void* stream);
+
+/* 2x2 gate on one bit (apply_q1, exact_quantum_emulator.py:87-103). */
+int dq_sv_apply1(void* psi, int n_qubits, int bit, const void* gate, void* stream);
+
+/* 4x4 gate [o1][o2][i1][i2] with index 1 on bit1 and index 2 on bit2
+ * (apply_q2, exact_quantum_emulator.py:51-85). */
+int dq_sv_apply2(void* psi, int n_qubits, int bit1, int bit2, const void* gate, void* stream);
+
+/* n_terms diagonal two-bit gates in one pass over psi: term k acts on bits
+ * term_bits[2k], term_bits[2k+1] (distinct, < n_qubits; device int) with
+ * diagonal term_diag[4k + 2 x1 + x2] (device complex128).  The coupling layer
+ * of the annealers (exact_quantum_annealer.py:97-100, diagonal gates of
+ * :33-58) as one launch. */
+int dq_sv_apply_diagonal(void* psi, int n_qubits, int n_terms, const int* term_bits, const void* term_diag,
+                         void* work, void* stream);
+
+/* gates[b] (device complex128 [n_qubits][2][2]) applied to every bit b set in
+ * target_mask: single-qubit layers (the mixing layer and the initial
+ * Hadamards of exact_quantum_annealer.py:92-102), 12 bits per pass over psi. */
+int dq_sv_apply1_layer(void* psi, int n_qubits, long long target_mask, const void* gates, void* stream);
+
+/* One-qubit density matrices rho[b] (device complex128 [n_qubits][2][2],
+ * rows s, columns s') of every bit b set in target_mask; other entries are
+ * left untouched.  dens_q1, exact_quantum_emulator.py:105-124. */
+int dq_sv_rdm1(const void* psi, int n_qubits, long long target_mask, void* rho, void* work, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -383,3 +383,75 @@ def test_mutated_oracle_is_caught():
         assert not np.allclose(ot.pass_through_node(t, m)[0], good, atol=1e-8)
     finally:
         ot.apply_matrix_on_axis = orig
+
+
+# ------------------------------------------------- state-vector emulator
+def _bare_exact(n, psi):
+    ex = oe.ExactQuantumEmulator.__new__(oe.ExactQuantumEmulator)
+    ex.n, ex.node2pos, ex.state = n, {i: i for i in range(n)}, psi.copy()
+    return ex
+
+
+def _brute_q1(n, p, g):
+    """2^n x 2^n operator of a one-qubit gate by loops over basis states; qubit
+    p is index bit n-1-p (axis 0 of the row-major reshape is the most
+    significant)."""
+    b = n - 1 - p
+    u = np.zeros((2**n, 2**n), dtype=complex)
+    for i in range(2**n):
+        for o in range(2):
+            u[(i & ~(1 << b)) | (o << b), i] += g[o, (i >> b) & 1]
+    return u
+
+
+def _brute_q2(n, pa, pb, g):
+    ba, bb = n - 1 - pa, n - 1 - pb
+    u = np.zeros((2**n, 2**n), dtype=complex)
+    for i in range(2**n):
+        x1, x2 = (i >> ba) & 1, (i >> bb) & 1
+        for o1 in range(2):
+            for o2 in range(2):
+                j = (i & ~(1 << ba) & ~(1 << bb)) | (o1 << ba) | (o2 << bb)
+                u[j, i] += g[o1, o2, x1, x2]
+    return u
+
+
+def _brute_rdm(n, psi, p):
+    b = n - 1 - p
+    rho = np.zeros((2, 2), dtype=complex)
+    for i in range(2**n):
+        if (i >> b) & 1 == 0:
+            a0, a1 = psi[i], psi[i | (1 << b)]
+            rho += np.array([[a0 * a0.conjugate(), a0 * a1.conjugate()], [a1 * a0.conjugate(), a1 * a1.conjugate()]])
+    return rho
+
+
+def test_exact_emulator_against_brute_force_operators():
+    """The oracle's tensordot/moveaxis state-vector updates equal explicit
+    operators built bit by bit, for ordered, reversed and non-adjacent qubit
+    pairs; its density matrices equal the bitwise partial trace; a planted
+    swap of the two qubits of apply_q2 is caught."""
+    n = 5
+    rng = np.random.default_rng(11)
+    psi = rng.standard_normal(2**n) + 1j * rng.standard_normal(2**n)
+    psi /= np.linalg.norm(psi)
+    ex = _bare_exact(n, psi)
+    ref = psi.copy()
+    for p in range(n):
+        g = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+        ex.apply_q1(g, p)
+        ref = _brute_q1(n, p, g) @ ref
+        assert np.allclose(ex.state, ref, atol=1e-12)
+    for pa, pb in [(0, 1), (3, 1), (0, 4), (4, 2)]:
+        g = (rng.standard_normal((2, 2, 2, 2)) + 1j * rng.standard_normal((2, 2, 2, 2))) / 2
+        ex.apply_q2(g, pa, pb)
+        ref = _brute_q2(n, pa, pb, g) @ ref
+        assert np.allclose(ex.state, ref, atol=1e-12)
+    for p in range(n):
+        assert np.allclose(ex.dens_q1(p), _brute_rdm(n, ref, p), atol=1e-12)
+    # planted mistake: the two qubits of a two-qubit gate exchanged
+    g = (rng.standard_normal((2, 2, 2, 2)) + 1j * rng.standard_normal((2, 2, 2, 2))) / 2
+    bad = _bare_exact(n, ref)
+    bad.apply_q2(g, 1, 3)
+    good = _brute_q2(n, 3, 1, g) @ ref
+    assert not np.allclose(bad.state, good, atol=1e-6)
