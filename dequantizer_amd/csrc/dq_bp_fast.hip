@@ -141,7 +141,8 @@ __global__ void __launch_bounds__(64 * NW)
   // that the per-node chain of dependent metadata reads (list entry, node
   // offset, adjacency) leaves the critical path; later nodes read it through
   // scalar loads.
-  constexpr int MT = 64;
+  constexpr int MT = 160;
+  static_assert(MT <= NT, "one thread per gathered node");
   __shared__ long long mt_toff[MT];
   __shared__ int mt_valid[MT], mt_e[MT][3], mt_s[MT][3], mt_code[MT][3];
   const int nmine = count > (int)blockIdx.x ? (count - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x : 0;

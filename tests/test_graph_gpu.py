@@ -77,3 +77,33 @@ def test_sweep_graph_rejects_aliased_buffers():
     other = torch.zeros_like(m)
     assert L.dq_bp_sweeps(h.ptr, _lib.DQ_C128, _lib.ptr(T), _lib.ptr(m), _lib.ptr(other), _lib.ptr(m),
                           _lib.ptr(other), 2, None, st) != 0
+
+
+def test_matrix_core_sweep_beyond_the_gathered_metadata():
+    """42,000 nodes: more than the 160 nodes per block (x 256 blocks) whose
+    graph metadata the complex64 matrix-core node kernel gathers into LDS, so
+    the later nodes of each block read it through scalar loads.  One sweep
+    against the complex128 matrix-core kernel (metadata through scalar loads
+    only) on the same inputs: FP32 accumulation, relative 2e-5 per message."""
+    n, chi = 42000, 16
+    print(f"BP sweep on {n} nodes", flush=True)
+    topo, h = _handle(n, chi, True)
+    L, st = _lib.lib(), _lib.stream_handle()
+    gen = torch.Generator(device=DEV).manual_seed(5)
+    T = torch.randn(h.tensor_elems, dtype=torch.complex128, device=DEV, generator=gen)
+    a = torch.randn((2 * topo.n_edges, chi, chi), dtype=torch.complex128, device=DEV, generator=gen)
+    m0 = (a @ a.conj().transpose(1, 2)).reshape(-1)
+    del a
+    out = {}
+    for code, dtype in ((_lib.DQ_C64, torch.complex64), (_lib.DQ_C128, torch.complex128)):
+        Td, md = T.to(dtype), m0.to(dtype)
+        o = [torch.zeros_like(md), torch.zeros_like(md)]
+        _lib.check(L.dq_bp_sweep(h.ptr, code, _lib.ptr(Td), _lib.ptr(md), _lib.ptr(md), _lib.ptr(o[0]),
+                                 _lib.ptr(o[1]), None, st))
+        out[code] = o
+        del Td, md
+    for k in range(2):
+        got = out[_lib.DQ_C64][k].view(-1, chi * chi).to(torch.complex128)
+        ref = out[_lib.DQ_C128][k].view(-1, chi * chi)
+        rel = (got - ref).abs().pow(2).sum(1).sqrt() / ref.abs().pow(2).sum(1).sqrt()
+        assert float(rel.max()) < 2e-5, (k, float(rel.max()))
