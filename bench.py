@@ -223,6 +223,10 @@ def main():
     if args.cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ann, ef, schedule[args.warmup + args.steps], args)
         result["vs_cpu_baseline"] = result["value"] / result["cpu_baseline"]["value"]
+        if args.exact_qubits:
+            cpu_sv = exact_cpu_baseline(args)
+            result["exact_sv"]["cpu_baseline"] = cpu_sv
+            result["exact_sv"]["vs_cpu_baseline"] = result["exact_sv"]["amplitude_layers_per_s"] / cpu_sv["value"]
     print(json.dumps(result))
 
 
@@ -324,7 +328,8 @@ def exact_run(args):
             "mixing_pass_gbs": passes * pass_bytes / mix_ms / 1e6,
             "mixing_pass_frac": passes * pass_bytes / mix_ms / 1e6 / PEAK_HBM_GBS,
             "rdm1_all_ms": r0.elapsed_time(r1), "rdm1_passes": passes,
-            "rdm1_pass_gbs": passes * pass_bytes / 2 / r0.elapsed_time(r1) / 1e6, "trace_err": trace_err}
+            "rdm1_pass_gbs": passes * pass_bytes / 2 / r0.elapsed_time(r1) / 1e6, "trace_err": trace_err,
+            "amplitude_layers_per_s": 2 ** n * args.exact_layers / seconds}
 
 
 def eagle_run(args, chi=32):
@@ -713,6 +718,36 @@ def cpu_baseline(ann, ef, step_times, args):
                       "GPU state after the timed steps: regauge (%d BP sweeps) + %d simple updates + %d one-site "
                       "gates, %.1f s" % (sweeps, ef.edges_number, ef.nodes_number, dt),
             "seconds": dt, "gates_per_s": ef.edges_number / dt}
+
+
+def exact_cpu_baseline(args, n=22):
+    """The oracle's state-vector emulator (the reference's tensordot updates,
+    NumPy complex128) on one annealing layer of a random 3-regular instance
+    with n qubits: the coupling gates one by one, then the mixing gates.
+    Reported per amplitude (amplitude-layers/s), the unit of the exact leg."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from dequantizer_amd.annealing import coupling_gates, mixing_gate, random_regular
+    from oracle import emulator as oe
+
+    ef = random_regular(n, 3, seed=args.seed)
+    ex = oe.ExactQuantumEmulator.__new__(oe.ExactQuantumEmulator)
+    rng = np.random.default_rng(args.seed)
+    psi = rng.standard_normal(2 ** n) + 1j * rng.standard_normal(2 ** n)
+    ex.n, ex.node2pos, ex.state = n, {i: i for i in range(n)}, psi / np.linalg.norm(psi)
+    gates = coupling_gates(ef, 0.5 * args.dt)
+    mg = mixing_gate(0.5 * args.dt)
+    t0 = time.perf_counter()
+    for (a, b), g in zip(ef.coupled_spin_pairs, gates):
+        ex.apply_q2(g, int(a), int(b))
+    for q in range(n):
+        ex.apply_q1(mg, q)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": 2 ** n / dt, "unit": "amplitude-layers/s", "cores": threads, "kind": "port",
+            "sample": "one annealing layer (%d coupling gates, %d mixing gates) of a %d-qubit random 3-regular "
+                      "instance on the oracle's state vector (NumPy complex128 tensordot), %.1f s"
+                      % (ef.edges_number, n, n, dt),
+            "seconds": dt}
 
 
 if __name__ == "__main__":
