@@ -73,6 +73,7 @@ SIGNATURES = {
     "dq_sv_apply2": (c_int, [vp, c_int, c_int, c_int, vp, vp]),
     "dq_sv_apply_diagonal": (c_int, [vp, c_int, c_int, vp, vp, vp, vp]),
     "dq_sv_apply1_layer": (c_int, [vp, c_int, c_ll, vp, vp]),
+    "dq_sv_apply_diagonal_layer1": (c_int, [vp, c_int, c_int, vp, vp, c_ll, vp, vp, vp]),
     "dq_sv_rdm1": (c_int, [vp, c_int, c_ll, vp, vp, vp]),
 }
 

@@ -798,6 +798,20 @@ int dq_sv_apply1_layer(void* psi, int n_qubits, long long target_mask, const voi
   return DQ_OK;
 }
 
+int dq_sv_apply_diagonal_layer1(void* psi, int n_qubits, int n_terms, const int* term_bits, const void* term_diag,
+                                long long target_mask, const void* gates, void* work, void* stream) {
+  int rc = sv_check(psi, n_qubits, "dq_sv_apply_diagonal_layer1");
+  if (rc) return rc;
+  if ((rc = sv_mask_check(n_qubits, target_mask, "dq_sv_apply_diagonal_layer1"))) return rc;
+  if (n_terms < 0) return fail(DQ_ERR_INVALID, "dq_sv_apply_diagonal_layer1: n_terms %d < 0", n_terms);
+  if (n_terms > 0 && (!term_bits || !term_diag || !work))
+    return fail(DQ_ERR_INVALID, "dq_sv_apply_diagonal_layer1: NULL term argument");
+  if (target_mask != 0 && !gates) return fail(DQ_ERR_INVALID, "dq_sv_apply_diagonal_layer1: gates is NULL");
+  RET_LAUNCH(launch_sv_diag_layer1(psi, n_qubits, n_terms, term_bits, term_diag, (unsigned long long)target_mask,
+                                   gates, work, S(stream)));
+  return DQ_OK;
+}
+
 int dq_sv_rdm1(const void* psi, int n_qubits, long long target_mask, void* rho, void* work, void* stream) {
   int rc = sv_check(psi, n_qubits, "dq_sv_rdm1");
   if (rc) return rc;

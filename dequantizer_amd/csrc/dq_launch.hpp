@@ -252,6 +252,8 @@ hipError_t launch_sv_apply1(void* psi, int n, int b, const void* g, hipStream_t 
 hipError_t launch_sv_apply2(void* psi, int n, int b1, int b2, const void* g, hipStream_t st);
 hipError_t launch_sv_diag(void* psi, int n, int K, const int* tb, const void* td, void* work, hipStream_t st);
 hipError_t launch_sv_layer1(void* psi, int n, unsigned long long targets, const void* gates, hipStream_t st);
+hipError_t launch_sv_diag_layer1(void* psi, int n, int K, const int* tb, const void* td, unsigned long long targets,
+                                 const void* gates, void* work, hipStream_t st);
 hipError_t launch_sv_rdm1(const void* psi, int n, unsigned long long targets, void* rho, void* work, hipStream_t st);
 long long sv_rdm_part_bytes();
 

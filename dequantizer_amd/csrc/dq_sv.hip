@@ -11,6 +11,9 @@
 //                   phase of an amplitude factors into a chunk constant, two
 //                   64-entry tables over the chunk's low bits and a table
 //                   over the low 12 bits shared by all chunks;
+//   k_sv_diag_tile  up to 64 diagonal terms fused into the low tile pass of
+//                   a single-qubit layer (a coupling layer plus the low 12
+//                   bits of a mixing layer in one pass);
 //   k_sv_tile_*     a tile of 2^12 amplitudes spanning up to 12 target
 //                   bits (the 12 lowest, or 8 higher ones with 16-amplitude
 //                   runs of the lowest 4 for coalescing): applies one 2x2
@@ -35,6 +38,8 @@ constexpr int kThreads = 256;
 // terms per launch of it (longer lists take several passes)
 constexpr int kDiagThreads = 1024;
 constexpr int kDiagTerms = 256;
+// diagonal terms of the pass fused with a single-qubit layer
+constexpr int kFusedTerms = 64;
 // blocks of the density-matrix pass (one partial sum each)
 constexpr int kSvRdmBlocks = 2048;
 
@@ -109,24 +114,29 @@ __global__ void __launch_bounds__(kThreads) k_sv_diag_low(cd* dlow, int lb, int 
   }
 }
 
-__global__ void __launch_bounds__(kDiagThreads) k_sv_diag(cd* psi, unsigned long long chunks, int lb, int K,
-                                                             const int* tb, const cd* td, const cd* dlow) {
-  // the terms (K <= kDiagTerms) and, per low bit q < lb, the list of terms
-  // with low bit q and one high bit (entry: k | high bit << 8 | (low bit
-  // first) << 16); list lb: terms with both bits high (k | b1 << 8 | b2 << 16)
-  __shared__ cd sd[kDiagTerms][4];
-  __shared__ unsigned int ent[kDiagTerms];
-  __shared__ int beg[kTileBits + 2];
-  // fq[q][x]: product of the entries of the terms of list q at low bit value
-  // x; c: product over list lb
-  __shared__ cd fq[kTileBits][2];
-  __shared__ cd c;
-  __shared__ cd ta[64], tbh[64];
-  const int la = lb / 2, lh = lb - la;
+// LDS state of the diagonal terms in a block: the terms (at most KMAX), and
+// per low bit q < lb the list of terms with low bit q and one high bit
+// (entry: k | high bit << 8 | (low bit first) << 16); list lb: the terms
+// with both bits high (k | b1 << 8 | b2 << 16).  fq[q][x]: product of the
+// entries of list q at low bit value x; c: product over list lb; ta, tbh: the
+// chunk's phase factors over its low la bits and its next lh bits.
+template <int KMAX>
+struct DiagLds {
+  cd sd[KMAX][4];
+  unsigned int ent[KMAX];
+  int beg[kTileBits + 2];
+  cd fq[kTileBits][2];
+  cd c;
+  cd ta[64], tbh[64];
+};
+
+// Stage the K terms and build the lists (thread q <= lb counts its terms,
+// thread 0 turns the counts into offsets, thread q writes its list in term
+// order).  Ends with a barrier.
+template <int KMAX, int NT>
+__device__ void diag_setup(DiagLds<KMAX>& d, int lb, int K, const int* tb, const cd* td) {
   const int t = threadIdx.x;
-  for (int i = t; i < 4 * K; i += kDiagThreads) sd[i >> 2][i & 3] = td[i];
-  // thread q <= lb counts its terms, thread 0 turns the counts into offsets,
-  // thread q writes its list (in term order)
+  for (int i = t; i < 4 * K; i += NT) d.sd[i >> 2][i & 3] = td[i];
   int mine = 0;
   if (t <= lb) {
     for (int k = 0; k < K; ++k) {
@@ -134,81 +144,98 @@ __global__ void __launch_bounds__(kDiagThreads) k_sv_diag(cd* psi, unsigned long
       const bool l1 = b1 < lb, l2 = b2 < lb;
       mine += (t == lb) ? (!l1 && !l2) : (l1 != l2 && (l1 ? b1 : b2) == t);
     }
-    beg[t + 1] = mine;
+    d.beg[t + 1] = mine;
   }
   __syncthreads();
   if (t == 0) {
-    beg[0] = 0;
-    for (int q = 0; q <= lb; ++q) beg[q + 1] += beg[q];
+    d.beg[0] = 0;
+    for (int q = 0; q <= lb; ++q) d.beg[q + 1] += d.beg[q];
   }
   __syncthreads();
   if (t <= lb) {
-    int m = beg[t];
+    int m = d.beg[t];
     for (int k = 0; k < K; ++k) {
       const int b1 = ldc(tb, 2 * k), b2 = ldc(tb, 2 * k + 1);
       const bool l1 = b1 < lb, l2 = b2 < lb;
       if (t == lb) {
-        if (!l1 && !l2) ent[m++] = (unsigned)k | ((unsigned)b1 << 8) | ((unsigned)b2 << 16);
+        if (!l1 && !l2) d.ent[m++] = (unsigned)k | ((unsigned)b1 << 8) | ((unsigned)b2 << 16);
       } else if (l1 != l2 && (l1 ? b1 : b2) == t) {
-        ent[m++] = (unsigned)k | ((unsigned)(l1 ? b2 : b1) << 8) | ((unsigned)l1 << 16);
+        d.ent[m++] = (unsigned)k | ((unsigned)(l1 ? b2 : b1) << 8) | ((unsigned)l1 << 16);
       }
     }
   }
   __syncthreads();
+}
+
+// The phase tables ta, tbh of the chunk whose high bits are `high`.  Needs
+// at least 128 threads; ends with a barrier.
+template <int KMAX>
+__device__ void diag_tables(DiagLds<KMAX>& d, int lb, unsigned long long high) {
+  const int t = threadIdx.x;
+  const int la = lb / 2, lh = lb - la;
+  if (t <= lb) {
+    cd f0 = mk<double>(1.0, 0.0), f1 = f0;
+    const int e1 = d.beg[t + 1];
+    for (int m = d.beg[t]; m < e1; ++m) {
+      const unsigned e = d.ent[m];
+      const int k = (int)(e & 255u);
+      if (t == lb) {
+        const int b1 = (int)((e >> 8) & 255u), b2 = (int)(e >> 16);
+        f0 = f0 * d.sd[k][2 * (int)((high >> b1) & 1ULL) + (int)((high >> b2) & 1ULL)];
+      } else {
+        const int xh = (int)((high >> ((e >> 8) & 255u)) & 1ULL);
+        if ((e >> 16) & 1u) {  // low bit first: entry 2 x_low + x_high
+          f0 = f0 * d.sd[k][xh];
+          f1 = f1 * d.sd[k][2 + xh];
+        } else {  // entry 2 x_high + x_low
+          f0 = f0 * d.sd[k][2 * xh];
+          f1 = f1 * d.sd[k][2 * xh + 1];
+        }
+      }
+    }
+    if (t == lb) d.c = f0;
+    else {
+      d.fq[t][0] = f0;
+      d.fq[t][1] = f1;
+    }
+  }
+  __syncthreads();
+  if (t < (1 << la)) {
+    cd f = d.c;
+    for (int q = 0; q < la; ++q) f = f * d.fq[q][(t >> q) & 1];
+    d.ta[t] = f;
+  } else if (t >= 64 && t - 64 < (1 << lh)) {
+    const int y = t - 64;
+    cd f = mk<double>(1.0, 0.0);
+    for (int q = 0; q < lh; ++q) f = f * d.fq[la + q][(y >> q) & 1];
+    d.tbh[y] = f;
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kDiagThreads) k_sv_diag(cd* psi, unsigned long long chunks, int lb, int K,
+                                                             const int* tb, const cd* td, const cd* dlow) {
+  __shared__ DiagLds<kDiagTerms> d;
+  const int la = lb / 2;
+  const int t = threadIdx.x;
+  diag_setup<kDiagTerms, kDiagThreads>(d, lb, K, tb, td);
   constexpr int kPer = kTile / kDiagThreads;
   for (unsigned long long ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
     const unsigned long long high = ch << lb;
     // the chunk's amplitudes are requested first, so that their loads are in
-    // flight while the tables below are built
+    // flight while the tables are built
     cd v[kPer];
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
       const int l = t + r * kDiagThreads;
       if (l < (1 << lb)) v[r] = psi[high | (unsigned long long)l];
     }
-    if (t <= lb) {
-      cd f0 = mk<double>(1.0, 0.0), f1 = f0;
-      const int e1 = beg[t + 1];
-      for (int m = beg[t]; m < e1; ++m) {
-        const unsigned e = ent[m];
-        const int k = (int)(e & 255u);
-        if (t == lb) {
-          const int b1 = (int)((e >> 8) & 255u), b2 = (int)(e >> 16);
-          f0 = f0 * sd[k][2 * (int)((high >> b1) & 1ULL) + (int)((high >> b2) & 1ULL)];
-        } else {
-          const int xh = (int)((high >> ((e >> 8) & 255u)) & 1ULL);
-          if ((e >> 16) & 1u) {  // low bit first: entry 2 x_low + x_high
-            f0 = f0 * sd[k][xh];
-            f1 = f1 * sd[k][2 + xh];
-          } else {  // entry 2 x_high + x_low
-            f0 = f0 * sd[k][2 * xh];
-            f1 = f1 * sd[k][2 * xh + 1];
-          }
-        }
-      }
-      if (t == lb) c = f0;
-      else {
-        fq[t][0] = f0;
-        fq[t][1] = f1;
-      }
-    }
-    __syncthreads();
-    if (t < (1 << la)) {
-      cd f = c;
-      for (int q = 0; q < la; ++q) f = f * fq[q][(t >> q) & 1];
-      ta[t] = f;
-    } else if (t >= 64 && t - 64 < (1 << lh)) {
-      const int y = t - 64;
-      cd f = mk<double>(1.0, 0.0);
-      for (int q = 0; q < lh; ++q) f = f * fq[la + q][(y >> q) & 1];
-      tbh[y] = f;
-    }
-    __syncthreads();
+    diag_tables(d, lb, high);
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
       const int l = t + r * kDiagThreads;
       if (l < (1 << lb)) {
-        const cd ph = dlow[l] * (ta[l & ((1 << la) - 1)] * tbh[l >> la]);
+        const cd ph = dlow[l] * (d.ta[l & ((1 << la) - 1)] * d.tbh[l >> la]);
         psi[high | (unsigned long long)l] = ph * v[r];
       }
     }
@@ -428,6 +455,58 @@ __global__ void __launch_bounds__(kThreads, 2) k_sv_tile_apply(cd* psi, unsigned
   }
 }
 
+// The low tile pass (lo = 12, h = 0) of a single-qubit layer with up to
+// kFusedTerms diagonal two-bit terms applied first: a coupling layer and the
+// low 12 bits of the following mixing layer in one pass over the vector.
+// A tile is a chunk of the diagonal pass (lb = 12); element r of thread tid
+// in layout L2 is l = (r << 8) | tid, so its phase factors are
+// dlow[l] * ta[tid & 63] * tbh[(r << 2) | (tid >> 6)].
+__global__ void __launch_bounds__(kThreads, 2) k_sv_diag_tile(cd* psi, unsigned long long blocks, TileSpec s,
+                                                              const cd* gates, int K, const int* tb, const cd* td,
+                                                              const cd* dlow) {
+  __shared__ cd tile[kTilePad];
+  __shared__ DiagLds<kFusedTerms> d;
+  const int tid = threadIdx.x;
+  diag_setup<kFusedTerms, kThreads>(d, kTileBits, K, tb, td);
+  const bool need1 = (s.tmask >> 4) & 15, need0 = s.tmask & 15;
+  cd cur[16];
+  for (unsigned long long blk = blockIdx.x; blk < blocks; blk += gridDim.x) {
+    const unsigned long long base = blk << kTileBits;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) cur[r] = psi[base | (unsigned long long)((r << 8) | tid)];
+    diag_tables(d, kTileBits, base);
+    const cd fa = d.ta[tid & 63];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int l = (r << 8) | tid;
+      cur[r] = (dlow[l] * (fa * d.tbh[(r << 2) | (tid >> 6)])) * cur[r];
+    }
+    gate_stage<2>(cur, s, gates);
+    if (need1 || need0) {
+      lds_write<2>(tile, cur, tid);
+      __syncthreads();
+      if (need1) {
+        lds_read<1>(cur, tile, tid);
+        gate_stage<1>(cur, s, gates);
+        lds_write<1>(tile, cur, tid);
+        __syncthreads();
+      }
+      if (need0) {
+        lds_read<0>(cur, tile, tid);
+        gate_stage<0>(cur, s, gates);
+        lds_write<0>(tile, cur, tid);
+        __syncthreads();
+      }
+      lds_read<2>(cur, tile, tid);
+    }
+    // every thread is done with the tile and the phase tables before the
+    // next ones are written
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) psi[base | (unsigned long long)((r << 8) | tid)] = cur[r];
+  }
+}
+
 __global__ void __launch_bounds__(kThreads) k_sv_tile_rdm(const cd* psi, unsigned long long blocks, TileSpec s,
                                                           double* part) {
   __shared__ cd tile[kTilePad];
@@ -643,6 +722,37 @@ hipError_t launch_sv_layer1(void* psi, int n, unsigned long long targets, const 
     }
     return hipGetLastError();
   });
+}
+
+// The diagonal terms, then the single-qubit gates of `targets`: with at
+// least 12 qubits and at most kFusedTerms terms the terms ride on the low
+// tile pass, otherwise they take a pass of their own.
+hipError_t launch_sv_diag_layer1(void* psi, int n, int K, const int* tb, const void* td, unsigned long long targets,
+                                 const void* gates, void* work, hipStream_t st) {
+  if (K == 0) return targets ? launch_sv_layer1(psi, n, targets, gates, st) : hipSuccess;
+  if (n < kTileBits || K > kFusedTerms) {
+    hipError_t e = launch_sv_diag(psi, n, K, tb, td, work, st);
+    if (e != hipSuccess || !targets) return e;
+    return launch_sv_layer1(psi, n, targets, gates, st);
+  }
+  cd* dlow = (cd*)work;
+  hipLaunchKernelGGL(k_sv_diag_low, dim3(1), dim3(kThreads), 0, st, dlow, kTileBits, K, tb, (const cd*)td);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  TileSpec s{};
+  s.n = n;
+  s.lo = kTileBits;
+  s.h = 0;
+  s.tmask = (int)(targets & ((1ULL << kTileBits) - 1ULL));
+  const unsigned long long blocks = 1ULL << (n - kTileBits);
+  const unsigned long long cap = (unsigned long long)sv_cus() * 2ULL;
+  const int grid = (int)(blocks < cap ? blocks : cap);
+  hipLaunchKernelGGL(k_sv_diag_tile, dim3(grid), dim3(kThreads), 0, st, (cd*)psi, blocks, s, (const cd*)gates, K,
+                     tb, (const cd*)td, (const cd*)dlow);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const unsigned long long rest = targets & ~((1ULL << kTileBits) - 1ULL);
+  return rest ? launch_sv_layer1(psi, n, rest, gates, st) : hipSuccess;
 }
 
 long long sv_rdm_part_bytes() { return (long long)kSvRdmBlocks * kTileBits * 4 * sizeof(double); }

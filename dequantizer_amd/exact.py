@@ -12,7 +12,8 @@ the reference tree): its measurement follows the exact emulator's rule
 The state is one device complex128 vector of 2^n amplitudes; every update is
 a libdq ``dq_sv_*`` launch.  A coupling layer (any number of diagonal
 two-qubit gates) is one pass over the vector, a layer of single-qubit gates
-one pass per 12 qubits.
+one pass per 12 qubits; an annealing layer (up to 64 couplings, then the
+mixing gates) rides the coupling phases on the first of those passes.
 """
 from dataclasses import dataclass
 from typing import Callable, Dict, Iterable, List, Optional, Sequence, Tuple
@@ -81,34 +82,47 @@ class StateVector:
         _lib.check(_lib.lib().dq_sv_apply2(_lib.ptr(self.psi), self.n, self._bit(q1), self._bit(q2), _lib.ptr(g),
                                            _lib.stream_handle()))
 
-    def apply_diagonal(self, pairs, diagonals):
-        """Diagonal two-qubit gates: term k multiplies the amplitude by
-        diagonals[k][2 x1 + x2], x1 = value of qubit pairs[k][0], x2 of
-        pairs[k][1].  One pass over the vector for all terms."""
+    def _terms(self, pairs, diagonals):
         pairs = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
         diagonals = np.asarray(diagonals, dtype=np.complex128).reshape(-1, 4)
         if pairs.shape[0] != diagonals.shape[0]:
             raise ValueError("one diagonal of 4 entries per pair")
-        if pairs.shape[0] == 0:
-            return
         if np.any(pairs[:, 0] == pairs[:, 1]):
             raise ValueError("a two-qubit term needs two distinct qubits")
-        bits = np.array([[self._bit(a), self._bit(b)] for a, b in pairs], dtype=np.int32)
-        tb = torch.from_numpy(bits.reshape(-1)).to(DEVICE)
-        td = _dev(diagonals)
-        _lib.check(_lib.lib().dq_sv_apply_diagonal(_lib.ptr(self.psi), self.n, int(pairs.shape[0]), _lib.ptr(tb),
-                                                   _lib.ptr(td), _lib.ptr(self.work), _lib.stream_handle()))
+        bits = np.array([[self._bit(a), self._bit(b)] for a, b in pairs], dtype=np.int32).reshape(-1)
+        return int(pairs.shape[0]), torch.from_numpy(bits).to(DEVICE), _dev(diagonals)
 
-    def apply1_layer(self, gates, qubits: Optional[Sequence[int]] = None):
-        """gates: one 2x2 gate for all listed qubits, or an (n, 2, 2) array
-        (gate of qubit q at [q]); qubits: None = all."""
+    def _gates(self, gates) -> torch.Tensor:
         gates = np.asarray(gates, dtype=np.complex128)
         if gates.shape == (2, 2):
             gates = np.broadcast_to(gates, (self.n, 2, 2))
         if gates.shape != (self.n, 2, 2):
             raise ValueError(f"gates must be (2, 2) or ({self.n}, 2, 2)")
-        by_bit = gates[::-1]  # bit b = qubit n - 1 - b
-        g = _dev(by_bit)
+        return _dev(gates[::-1])  # bit b = qubit n - 1 - b
+
+    def apply_diagonal(self, pairs, diagonals):
+        """Diagonal two-qubit gates: term k multiplies the amplitude by
+        diagonals[k][2 x1 + x2], x1 = value of qubit pairs[k][0], x2 of
+        pairs[k][1].  One pass over the vector for all terms."""
+        k, tb, td = self._terms(pairs, diagonals)
+        if k == 0:
+            return
+        _lib.check(_lib.lib().dq_sv_apply_diagonal(_lib.ptr(self.psi), self.n, k, _lib.ptr(tb), _lib.ptr(td),
+                                                   _lib.ptr(self.work), _lib.stream_handle()))
+
+    def apply_diagonal_layer1(self, pairs, diagonals, gates, qubits: Optional[Sequence[int]] = None):
+        """apply_diagonal, then apply1_layer: one annealing layer (with at
+        least 12 qubits and at most 64 terms, in one pass fewer)."""
+        k, tb, td = self._terms(pairs, diagonals)
+        g = self._gates(gates)
+        _lib.check(_lib.lib().dq_sv_apply_diagonal_layer1(_lib.ptr(self.psi), self.n, k, _lib.ptr(tb), _lib.ptr(td),
+                                                          self._mask(qubits), _lib.ptr(g), _lib.ptr(self.work),
+                                                          _lib.stream_handle()))
+
+    def apply1_layer(self, gates, qubits: Optional[Sequence[int]] = None):
+        """gates: one 2x2 gate for all listed qubits, or an (n, 2, 2) array
+        (gate of qubit q at [q]); qubits: None = all."""
+        g = self._gates(gates)
         _lib.check(_lib.lib().dq_sv_apply1_layer(_lib.ptr(self.psi), self.n, self._mask(qubits), _lib.ptr(g),
                                                  _lib.stream_handle()))
 
@@ -212,10 +226,9 @@ def run_exact_quantum_annealer(energy_function: EnergyFunction, scheduler: Itera
     history: Optional[List[np.ndarray]] = [] if record_history else None
     pairs = np.asarray(ef.coupled_spin_pairs, dtype=np.int64).reshape(-1, 2)
     for mixing_time, coupling_time in scheduler:
-        if pairs.shape[0]:
-            g = coupling_gates(ef, coupling_time).reshape(-1, 4, 4)
-            sv.apply_diagonal(pairs, np.diagonal(g, axis1=1, axis2=2))
-        sv.apply1_layer(mixing_gate(mixing_time))
+        diag = np.diagonal(coupling_gates(ef, coupling_time).reshape(-1, 4, 4), axis1=1, axis2=2) if pairs.shape[0] \
+            else np.zeros((0, 4), dtype=np.complex128)
+        sv.apply_diagonal_layer1(pairs, diag, mixing_gate(mixing_time))
         if state_callback is not None:
             state_callback(sv)
         if history is not None:

@@ -322,6 +322,13 @@ int dq_sv_apply_diagonal(void* psi, int n_qubits, int n_terms, const int* term_b
  * Hadamards of exact_quantum_annealer.py:92-102), 12 bits per pass over psi. */
 int dq_sv_apply1_layer(void* psi, int n_qubits, long long target_mask, const void* gates, void* stream);
 
+/* dq_sv_apply_diagonal followed by dq_sv_apply1_layer: one annealing layer
+ * (the coupling gates, then the mixing gates; exact_quantum_annealer.py:97-102).
+ * With n_qubits >= 12 and n_terms <= 64 the diagonal terms are applied in the
+ * first pass of the single-qubit layer, saving one pass over psi. */
+int dq_sv_apply_diagonal_layer1(void* psi, int n_qubits, int n_terms, const int* term_bits, const void* term_diag,
+                                long long target_mask, const void* gates, void* work, void* stream);
+
 /* One-qubit density matrices rho[b] (device complex128 [n_qubits][2][2],
  * rows s, columns s') of every bit b set in target_mask; other entries are
  * left untouched.  dens_q1, exact_quantum_emulator.py:105-124. */

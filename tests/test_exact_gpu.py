@@ -126,6 +126,29 @@ def test_diagonal_layer_longer_than_one_launch():
     assert _diff(sv, ex) < TOL
 
 
+@pytest.mark.parametrize("n,k", [(5, 10), (12, 40), (13, 64), (13, 0), (17, 45), (17, 65)])
+def test_diagonal_then_layer_vs_oracle(n, k):
+    """dq_sv_apply_diagonal_layer1: the diagonal terms ride on the low tile
+    pass (12 or more qubits, at most 64 terms) or take their own pass (fewer
+    qubits, more terms); subset and full layers."""
+    print(f"diagonal + layer n={n} terms={k}", flush=True)
+    rng = np.random.default_rng(1000 + 10 * n + k)
+    psi = _state(rng, n)
+    sv = dx.StateVector(n)
+    _load(sv, psi)
+    ex = _oracle(n, psi)
+    for subset in (sorted(rng.choice(n, size=max(1, n // 3), replace=False).tolist()), None):
+        terms = [(int(a), int(b)) for a, b in (rng.choice(n, size=2, replace=False) for _ in range(k))]
+        diags = np.exp(1j * rng.uniform(-np.pi, np.pi, (k, 4)))
+        gates = np.array([_unitary(rng, 2) for _ in range(n)])
+        sv.apply_diagonal_layer1(terms, diags, gates, subset)
+        for (a, b), d in zip(terms, diags):
+            ex.apply_q2(np.diag(d).reshape(2, 2, 2, 2), a, b)
+        for q in (range(n) if subset is None else subset):
+            ex.apply_q1(gates[q], q)
+        assert _diff(sv, ex) < TOL, subset
+
+
 def test_set_basis_and_measure_vs_oracle():
     n = 13
     rng = np.random.default_rng(7)
