@@ -3,7 +3,8 @@
 // Bit b of an amplitude's index is one qubit; the host maps the reference's
 // qubit positions (row-major reshape of the vector to n axes of 2,
 // dequantizer/src/quantum_emulator/exact_quantum_emulator.py:67-76) to bits
-// as b = n - 1 - pos.  Every kernel streams the vector through HBM once:
+// as b = n - 1 - pos.  Each kernel below but the small table builders
+// streams the vector through HBM once:
 //   k_sv_basis      write a computational basis state;
 //   k_sv_apply1     one 2x2 gate on one bit (pairs of amplitudes);
 //   k_sv_apply2     one 4x4 gate on two bits (quads of amplitudes);
