@@ -340,6 +340,394 @@ def exact_run(args):
             "amplitude_layers_per_s": 2 ** n * args.exact_layers / seconds}
 
 
+def eagle_run(args, chi=32):
+    """Kicked-Ising dynamics on the 127-qubit heavy-hex lattice: RX(theta) on
+    every qubit, then RZZ(-pi/2) in three edge-colour layers, each followed by
+    a regauge (synchronous BP to Vidal distance 1e-5, at most 100 sweeps);
+    chi = 32, complex64; theta in linspace(0, pi/2, 9) as nine instances of
+    one batched engine.  <Z> of qubit 62 after the last step, per angle."""
+    import torch
+    from dequantizer_amd.drivers import run_eagle_simulations
+    thetas = np.linspace(0.0, np.pi / 2, 9)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    zs = run_eagle_simulations(thetas, chi, args.eagle_steps, 1e-5, is_infinite=False, synchronous_update=True,
+                               dtype=torch.complex64, chi=chi)
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t0
+    return {"qubits": 127, "chi": chi, "steps": args.eagle_steps, "angles": 9, "seconds": seconds,
+            "ms_per_step_all_angles": 1e3 * seconds / args.eagle_steps,
+            "z62_final": [round(float(z[-1]), 6) for z in zs], "thetas": [round(float(t), 6) for t in thetas]}
+
+
+def lattice_run(args, chi=16, tol=1e-8, max_sweeps=60):
+    """Periodic L^3 cubic lattice, every node of degree 6, chi = 16, phys 2
+    (256 MiB per node tensor in complex64): node tensors with entries
+    (x + iy) / sqrt(2 chi^6), x, y ~ N(0, 1) (torch generator, seed 5),
+    messages A A^H / tr.  Synchronous BP until the mean message change is
+    below 1e-8, then the Vidal gauge, in complex64 and in complex128; the
+    complex64 messages and lambdas are compared with the complex128 ones
+    (that path matches the oracle at this node shape in the tests)."""
+    import torch
+    import dequantizer_amd as dq
+    lat = dq.get_nd_lattice([args.lattice_size] * 3, 2, chi, False)
+    trav = list(lat.get_traversal_iterator())
+    out = {"L": args.lattice_size, "nodes": lat.nodes_number, "edges": lat.edges_number, "degree": 6, "chi": chi}
+    res = {}
+    for name, dt, np_dt in (("c64", torch.complex64, np.complex64), ("c128", torch.complex128, np.complex128)):
+        gen = torch.Generator(device="cuda").manual_seed(5)
+        tensors = {}
+        for el in trav:
+            if isinstance(el, dq.Node):
+                x = torch.randn((*el.bond_shape, 2), dtype=torch.float64, device="cuda", generator=gen)
+                y = torch.randn((*el.bond_shape, 2), dtype=torch.float64, device="cuda", generator=gen)
+                tensors[el.id] = (torch.complex(x, y) / np.sqrt(2.0 * chi ** 6)).to(dt)
+                del x, y
+        rng = np.random.default_rng(5)
+        msgs = {}
+        for el in trav:
+            for nb in el.neighbors:
+                a = rng.standard_normal((chi, chi)) + 1j * rng.standard_normal((chi, chi))
+                m = a @ a.conj().T
+                msgs[dq.MessageID(el.id, nb.id)] = (m / np.trace(m)).astype(np_dt)
+        bp = dq.get_belief_propagation_map(trav, True)
+        vg = dq.get_vidal_gauge_fixing_map(trav, 1e-10)
+        m = bp(tensors, msgs)  # first sweep: upload and workspace set-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sweeps, r = 1, float("inf")
+        while sweeps < max_sweeps:
+            new = bp(tensors, m)
+            r = float(dq.messages_frob_distance(new, m))
+            m = new
+            sweeps += 1
+            if r < tol:
+                break
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        gam, lam = vg(tensors, m)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        out[f"sweeps_{name}"] = sweeps
+        out[f"residual_{name}"] = r
+        out[f"sweep_ms_{name}"] = 1e3 * (t1 - t0) / (sweeps - 1)
+        out[f"vidal_gauge_ms_{name}"] = 1e3 * (t2 - t1)
+        res[name] = ({k: v.cpu().numpy() for k, v in m.items()}, {k: v.cpu().numpy() for k, v in lam.items()})
+        del tensors, gam, lam, m, new, bp, vg
+        torch.cuda.empty_cache()
+    eu = 2 * lat.edges_number
+    out["eu_per_s_c64"] = eu / (out["sweep_ms_c64"] * 1e-3)
+    out["algorithmic_tflops_c64"] = eu * eu_flops(6, chi, 2) / (out["sweep_ms_c64"] * 1e-3) / 1e12
+    (m64, l64), (m128, l128) = res["c64"], res["c128"]
+    out["message_rel_err_c64_vs_c128"] = float(max(np.linalg.norm(m64[k] - m128[k]) / np.linalg.norm(m128[k])
+                                                   for k in m128))
+    out["lambda_abs_err_c64_vs_c128"] = float(max(np.abs(l64[k] - l128[k]).max() for k in l128))
+    return out
+
+
+def adjoint_timing(ef, args):
+    """Energy of the annealing run and its gradient with respect to the
+    schedule knots and every coupling (hand-written adjoint, complex128,
+    synchronous BP, fixed sweeps per regauge): wall time of one forward and
+    of forward + adjoint on the same instance.  The forward + adjoint runs
+    twice: the first call takes the device memory for the forward records
+    (~1 GB per step) from the driver, which can stall for up to a second
+    while memory freed by earlier work is cleared; the second call reuses
+    the allocator's cached blocks.  ``seconds`` is the second call (a
+    gradient-based schedule search calls it repeatedly),
+    ``first_call_seconds`` the first."""
+    import torch
+    from dequantizer_amd import PRNGKey
+    from dequantizer_amd.adjoint import AnnealingGradient
+    gc.collect()
+    torch.cuda.empty_cache()  # blocks cached by the earlier legs
+    t0 = time.perf_counter()
+    run = AnnealingGradient(ef, PRNGKey(args.seed), chi=args.chi, n_steps=args.adjoint_steps, dt=args.dt,
+                            n_knots=args.knots, fixed_sweeps=args.fixed_sweeps or 10)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    knots = np.linspace(0.0, 1.0, args.knots)
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats()
+    held_gb = torch.cuda.memory_allocated() / 1e9
+    retries0 = torch.cuda.memory_stats().get("num_alloc_retries", 0)
+    t0 = time.perf_counter()
+    e_fwd = run.value(knots)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    run.value_and_grad(knots)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    e, dk, dj = run.value_and_grad(knots)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    return {"seconds": t3 - t2, "first_call_seconds": t2 - t1, "forward_seconds": t1 - t0, "setup_seconds": setup,
+            "steps": args.adjoint_steps,
+            "knots": args.knots, "couplings": int(ef.edges_number), "dtype": "c128", "backward_cut": run.cut,
+            "sweeps_per_regauge": args.fixed_sweeps or 10, "energy": e, "energy_forward_only": e_fwd,
+            "grad_norm_knots": float(np.linalg.norm(dk)), "grad_norm_couplings": float(np.linalg.norm(dj)),
+            "peak_device_memory_gb": torch.cuda.max_memory_allocated() / 1e9,
+            "held_before_gb": held_gb,
+            "allocator_retries": torch.cuda.memory_stats().get("num_alloc_retries", 0) - retries0}
+
+
+def plain_gpu_baseline(ann, ef, step_times, args, sample_sweeps=2, sample_gates=150):
+    """The same Trotter step written as plain torch eager ops on the GPU: one
+    node, edge or gate at a time (tensordot for contractions in complex64,
+    torch.linalg eigh / svd / qr in complex128), no batching, no fused kernels.
+    Started from the engine's state after the timed steps.  A bounded sample is
+    timed (the first Vidal gauge and distance, ``sample_sweeps`` BP sweeps, the
+    final gauge and distance, ``sample_gates`` simple updates, all one-site
+    gates) and the step time is extrapolated to the full work (fixed sweeps and
+    every gate).  Reports EU/s = EU per step / step time."""
+    import torch
+    from dequantizer_amd.annealing import coupling_gates, mixing_gate
+    emu = ann.emu
+    emu._flush()
+    topo, chi, eps = emu.topo, emu.chi, 1e-10
+    c64, c128 = torch.complex64, torch.complex128
+    dims = emu.dims.copy()
+    lam_all = emu.lam.detach().view(-1, chi)
+    lams = [lam_all[j, : int(dims[j])].clone() for j in range(topo.n_edges)]
+    adj = [[(int(topo.adj_edge[k]), int(topo.adj_side[k])) for k in range(topo.adj_ptr[i], topo.adj_ptr[i + 1])]
+           for i in range(topo.n_nodes)]
+    gam = []
+    for i in range(topo.n_nodes):
+        deg, off = int(topo.deg[i]), int(emu.h.offsets[i])
+        full = emu.gam.detach()[off: off + chi ** deg * 2].view(*((chi,) * deg), 2)
+        gam.append(full[tuple(slice(0, int(dims[e])) for e, _ in adj[i])].clone())
+
+    def bcast(v, axis, nd):
+        shape = [1] * nd
+        shape[axis] = v.shape[0]
+        return v.reshape(shape)
+
+    def sym(t, ls):
+        for a, lm in enumerate(ls):
+            t = t * bcast(torch.sqrt(lm * (lm > eps)).to(c64), a, t.ndim)
+        return t / torch.linalg.vector_norm(t)
+
+    def on_axis(t, m, a):
+        return torch.movedim(torch.tensordot(m, t, dims=([1], [a])), 0, a)
+
+    def node_update(t, msgs):
+        out = []
+        for idx in range(t.ndim - 1):
+            u = t
+            for a, m in enumerate(msgs):
+                if a != idx:
+                    u = on_axis(u, m, a)
+            others = [a for a in range(t.ndim) if a != idx]
+            g = torch.tensordot(t.conj(), u, dims=(others, others))
+            out.append(g / torch.trace(g))
+        return out
+
+    def edge_svd(mf, mb):
+        lf, uf = torch.linalg.eigh(mf.to(c128))
+        lb, ub = torch.linalg.eigh(mb.to(c128))
+        inv = lambda x: 1.0 / ((x < eps) * 3.4028234663852886e38 + x)
+        fn = lambda u, v: (u * v.to(c128)) @ u.conj().T
+        isf, isb = fn(uf, torch.sqrt(inv(lf))), fn(ub, torch.sqrt(inv(lb)))
+        sf, sb = fn(uf, torch.sqrt(lf * (lf > eps))), fn(ub, torch.sqrt(lb * (lb > eps)))
+        u, lm, vh = torch.linalg.svd(sf @ sb.T, full_matrices=False)
+        lm = lm / torch.linalg.vector_norm(lm)
+        lm = lm * (lm > eps)
+        mask = (lm > eps).to(c128)
+        u = u * mask
+        u = u / torch.linalg.vector_norm(u)
+        vh = vh * mask[:, None]
+        vh = vh / torch.linalg.vector_norm(vh)
+        return (isf @ u).to(c64), (isb @ vh.T).to(c64), lm
+
+    def absorb(t, xs):
+        for a, x in enumerate(xs):
+            t = on_axis(t, x.T, a)
+        return t / torch.linalg.vector_norm(t)
+
+    def vdist(t, ls):
+        total = 0.0
+        for idx in range(t.ndim - 1):
+            u = t
+            for a, lm in enumerate(ls):
+                if a != idx:
+                    u = u * bcast((lm ** 2).to(c64), a, t.ndim)
+            others = [a for a in range(t.ndim) if a != idx]
+            r = torch.tensordot(t.conj(), u, dims=(others, others)).to(c128)
+            r = r / torch.trace(r)
+            ideal = torch.diag((torch.diagonal(r).real > eps).to(c128))
+            total = total + torch.linalg.svdvals(r - ideal / torch.trace(ideal)).sum()
+        return total
+
+    def regauge_vg(tensors, msgs):
+        xs, new_l = {}, {}
+        for j in range(topo.n_edges):
+            xf, xb, lm = edge_svd(msgs[(j, 0)], msgs[(j, 1)])
+            xs[(j, 0)], xs[(j, 1)], new_l[j] = xf, xb, lm
+        new_t = [absorb(tensors[i], [xs[(e, s)] for e, s in adj[i]]) for i in range(topo.n_nodes)]
+        dist = sum(vdist(new_t[i], [new_l[e] for e, _ in adj[i]]) for i in range(topo.n_nodes))
+        return new_t, new_l, dist
+
+    def sweep(tensors, e2n):
+        n2e = {}
+        for i in range(topo.n_nodes):
+            outs = node_update(tensors[i], [e2n[(e, s)] for e, s in adj[i]])
+            for (e, s), m in zip(adj[i], outs):
+                n2e[(e, s)] = m
+        return {(e, s): n2e[(e, 1 - s)] for (e, s) in n2e}
+
+    def su(i_a, i_b, j, g):
+        mat = g.permute(0, 2, 1, 3).reshape(4, 4)
+        uu, ss, vvh = torch.linalg.svd(mat)
+        r = int((ss > eps).sum())
+        root = torch.sqrt(ss[:r])
+        h1 = (uu[:, :r] * root).reshape(2, 2, r).permute(2, 0, 1).to(c64)
+        h2 = (vvh[:r] * root[:, None]).reshape(r, 2, 2).to(c64)
+        sides = []
+        for node, h in ((i_a, h1), (i_b, h2)):
+            t = gam[node]
+            idx = [k for k, (e, _) in enumerate(adj[node]) if e == j][0]
+            for a, (e, _) in enumerate(adj[node]):
+                if a != idx:
+                    t = t * bcast(lams[e].to(c64), a, t.ndim)
+            th = torch.einsum("...i,qoi->...qo", t, h)
+            th = torch.movedim(th, -2, idx + 1)
+            shape = list(t.shape)
+            shape[idx] *= r
+            th = th.reshape(shape)
+            th = torch.movedim(th / torch.linalg.vector_norm(th), idx, -1)
+            q, rr = torch.linalg.qr(th.reshape(-1, th.shape[-1]).to(c128))
+            q = q.reshape(*th.shape[:-1], -1)
+            for a, (e, _) in enumerate([x for k, x in enumerate(adj[node]) if k != idx]):
+                lm = lams[e]
+                q = q * bcast((1.0 / ((lm < eps) * 3.4028234663852886e38 + lm)).to(c128), a, q.ndim)
+            sides.append((q, rr, idx))
+        (qa, ra, ia), (qb, rb, ib) = sides
+        lam_rep = lams[j].repeat_interleave(r)
+        u, sv, vh = torch.linalg.svd((ra * lam_rep.to(c128)) @ rb.T, full_matrices=False)
+        sv = sv * (sv > eps)
+        rank = min(args.chi, sv.shape[0])
+        gam[i_a] = torch.movedim(torch.tensordot(qa, u[:, :rank], dims=([qa.ndim - 1], [0])), -1, ia).to(c64)
+        gam[i_b] = torch.movedim(torch.tensordot(qb, vh[:rank], dims=([qb.ndim - 1], [1])), -1, ib).to(c64)
+        lams[j] = sv[:rank].to(torch.float64)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tensors = [sym(gam[i], [lams[e] for e, _ in adj[i]]) for i in range(topo.n_nodes)]
+    msgs = {(j, s): torch.diag(lams[j].to(c64)) for j in range(topo.n_edges) for s in range(2)}
+    regauge_vg(tensors, msgs)
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(sample_sweeps):
+        msgs = sweep(tensors, msgs)
+    torch.cuda.synchronize()
+    t_sweep = (time.perf_counter() - t0) / sample_sweeps
+    t0 = time.perf_counter()
+    new_t, new_l, _ = regauge_vg(tensors, {k: v for k, v in msgs.items()})
+    torch.cuda.synchronize()
+    t_last = time.perf_counter() - t0
+    gam[:] = new_t
+    for j in range(topo.n_edges):
+        lams[j] = new_l[j]
+    mix, cpl = step_times
+    gates = torch.from_numpy(coupling_gates(ef, cpl)).to("cuda")
+    edge_of = {}
+    for j in range(topo.n_edges):
+        a, b = (int(x) for x in topo.edge_node[j])
+        edge_of[(a, b)] = edge_of[(b, a)] = j
+    nidx = topo.node_index[0]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k, (a, b) in enumerate(ef.coupled_spin_pairs[:sample_gates]):
+        ia, ib = nidx[int(a)], nidx[int(b)]
+        su(ia, ib, edge_of[(ia, ib)], gates[k])
+    torch.cuda.synchronize()
+    t_gate = (time.perf_counter() - t0) / min(sample_gates, ef.edges_number)
+    mg = torch.from_numpy(mixing_gate(mix)).to("cuda", c64)
+    t0 = time.perf_counter()
+    for i in range(topo.n_nodes):
+        t = torch.tensordot(gam[i], mg, dims=([gam[i].ndim - 1], [1]))
+        gam[i] = t / torch.linalg.vector_norm(t)
+    torch.cuda.synchronize()
+    t_apply1 = time.perf_counter() - t0
+    sweeps = args.fixed_sweeps or 10
+    t_step = t_first + sweeps * t_sweep + t_last + ef.edges_number * t_gate + t_apply1
+    eu = sweeps * 2 * ef.edges_number
+    return {"value": eu / t_step, "unit": "edge-updates/s", "kind": "torch eager, one element at a time",
+            "ms_per_step": 1e3 * t_step, "gates_per_s": 1.0 / t_gate,
+            "sample": "timed: first Vidal gauge + distance %.2f s, %d BP sweeps at %.3f s each, final gauge + "
+                      "distance %.2f s, %d simple updates at %.2f ms each, %d one-site gates %.3f s; step "
+                      "extrapolated to %d sweeps and %d gates" % (
+                          t_first, sample_sweeps, t_sweep, t_last, min(sample_gates, ef.edges_number),
+                          1e3 * t_gate, topo.n_nodes, t_apply1, sweeps, ef.edges_number)}
+
+
+def cpu_baseline(ann, ef, step_times, args):
+    """The oracle (NumPy complex128 restatement) on one Trotter step of the
+    same instance, started from the GPU state: regauge + all simple updates
+    + mixing gates.  Reports EU/s = oracle BP EU in that step / its time."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import torch
+    from oracle import annealing as oa
+    from oracle import emulator as oe
+    from oracle import graph as og
+
+    emu = ann.emu
+    emu._flush()
+    topo = emu.topo
+    chi = emu.chi
+    g = og.graph_from_pairs(ef.nodes_number, ef.coupled_spin_pairs)
+    gam = emu.gam.detach().cpu().numpy().astype(np.complex128)
+    lam = emu.lam.detach().cpu().numpy().reshape(-1, chi)
+    tensors, lambdas = {}, {}
+    for j, e in enumerate(topo.edges):
+        d = int(emu.dims[j])
+        lambdas[e.id] = lam[j, :d].copy()
+        g.edges[e.id].dimension = d
+    for i, node in enumerate(topo.nodes):
+        deg = int(topo.deg[i])
+        off = int(emu.h.offsets[i])
+        full = gam[off: off + chi ** deg * 2].reshape((chi,) * deg + (2,))
+        dims = [int(emu.dims[topo.adj_edge[k]]) for k in range(topo.adj_ptr[i], topo.adj_ptr[i + 1])]
+        tensors[node.id] = full[tuple(slice(0, x) for x in dims) + (slice(None),)].copy()
+        g.nodes[node.id].bond_shape = tuple(dims)
+    # oracle emulator with this state (constructed on |0>, then replaced)
+    o = oe.BPQuantumEmulator.__new__(oe.BPQuantumEmulator)
+    o.tensor_graph = g
+    o.prng_key = og.make_key(args.seed)
+    o.max_chi, o.max_iters, o.per_regauge, o.acc = args.chi, 100, ef.edges_number, 1e-5
+    o.gates_since = ef.edges_number
+    o.traverser = g.traversal(ordering=args.traversal)
+    from oracle import mappings as om
+    o.bp_map = om.belief_propagation_map(o.traverser, args.sync)
+    o.vg_map = om.vidal_gauge_fixing_map(o.traverser, 1e-10)
+    o.vd_map = om.vidal_gauge_distance_map(o.traverser, 1e-10)
+    o.sg_map = om.symmetric_gauge_fixing_map(o.traverser, 1e-10)
+    o.tensors, o.lambdas = tensors, lambdas
+    o.vidal_distances_after_regauging, o.truncation_affected_vidal_distances = [], []
+    o.truncation_errors, o.entropies, o.bp_iterations = [], [], []
+    o._initial_messages = None
+    o.fixed_sweeps = args.fixed_sweeps or None
+    mix, cpl = step_times
+    deg = [g.nodes[i].degree for i in range(ef.nodes_number)]
+    t0 = time.perf_counter()
+    for (a, b), jv in zip(ef.coupled_spin_pairs, ef.coupling_amplitudes):
+        a, b = int(a), int(b)
+        o.apply_q2(oa.coupling_gate(jv, ef.fields[a], ef.fields[b], deg[a], deg[b], cpl), a, b)
+    mg = oa.mixing_gate(mix)
+    for i in range(ef.nodes_number):
+        o.apply_q1(mg, i)
+    dt = time.perf_counter() - t0
+    sweeps = sum(o.bp_iterations)
+    eu = sweeps * 2 * ef.edges_number
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": eu / dt, "unit": "edge-updates/s", "cores": threads, "kind": "port",
+            "sample": "one Trotter step of the same instance on the oracle (NumPy complex128), started from the "
+                      "GPU state after the timed steps: regauge (%d BP sweeps) + %d simple updates + %d one-site "
+                      "gates, %.1f s" % (sweeps, ef.edges_number, ef.nodes_number, dt),
+            "seconds": dt, "gates_per_s": ef.edges_number / dt}
+
+
 def exact_cpu_baseline(args, n=22):
     """The oracle's state-vector emulator (the reference's tensordot updates,
     NumPy complex128) on one annealing layer of a random 3-regular instance
