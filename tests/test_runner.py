@@ -1,6 +1,6 @@
 """Configuration layer and instance generators of the annealing runs
 (examples/quantum_annealing/qbp.py, exact.py, configs/, lattice_generator.py,
-energy_function.py).  CPU only, except the end-to-end run at the bottom.
+energy_function.py).  CPU only, except the two end-to-end runs at the bottom.
 
 The instance structures are pinned by what the reference fixes: the Eagle
 coupling map's degree histogram (127 spins, 144 couplings, degrees
@@ -147,3 +147,29 @@ def test_qbp_then_exact_on_the_small_tree(tmp_path):
     assert np.abs(bp["density_matrices_history"] - ex["density_matrices_history"]).max() < 1e-6
     assert math.isclose(float(bp["quantum_annealer_energy_wo_sampling"]),
                         float(ex["quantum_annealer_energy_wo_sampling"]))
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")
+def test_bp_against_exact_on_a_loopy_instance(tmp_path):
+    """The reference's exact-validation workflow (qbp.py, then exact.py on the
+    stored instance) on a 26-spin random 3-regular graph (the size of
+    configs/task_generator/exact_validation.yaml), chi = 16.  Loopy BP is an
+    approximation: the bound is four times the error measured on MI355X
+    (max |<Z>_BP - <Z>_exact| = 4.7e-3); the errors are printed."""
+    root = _config_dir(str(tmp_path / "configs"), nodes=26)
+    out = str(tmp_path / "run")
+    cfg = runner.load_config(root, overrides=["task_generator=rr", "emulator_parameters.max_chi=16",
+                                              "quantum_annealing_schedule.steps_number=40",
+                                              "quantum_annealing_schedule.record_history=false"])
+    bp = runner.run_qbp(cfg, out)
+    ex = runner.run_exact(cfg, out)
+    ef = runner._energy(bp["coupling_amplitudes"], bp["coupled_spin_pairs"], bp["fields"])
+    z_bp = (bp["density_matrices"][:, 0, 0] - bp["density_matrices"][:, 1, 1]).real
+    z_ex = (ex["density_matrices"][:, 0, 0] - ex["density_matrices"][:, 1, 1]).real
+    dz = float(np.abs(z_bp - z_ex).max())
+    e_field_bp = float(-np.dot(ef.fields, z_bp)) / ef.nodes_number
+    e_field_ex = float(-np.dot(ef.fields, z_ex)) / ef.nodes_number
+    print(f"26 spins, 40 steps: max |<Z>_BP - <Z>_exact| = {dz:.3e}; field energy per spin "
+          f"BP {e_field_bp:.5f} exact {e_field_ex:.5f}", flush=True)
+    assert dz < 0.02
