@@ -24,6 +24,7 @@ import torch
 from . import _lib
 from . import random as dqr
 from .annealing import HADAMARD, EnergyFunction, coupling_gates, mixing_gate
+from .emulator import QuantumEmulator
 from .graph import Node, TensorGraph
 
 DEVICE = "cuda"
@@ -147,13 +148,12 @@ class StateVector:
         return result
 
 
-class ExactQuantumEmulator:
+class ExactQuantumEmulator(QuantumEmulator):
     """exact_quantum_emulator.py:11-147: qubit order = node order of the
     graph's default traversal (:24-31)."""
 
     def __init__(self, tensor_graph: TensorGraph, key):
-        self.tensor_graph = tensor_graph
-        self.prng_key = dqr.as_key(key)
+        super().__init__(tensor_graph, dqr.as_key(key))
         node2pos: Dict = {}
         for el in tensor_graph.get_traversal_iterator() or iter([]):
             if isinstance(el, Node):
