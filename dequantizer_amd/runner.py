@@ -317,7 +317,8 @@ def main(argv: Optional[Sequence[str]] = None) -> int:
     ap.add_argument("--output-dir", default="outputs/run",
                     help="qbp: where result.npz goes; exact: where the BP run's result.npz is")
     ap.add_argument("overrides", nargs="*", help="group=option or group.key=value")
-    args = ap.parse_args(argv)
+    # overrides may come before or after the options (group=option ...)
+    args = ap.parse_intermixed_args(argv)
     logging.basicConfig(level=logging.INFO)
     cfg = load_config(args.config_dir, args.config_name, args.overrides)
     if args.mode == "qbp":

@@ -173,3 +173,23 @@ def test_bp_against_exact_on_a_loopy_instance(tmp_path):
     print(f"26 spins, 40 steps: max |<Z>_BP - <Z>_exact| = {dz:.3e}; field energy per spin "
           f"BP {e_field_bp:.5f} exact {e_field_ex:.5f}", flush=True)
     assert dz < 0.02
+
+
+def test_command_line(tmp_path):
+    """python -m dequantizer_amd.runner: help text, a malformed override
+    rejected, and (without a GPU) the run raises instead of computing on the
+    host."""
+    import subprocess
+    import sys
+
+    import torch
+
+    r = subprocess.run([sys.executable, "-m", "dequantizer_amd.runner", "--help"], capture_output=True, text=True,
+                       cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))), timeout=120)
+    assert r.returncode == 0 and "qbp" in r.stdout and "--config-dir" in r.stdout
+    root = _config_dir(str(tmp_path / "configs"))
+    with pytest.raises(ValueError):
+        runner.main(["qbp", "--config-dir", root, "--output-dir", str(tmp_path / "o"), "emulator_parameters"])
+    if not torch.cuda.is_available():
+        with pytest.raises(RuntimeError, match="needs a ROCm GPU"):
+            runner.main(["qbp", "--config-dir", root, "--output-dir", str(tmp_path / "o")])
