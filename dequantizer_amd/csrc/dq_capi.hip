@@ -74,8 +74,17 @@ struct dq_graph {
   double* node_dist = nullptr;  // n_nodes
 };
 
+// Captured sweep graphs bake in the schedule arrays and the workspace: any
+// change to either makes them stale.
+static void drop_sweep_graphs(dq_graph* g) {
+  for (auto& sg : g->sweep_graphs)
+    if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+  g->sweep_graphs.clear();
+}
+
 static int ws_ensure(dq_graph* g, size_t bytes) {
   if (bytes <= g->ws_bytes) return DQ_OK;
+  drop_sweep_graphs(g);
   if (g->ws) HIP_TRY(hipFree(g->ws));
   g->ws = nullptr;
   g->ws_bytes = 0;
@@ -225,8 +234,7 @@ int dq_graph_create(dq_graph** out, int n_nodes, int n_edges, int chi, int phys,
 
 int dq_graph_destroy(dq_graph* g) {
   if (!g) return DQ_OK;
-  for (auto& sg : g->sweep_graphs)
-    if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
+  drop_sweep_graphs(g);
   if (g->capture_stream) (void)hipStreamDestroy(g->capture_stream);
   void* ptrs[] = {g->d_deg,       g->d_adj_ptr,       g->d_adj_edge,      g->d_adj_side,   g->d_toff,
                   g->d_edge_node, g->d_node_inst,     g->d_edge_inst,     g->d_inst_node_ptr,
@@ -267,6 +275,8 @@ int dq_bp_set_schedule(dq_graph* g, int n_levels, const int* level_ptr, const in
   const int nadj = 2 * g->dev.n_edges;
   for (int i = 0; i < nadj; ++i)
     if (node_src[i] > 2 || edge_src[i] > 1) return fail(DQ_ERR_INVALID, "bad source code");
+  // graphs replayed after this call must not read the freed schedule arrays
+  drop_sweep_graphs(g);
   if (g->d_level_nodes) HIP_TRY(hipFree(g->d_level_nodes));
   if (g->d_node_src) HIP_TRY(hipFree(g->d_node_src));
   if (g->d_edge_src) HIP_TRY(hipFree(g->d_edge_src));

@@ -103,3 +103,90 @@ def test_emulator_flow(name, dtype, tol_oracle):
     # measurements agree under shared keys
     for nid in gp.nodes:
         assert bp.measure(nid) == bo.measure(nid)
+
+
+def _layer(emus, trav, grng):
+    for el in trav:
+        if isinstance(el, dq.Node):
+            x = grng.standard_normal((2, 2, 2))
+            q, _ = np.linalg.qr(x[..., 0] + 1j * x[..., 1])
+            for emu, inst in emus:
+                emu.apply_q1(q, el.id) if inst is None else emu.queue_q1(inst, q, el.id)
+        else:
+            g = np.diag(np.exp(1j * grng.standard_normal(4))).reshape(2, 2, 2, 2)
+            for emu, inst in emus:
+                emu.apply_q2(g, *el.id) if inst is None else emu.apply_q2(inst, g, *el.id)
+
+
+def test_non_convergence_warns_and_records_the_cap(caplog):
+    """max_belief_propagation_iterations=1 with an unreachable accuracy: every
+    BP-to-Vidal loop stops after one iteration with a warning, as in the
+    reference (bp_quantum_emulator.py:254-261, 284-291); the recorded
+    iteration counts and Vidal distances equal the oracle emulator's."""
+    import logging
+    gp, go = _build(dq.TensorGraph(), og.small_heavy_hex), og.small_heavy_hex()
+    key = dq.PRNGKey(7)
+    with caplog.at_level(logging.WARNING):
+        bp = BPQuantumEmulator(gp, key, 8, gp.edges_number, belief_propagation_accuracy=1e-300,
+                               max_belief_propagation_iterations=1)
+        bo = oe.BPQuantumEmulator(go, key, 8, go.edges_number, belief_propagation_accuracy=1e-300,
+                                  max_belief_propagation_iterations=1)
+        trav = list(gp.get_traversal_iterator())
+        _layer([(bp, None), (bo, None)], trav, np.random.default_rng(5))
+        rp = _host(bp.dens_q1(0))
+    warned = [r for r in caplog.records if r.levelno == logging.WARNING and "dequantizer_amd" in r.name]
+    assert len(warned) >= 2, [r.getMessage() for r in caplog.records]
+    assert bp.bp_iterations == bo.bp_iterations
+    assert all(n == 1 for n in bp.bp_iterations)
+    np.testing.assert_allclose(bp.after_regauging_vidal_distances, bo.vidal_distances_after_regauging, atol=1e-9)
+    assert min(bp.after_regauging_vidal_distances) > 0
+    assert np.abs(rp - bo.dens_q1(0)).max() < 1e-9
+
+
+def test_emulator_rejects_malformed_gates_and_missing_edges():
+    """apply_q2 with a gate of the wrong rank or shape, or on two nodes without
+    an edge: ValueError (tensor_graph.py:305-315 through apply_q2)."""
+    gp = _build(dq.TensorGraph(), og.small_heavy_hex)
+    bp = BPQuantumEmulator(gp, dq.PRNGKey(1), 4, gp.edges_number)
+    a, b = next(iter(gp.edges))
+    with pytest.raises(ValueError):
+        bp.apply_q2(np.eye(4), a, b)
+    with pytest.raises(ValueError):
+        bp.apply_q2(np.ones((2, 2, 3, 2)), a, b)
+    far = [n for n in gp.nodes if (a, n) not in gp.edges and (n, a) not in gp.edges and n != a][0]
+    with pytest.raises(ValueError):
+        bp.apply_q2(np.eye(4).reshape(2, 2, 2, 2), a, far)
+    # a rejected gate leaves the state usable
+    bp.apply_q2(np.eye(4).reshape(2, 2, 2, 2), a, b)
+    assert abs(float(_host(bp.dens_q1(a))[0, 0].real) - 1.0) < 1e-9
+
+
+def test_batched_measure_leaves_the_other_instances_alone():
+    """Measuring a node of instance 0 of a two-instance BatchedBPEmulator
+    re-canonicalises instance 0 only: instance 1's one- and two-site density
+    matrices stay equal to the oracle emulator's for instance 1, and instance
+    0 follows the oracle's measurement flow."""
+    from dequantizer_amd.emulator import BatchedBPEmulator
+    graphs = [_build(dq.TensorGraph(), og.small_heavy_hex) for _ in range(2)]
+    keys = [dq.PRNGKey(11), dq.PRNGKey(12)]
+    be = BatchedBPEmulator(graphs, keys, 8, [g.edges_number for g in graphs], belief_propagation_accuracy=1e-8)
+    bo = [oe.BPQuantumEmulator(og.small_heavy_hex(), k, 8, graphs[0].edges_number, belief_propagation_accuracy=1e-8)
+          for k in keys]
+    trav = list(graphs[0].get_traversal_iterator())
+    for inst in range(2):
+        _layer([(be, inst), (bo[inst], None)], trav, np.random.default_rng(20 + inst))
+    nodes = list(graphs[0].nodes)
+    edges = list(graphs[0].edges)
+    before1 = _host(be.rdm1(1, nodes))
+    r = be.measure(0, nodes[0])
+    assert r == bo[0].measure(nodes[0])
+    after1 = _host(be.rdm1(1, nodes))
+    assert np.abs(after1 - before1).max() < 1e-12
+    for k, n in enumerate(nodes):
+        assert np.abs(after1[k] - bo[1].dens_q1(n)).max() < 1e-9, n
+    r2 = _host(be.rdm2(1, edges))
+    for k, e in enumerate(edges):
+        assert np.abs(r2[k] - bo[1].dens_q2(*e)).max() < 1e-9, e
+    r0 = _host(be.rdm1(0, nodes))
+    for k, n in enumerate(nodes):
+        assert np.abs(r0[k] - bo[0].dens_q1(n)).max() < 1e-9, n

@@ -107,3 +107,74 @@ def test_matrix_core_sweep_beyond_the_gathered_metadata():
         ref = out[_lib.DQ_C128][k].view(-1, chi * chi)
         rel = (got - ref).abs().pow(2).sum(1).sqrt() / ref.abs().pow(2).sum(1).sqrt()
         assert float(rel.max()) < 2e-5, (k, float(rel.max()))
+
+
+def _set_schedule(h, sched):
+    import ctypes
+    L = _lib.lib()
+    n_levels, level_ptr, level_nodes, node_src, edge_src = sched
+    ip, ucp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_ubyte)
+    lp = np.ascontiguousarray(level_ptr, dtype=np.int32)
+    ln = np.ascontiguousarray(level_nodes, dtype=np.int32)
+    ns = np.ascontiguousarray(node_src, dtype=np.uint8)
+    es = np.ascontiguousarray(edge_src, dtype=np.uint8)
+    _lib.check(L.dq_bp_set_schedule(h.ptr, n_levels, lp.ctypes.data_as(ip), ln.ctypes.data_as(ip),
+                                    ns.ctypes.data_as(ucp), es.ctypes.data_as(ucp)))
+
+
+@pytest.mark.parametrize("chi,dtype", [(16, torch.complex64), (4, torch.complex128)])
+def test_sweep_graph_after_schedule_change(chi, dtype):
+    """dq_bp_sweeps captured under the synchronous schedule, then the same
+    handle switched to the asynchronous (levelled) schedule and dq_bp_sweeps
+    called again with the same buffers: the second call must run the new
+    schedule (equal to eager dq_bp_sweep under it), not replay the graph
+    captured for the old one.  Then back to the synchronous schedule."""
+    import dequantizer_amd as dq
+    import networkx as nx
+    from dequantizer_amd.engine import Topology
+    g = dq.TensorGraph()
+    for _ in range(24):
+        g.add_node(2)
+    for a, b in nx.random_regular_graph(3, 24, seed=2).edges:
+        g.add_edge((a, b), chi)
+    topo = Topology([list(g.get_traversal_iterator(ordering="dfs"))])
+    h = topo.handle(chi, 2, True)
+    code = _lib.DQ_C64 if dtype == torch.complex64 else _lib.DQ_C128
+    L, st = _lib.lib(), _lib.stream_handle()
+    gen = torch.Generator(device=DEV).manual_seed(3)
+    T = torch.randn(h.tensor_elems, dtype=dtype, device=DEV, generator=gen)
+    nm = 2 * topo.n_edges * chi * chi
+    a = torch.randn((2 * topo.n_edges, chi, chi), dtype=dtype, device=DEV, generator=gen)
+    m0 = (a @ a.conj().transpose(1, 2)).reshape(-1)
+    bufs = [torch.zeros(nm, dtype=dtype, device=DEV) for _ in range(4)]
+    n_sweeps = 3
+
+    def eager():
+        cur = [m0.clone(), m0.clone()]
+        for _ in range(n_sweeps):
+            nxt = [torch.zeros_like(m0), torch.zeros_like(m0)]
+            _lib.check(L.dq_bp_sweep(h.ptr, code, _lib.ptr(T), _lib.ptr(cur[0]), _lib.ptr(cur[1]), _lib.ptr(nxt[0]),
+                                     _lib.ptr(nxt[1]), None, st))
+            cur = nxt
+        return cur
+
+    def graph_run():
+        bufs[0].copy_(m0)
+        bufs[1].copy_(m0)
+        bufs[2].zero_()
+        bufs[3].zero_()
+        _lib.check(L.dq_bp_sweeps(h.ptr, code, _lib.ptr(T), _lib.ptr(bufs[0]), _lib.ptr(bufs[1]), _lib.ptr(bufs[2]),
+                                  _lib.ptr(bufs[3]), n_sweeps, None, st))
+        return bufs[2].clone(), bufs[3].clone()
+
+    results = {}
+    for name, sched in (("sync", topo.schedule(True)), ("async", topo.schedule(False)), ("sync2", topo.schedule(True))):
+        _set_schedule(h, sched)
+        got = graph_run()
+        ref = eager()
+        torch.cuda.synchronize()
+        assert torch.equal(got[0], ref[0]) and torch.equal(got[1], ref[1]), name
+        results[name] = got
+    # the two schedules give different iterates (the test would be vacuous otherwise)
+    assert not torch.equal(results["sync"][0], results["async"][0])
+    assert torch.equal(results["sync"][0], results["sync2"][0])
