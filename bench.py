@@ -3,10 +3,12 @@
 Workload (one GPU): N = 1000 spins, random 3-regular graph (1500 edges),
 J in {-1, +1}, h ~ N(0, 0.1^2), transverse-field annealing with dt = 0.1 over a
 200-step linear schedule, chi = 16, complex64 contractions with FP64
-decompositions, regauge (BP to Vidal distance 1e-5, at most 100 sweeps) once
-per layer.  A timed step is one Trotter step: regauge + 1500 simple updates
-+ 1000 one-site gates.  The metric is edge-updates/s (one EU = one directed
-node->edge message recomputation) over the whole timed run.
+decompositions, a regauge of exactly 10 synchronous BP sweeps (then the Vidal
+gauge and distance) before every layer.  A timed step is one whole forward
+run: the engine set up on |+...+>, 200 Trotter steps (regauge + 1500 simple
+updates + 1000 one-site gates each) and the final one- and two-site density
+matrices.  The metric is edge-updates/s (one EU = one directed node->edge
+message recomputation) over the timed runs.
 
 Prints one JSON line.  ``--gpus`` above 1 is refused: this build measures one
 GPU.
@@ -22,6 +24,7 @@ import numpy as np
 
 METRIC = "edge-updates/s & SU gates/s at χ=16 vs plain GPU and CPU; fwd+adjoint time"
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 # HBM bytes of one BP sweep (one node-kernel launch with the synchronous edge
 # update fused in) of the default workload (N=1000, chi=16, complex64), from
@@ -44,11 +47,43 @@ def eu_bytes(d: int, chi: int, p: int, elem: int = 8) -> float:
     return elem * (chi ** d * p + d * chi * chi) / d + elem * chi * chi
 
 
+def su_flops(m_a: int, m_b: int, n: int, k: int) -> float:
+    """Algorithmic FLOPs of one simple update: a reduced QR of each side's
+    (m x n) matrix (2 m n^2 - 2/3 n^3 complex multiply-adds), the SVD of the
+    n x n core (21 n^3 real flops, four times for complex arithmetic) and the
+    two new Gammas (m n k complex multiply-adds per side, k kept values).  At
+    chi = 16, degree 3, gate rank 2 (m = 512, n = 32, k = 16): 23.4 MFLOP."""
+    qr = sum(2.0 * m * n * n - 2.0 / 3.0 * n ** 3 for m in (m_a, m_b))
+    gam = (m_a + m_b) * n * k
+    return 8.0 * (qr + gam) + 84.0 * n ** 3
+
+
+def annealing_run(ef, schedule, args, dtype, sweep_events=None, su_events=None):
+    """One whole forward annealing run (quantum_annealer.py:90-152): the
+    engine set up on |+...+> (Vidal canonicalisation from random messages),
+    every Trotter step of the schedule (regauge, coupling gates, mixing
+    gates), then the one- and two-site density matrices of every spin and
+    edge and the BP energy."""
+    from dequantizer_amd import PRNGKey
+    from dequantizer_amd.annealing import Annealer
+    ann = Annealer([ef], [PRNGKey(args.seed)], max_chi=args.chi, layers_per_regauging=1, accuracy=1e-5,
+                   max_belief_propagation_iterations=100, synchronous_update=args.sync,
+                   traversal_type=args.traversal, dtype=dtype, chi=args.chi,
+                   fixed_sweeps=args.fixed_sweeps or None)
+    ann.emu.sweep_events = sweep_events
+    ann.emu.su_events = su_events
+    for mix, cpl in schedule:
+        ann.step(mix, cpl)
+    res = ann.finish(two_site=True)[0]
+    ann.emu.sweep_events = ann.emu.su_events = None
+    return ann, res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=3, help="timed whole annealing runs")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed whole annealing runs before them")
     ap.add_argument("--nodes", type=int, default=1000)
     ap.add_argument("--chi", type=int, default=16)
     ap.add_argument("--schedule-steps", type=int, default=200)
@@ -60,6 +95,8 @@ def main():
     ap.add_argument("--fixed-sweeps", type=int, default=10,
                     help="BP sweeps per regauge with the convergence test off (fixed work); 0 = converge to 1e-5")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="run the oracle CPU leg (1) or not (0)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="BLAS threads of the CPU leg (0 = the host's usable cores); a 1-thread run follows")
     ap.add_argument("--plain-gpu", type=int, default=1, help="run the torch-eager GPU comparator (1) or not (0)")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="Trotter steps of the forward+adjoint timing (0 skips it)")
@@ -74,12 +111,17 @@ def main():
                     help="L of the periodic L^3 lattice leg (degree 6, chi 16, phys 2, Gaussian tensors): BP to "
                          "1e-8 and the Vidal gauge in complex64 and complex128; 0 skips it")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
-                    help="before the timed steps, one device-to-device copy of this many MiB (a kernel of known "
+                    help="before the timed runs, one device-to-device copy of this many MiB (a kernel of known "
                          "traffic against which hardware byte counters are scaled); 0 skips it")
     ap.add_argument("--exact-qubits", type=int, default=30,
                     help="exact state-vector leg: qubits of a random 3-regular instance (16 * 2^n bytes of HBM); "
                          "0 skips it")
     ap.add_argument("--exact-layers", type=int, default=10, help="timed annealing layers of the exact leg")
+    ap.add_argument("--async-steps", type=int, default=5,
+                    help="Trotter steps timed with the emulator's default asynchronous depth-first BP schedule "
+                         "(0 skips that leg)")
+    ap.add_argument("--ising-cube", type=int, default=1,
+                    help="time the 6^3 open-cube 3D Ising canonicalisation (launch-bound, chi = 2); 0 skips it")
     ap.add_argument("--gate-order", default="colored", choices=["colored", "pairs"],
                     help="coupling-gate order: pair list sorted by edge colour, or the generator's pair order")
     args = ap.parse_args()
@@ -88,32 +130,23 @@ def main():
         print("bench.py measures one GPU only (--gpus 1); runs over several GPUs are not part of this "
               "benchmark", file=sys.stderr)
         sys.exit(2)
+    if args.steps < 1:
+        raise SystemExit("--steps must be at least 1")
 
     import torch
 
-    from dequantizer_amd.annealing import Annealer, edge_colored, get_scheduler, linear_schedule, random_regular
-    from dequantizer_amd import PRNGKey
+    from dequantizer_amd.annealing import edge_colored, get_scheduler, linear_schedule, random_regular
 
     torch.cuda.init()
     ef = random_regular(args.nodes, 3, seed=args.seed, couplings="pm1", field_std=0.1)
     if args.gate_order == "colored":
         ef = edge_colored(ef)
     schedule = get_scheduler(args.dt, linear_schedule, args.schedule_steps)
-    if args.warmup + args.steps > len(schedule):
-        raise SystemExit("warmup + steps exceeds the schedule length")
     t_build = time.perf_counter()
-    ann = Annealer([ef], [PRNGKey(args.seed)], max_chi=args.chi, layers_per_regauging=1, accuracy=1e-5,
-                   max_belief_propagation_iterations=100, synchronous_update=args.sync,
-                   traversal_type=args.traversal, dtype=torch.complex64, chi=args.chi,
-                   fixed_sweeps=args.fixed_sweeps or None)
-    emu = ann.emu
     for k in range(args.warmup):
-        ann.step(*schedule[k])
-        emu._flush()
+        annealing_run(ef, schedule, args, torch.complex64)
         torch.cuda.synchronize()
-        print(f"bench: warmup step {k} done, chi={emu.chi}, bp sweeps so far {sum(emu.bp_iterations[0])}",
-              file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
+        print(f"bench: warmup run {k} done", file=sys.stderr, flush=True)
     t_build = time.perf_counter() - t_build
 
     if args.copy_probe_mb:
@@ -122,37 +155,78 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    it_before = sum(emu.bp_iterations[0])
-    emu.sweep_events = []
+    sweep_events, su_events = [], []
+    runs = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.warmup, args.warmup + args.steps):
-        ann.step(*schedule[k])
-        if k % 2 == 0:
-            print(f"bench: timed step {k} queued", file=sys.stderr, flush=True)
-    emu._flush()
+    for k in range(args.steps):
+        runs.append(annealing_run(ef, schedule, args, torch.complex64, sweep_events, su_events))
+        print(f"bench: timed run {k} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    sweeps = sum(emu.bp_iterations[0]) - it_before
+    elapsed = time.perf_counter() - t0
+    ann, res = runs[-1]
+    emu = ann.emu
+    sweeps = sum(sum(r.bp_iterations) for _, r in runs)
     n_eu = sweeps * 2 * ef.edges_number
-    n_gates = args.steps * ef.edges_number
-    # live per-launch duration of the BP sweep (HIP events on torch's stream)
-    # each entry times one launch of n sweeps (the regauge's sweeps replay as
-    # one captured graph): the per-sweep duration is the total over the sweeps
-    total_s, n_sweeps_timed = 0.0, 0
-    for ev0, ev1, act, n_sw in emu.sweep_events:
-        if (act if isinstance(act, bool) else int(act.sum().item()) > 0):
-            total_s += ev0.elapsed_time(ev1) * 1e-3
-            n_sweeps_timed += n_sw
-    durs = [total_s / n_sweeps_timed] if n_sweeps_timed else []
-    emu.sweep_events = None
+    n_trotter = args.steps * len(schedule)
+    n_gates = n_trotter * ef.edges_number
     d, p, chi = 3, 2, args.chi
+
+    # BP sweeps: HIP events on torch's stream around each launch (a regauge's
+    # fixed sweeps replay as one captured HIP graph); total time / sweeps
+    bp_s, bp_sweeps_timed = 0.0, 0
+    for ev0, ev1, act, n_sw in sweep_events:
+        if (act if isinstance(act, bool) else int(act.sum().item()) > 0):
+            bp_s += ev0.elapsed_time(ev1) * 1e-3
+            bp_sweeps_timed += n_sw
+    sweep_s = bp_s / bp_sweeps_timed if bp_sweeps_timed else float("nan")
     flops_per_sweep = 2 * ef.edges_number * eu_flops(d, chi, p)
     bytes_per_sweep = 2 * ef.edges_number * eu_bytes(d, chi, p)
-    sweep_s = float(np.mean(durs)) if durs else float("nan")
-    achieved_tflops = flops_per_sweep / sweep_s / 1e12 if durs else None
+    bp_tflops = flops_per_sweep / sweep_s / 1e12 if bp_sweeps_timed else None
+    # simple updates: HIP events around each launch (one edge-colour class of
+    # gates per launch); per-launch durations in launch order
+    su_durs = np.asarray([a.elapsed_time(b) * 1e-3 for a, b, _ in su_events])
+    su_counts = np.asarray([c for _, _, c in su_events])
+    su_s = float(su_durs.sum())
+    gate_flops = su_flops(p * chi ** (d - 1), p * chi ** (d - 1), 2 * chi, chi)
+    su_launch_s = float(su_durs.mean()) if len(su_durs) else float("nan")
+    su_gates_per_launch = float(su_counts.mean()) if len(su_counts) else float("nan")
+    su_tflops = su_gates_per_launch * gate_flops / su_launch_s / 1e12 if len(su_durs) else None
+    # the full-rank half of each run (the bond spectra fill around step 100)
+    per_run = len(su_events) // args.steps if args.steps else 0
+    tail = np.concatenate([su_durs[r * per_run + per_run // 2:(r + 1) * per_run] for r in range(args.steps)])
+    tail_n = np.concatenate([su_counts[r * per_run + per_run // 2:(r + 1) * per_run] for r in range(args.steps)])
+    su_tail_tflops = float(tail_n.mean() * gate_flops / tail.mean() / 1e12) if len(tail) else None
+    vd = np.asarray(res.vidal_distances_after_regauging, dtype=np.float64)
+    z = (res.density_matrices[:, 0, 0] - res.density_matrices[:, 1, 1]).real
 
+    bp_roof = {
+        "bound": "mfma",
+        "kernel": "BP sweep (k_bp_nodes_d3c16, the synchronous edge update fused in)",
+        "achieved": bp_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "frac": (bp_tflops / PEAK_FP32_TFLOPS) if bp_tflops else None,
+        "traffic": (TRAFFIC_BP_SWEEP_DEFAULT if (args.nodes, args.chi) == (1000, 16) else None),
+        "traffic_unit": "B per launch",
+        "algorithmic_flops_per_launch": flops_per_sweep, "algorithmic_bytes_per_launch": bytes_per_sweep,
+        "avg_launch_s": sweep_s, "launches": bp_sweeps_timed, "share_of_run": bp_s / elapsed,
+        "timing": "HIP events around each sweep launch on torch's stream (a regauge's fixed sweeps replay as one "
+                  "captured HIP graph), total time divided by the number of sweeps",
+    }
+    su_roof = {
+        "bound": "mfma",
+        "kernel": "simple update (k_su, one edge-colour class of gates per launch)",
+        "achieved": su_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        "frac": (su_tflops / PEAK_FP64_TFLOPS) if su_tflops else None,
+        "traffic": None,
+        "algorithmic_flops_per_launch": su_gates_per_launch * gate_flops,
+        "algorithmic_flops_per_gate": gate_flops,
+        "avg_launch_s": su_launch_s, "launches": int(len(su_durs)), "share_of_run": su_s / elapsed,
+        "achieved_second_half_of_run": su_tail_tflops,
+        "frac_second_half_of_run": (su_tail_tflops / PEAK_FP64_TFLOPS) if su_tail_tflops else None,
+        "timing": "HIP events around each simple-update launch on torch's stream, mean over all launches of the "
+                  "timed runs (FP64 peak: the decompositions run in FP64)",
+    }
+    dominant, secondary = (su_roof, bp_roof) if su_s > bp_s else (bp_roof, su_roof)
     result = {
         "metric": METRIC,
         "value": n_eu / elapsed,
@@ -167,37 +241,42 @@ def main():
         "dtype": "c64",
         "data": "synthetic",
         "config": {
-            "workload": "BP annealing, random 3-regular N=%d (%d edges), chi=%d, dt=%g, %d-step linear schedule, "
-                        "steps %d..%d timed" % (args.nodes, ef.edges_number, chi, args.dt, args.schedule_steps,
-                                               args.warmup, args.warmup + args.steps - 1),
+            "workload": "BP annealing, random 3-regular N=%d (%d edges), chi=%d, dt=%g, %d-step linear schedule; "
+                        "a step is one whole forward run (set-up, %d Trotter steps, final one- and two-site "
+                        "density matrices)" % (args.nodes, ef.edges_number, chi, args.dt, args.schedule_steps,
+                                               args.schedule_steps),
             "nodes": args.nodes, "edges": ef.edges_number, "chi": chi, "phys": p,
             "bp_schedule": ("sync" if args.sync else "async-" + args.traversal),
-            "bp_accuracy": 1e-5, "regauge": "every layer", "gate_order": args.gate_order,
-            "sweeps_per_regauge": (args.fixed_sweeps or "to convergence"),
+            "regauge": "every layer", "gate_order": args.gate_order,
+            "sweeps_per_regauge": (args.fixed_sweeps or "to Vidal distance 1e-5"),
+            "bp_accuracy": ("not enforced (fixed sweeps)" if args.fixed_sweeps else 1e-5),
         },
+        "trotter_steps_timed": n_trotter,
+        "ms_per_trotter_step": 1e3 * elapsed / n_trotter,
         "su_gates_per_s": n_gates / elapsed,
-        "autotune": [ln for ln in emu.tuning_report.strip().split("\n") if ln],
+        "su_gates_per_s_in_launches": n_gates / su_s if su_s else None,
+        "bp_eu_per_s_in_sweeps": (2 * ef.edges_number / sweep_s) if bp_sweeps_timed else None,
         "bp_sweeps_timed": int(sweeps),
-        "bp_eu_per_s_in_sweeps": (2 * ef.edges_number / sweep_s) if durs else None,
+        "vidal_distance_after_fixed_sweeps": {"max": float(vd.max()), "median": float(np.median(vd)),
+                                              "final": float(vd[-1]), "regauges": int(len(vd))},
+        "energy_per_spin_bp": res.bp_energy / ef.nodes_number,
+        "magnetisation": float(z.mean()),
+        "max_truncation_error": float(np.max(res.truncation_errors)),
+        "autotune": [ln for ln in emu.tuning_report.strip().split("\n") if ln],
         "setup_and_warmup_s": t_build,
-        "roofline": {
-            "bound": "mfma",
-            "kernel": "BP sweep (k_bp_nodes_d3c16, the synchronous edge update fused in)",
-            "achieved": achieved_tflops,
-            "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": (achieved_tflops / PEAK_FP32_TFLOPS) if achieved_tflops else None,
-            "traffic": (TRAFFIC_BP_SWEEP_DEFAULT if (args.nodes, args.chi) == (1000, 16) else None),
-            "traffic_unit": "B per launch",
-            "algorithmic_flops_per_launch": flops_per_sweep,
-            "algorithmic_bytes_per_launch": bytes_per_sweep,
-            "avg_launch_s": sweep_s,
-            "launches": n_sweeps_timed,
-            "timing": "HIP events around each sweep launch (a regauge's fixed sweeps replay as one captured HIP "
-                      "graph), total time divided by the number of sweeps",
-        },
+        "roofline": dominant,
+        "roofline_other": secondary,
         "cpu_baseline": None,
     }
+    del runs, ann, emu
+    gc.collect()
+    torch.cuda.empty_cache()
+    if args.async_steps:
+        result["async_dfs"] = async_dfs_run(ef, schedule, args)
+        print(f"bench: asynchronous dfs leg done", file=sys.stderr, flush=True)
+    if args.ising_cube:
+        result["ising_cube"] = ising_cube_run(args)
+        print(f"bench: Ising cube leg done", file=sys.stderr, flush=True)
     if args.batch_instances:
         result["batched"] = batched_run(args)
         print(f"bench: batched run done: {result['batched']['seconds']:.1f} s", file=sys.stderr, flush=True)
@@ -215,19 +294,105 @@ def main():
     if args.adjoint_steps:
         result["fwd_adjoint"] = adjoint_timing(ef, args)
         print(f"bench: forward+adjoint done: {result['fwd_adjoint']['seconds']:.1f} s", file=sys.stderr, flush=True)
-    if args.plain_gpu:
-        result["plain_gpu"] = plain_gpu_baseline(ann, ef, schedule[args.warmup + args.steps], args)
-        result["vs_plain_gpu"] = result["value"] / result["plain_gpu"]["value"]
-        print(f"bench: plain GPU comparator done: {result['plain_gpu']['ms_per_step']:.0f} ms/step", file=sys.stderr,
-              flush=True)
-    if args.cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(ann, ef, schedule[args.warmup + args.steps], args)
-        result["vs_cpu_baseline"] = result["value"] / result["cpu_baseline"]["value"]
-        if args.exact_qubits:
-            cpu_sv = exact_cpu_baseline(args)
-            result["exact_sv"]["cpu_baseline"] = cpu_sv
-            result["exact_sv"]["vs_cpu_baseline"] = result["exact_sv"]["amplitude_layers_per_s"] / cpu_sv["value"]
+    if args.plain_gpu or args.cpu_baseline:
+        # the comparators start from the state of a run stopped halfway (full
+        # bond spectra), and time one more Trotter step
+        half = len(schedule) // 2
+        ann, _ = annealing_run(ef, schedule[:half], args, torch.complex64)
+        nxt = schedule[half]
+        if args.plain_gpu:
+            result["plain_gpu"] = plain_gpu_baseline(ann, ef, nxt, args)
+            result["plain_gpu"]["gpu_ms_per_trotter_step"] = result["ms_per_trotter_step"]
+            result["vs_plain_gpu"] = result["value"] / result["plain_gpu"]["value"]
+            print(f"bench: plain GPU comparator done: {result['plain_gpu']['ms_per_step']:.0f} ms/step",
+                  file=sys.stderr, flush=True)
+        if args.cpu_baseline:
+            result["cpu_baseline"] = cpu_baseline(ann, ef, nxt, args, args.cpu_threads or usable_cores())
+            result["vs_cpu_baseline"] = result["value"] / result["cpu_baseline"]["value"]
+            print(f"bench: CPU leg done", file=sys.stderr, flush=True)
+            result["cpu_baseline_1thread"] = cpu_baseline(ann, ef, nxt, args, 1)
+            print(f"bench: one-thread CPU leg done", file=sys.stderr, flush=True)
+            if args.exact_qubits:
+                cpu_sv = exact_cpu_baseline(args)
+                result["exact_sv"]["cpu_baseline"] = cpu_sv
+                result["exact_sv"]["vs_cpu_baseline"] = result["exact_sv"]["amplitude_layers_per_s"] / cpu_sv["value"]
     print(json.dumps(result))
+
+
+def usable_cores() -> int:
+    """CPUs this process may run on (its affinity mask)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (lscpu), for the CPU legs."""
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def async_dfs_run(ef, schedule, args, warm=5):
+    """The emulator's default BP schedule (asynchronous sweeps in depth-first
+    traversal order, bp_quantum_emulator.py:56-59): every sweep reads the
+    messages already updated in it, which levels the 1000-node graph into
+    about 730 dependent launches per sweep.  Trotter steps ``warm`` onwards
+    are timed with the same fixed sweeps per regauge."""
+    import torch
+    from dequantizer_amd import PRNGKey
+    from dequantizer_amd.annealing import Annealer
+    ann = Annealer([ef], [PRNGKey(args.seed)], max_chi=args.chi, layers_per_regauging=1, accuracy=1e-5,
+                   max_belief_propagation_iterations=100, synchronous_update=False, traversal_type="dfs",
+                   dtype=torch.complex64, chi=args.chi, fixed_sweeps=args.fixed_sweeps or 10)
+    for mix, cpl in schedule[:warm]:
+        ann.step(mix, cpl)
+    ann.emu._flush()
+    torch.cuda.synchronize()
+    it0 = sum(ann.emu.bp_iterations[0])
+    t0 = time.perf_counter()
+    for mix, cpl in schedule[warm: warm + args.async_steps]:
+        ann.step(mix, cpl)
+    ann.emu._flush()
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t0
+    sweeps = sum(ann.emu.bp_iterations[0]) - it0
+    n_levels = ann.emu.topo.schedule(False)[0]
+    return {"steps": args.async_steps, "seconds": seconds, "ms_per_trotter_step": 1e3 * seconds / args.async_steps,
+            "sweeps": int(sweeps), "levels_per_sweep": int(n_levels),
+            "value": sweeps * 2 * ef.edges_number / seconds, "unit": "edge-updates/s"}
+
+
+def ising_cube_run(args):
+    """The 3D classical Ising square-root network on the open 6^3 cube (chi =
+    2, 216 nodes, 540 edges; examples/3d_ising_canonicalization.py:89-160):
+    asynchronous depth-first BP to a message residual of 1e-10, then the
+    Vidal gauge and the mean magnetisation, for beta = 0.1 .. 0.6.  At chi = 2
+    every launch is latency-bound: the figure is time per sweep, not a
+    roofline fraction."""
+    import torch
+    from dequantizer_amd.drivers import process_ising_sqrt_partition_network
+    betas = [0.1, 0.2, 0.3, 0.4, 0.5, 0.6]
+    process_ising_sqrt_partition_network(3, 6, betas[:1], -1.0, 0.5, 1e-10, True, criterion="residual",
+                                         max_iterations=500)  # set-up of the handles and schedules
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = process_ising_sqrt_partition_network(3, 6, betas, -1.0, 0.5, 1e-10, True, criterion="residual",
+                                               max_iterations=500)
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t0
+    sweeps = int(sum(res.number_of_iterations))
+    return {"L": 6, "nodes": 216, "edges": 540, "chi": 2, "betas": betas, "seconds": seconds, "sweeps": sweeps,
+            "ms_per_sweep": 1e3 * seconds / sweeps, "iterations": [int(x) for x in res.number_of_iterations],
+            "magnetisation": [round(float(m), 10) for m in res.average_magnetization],
+            "note": "each sweep includes the host-side residual check (one device read-back per sweep)"}
 
 
 def batched_run(args):
@@ -662,10 +827,13 @@ def plain_gpu_baseline(ann, ef, step_times, args, sample_sweeps=2, sample_gates=
                           1e3 * t_gate, topo.n_nodes, t_apply1, sweeps, ef.edges_number)}
 
 
-def cpu_baseline(ann, ef, step_times, args):
-    """The oracle (NumPy complex128 restatement) on one Trotter step of the
-    same instance, started from the GPU state: regauge + all simple updates
-    + mixing gates.  Reports EU/s = oracle BP EU in that step / its time."""
+def cpu_baseline(ann, ef, step_times, args, threads: int):
+    """The oracle (NumPy complex128 restatement; the reference computes in
+    complex128) on one Trotter step of the same instance, started from the
+    GPU state: regauge + all simple updates + mixing gates, with the BLAS /
+    OpenMP thread pools limited to ``threads``.  Reports EU/s = oracle BP EU in
+    that step / its time."""
+    from threadpoolctl import threadpool_limits
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import torch
     from oracle import annealing as oa
@@ -710,21 +878,23 @@ def cpu_baseline(ann, ef, step_times, args):
     o.fixed_sweeps = args.fixed_sweeps or None
     mix, cpl = step_times
     deg = [g.nodes[i].degree for i in range(ef.nodes_number)]
-    t0 = time.perf_counter()
-    for (a, b), jv in zip(ef.coupled_spin_pairs, ef.coupling_amplitudes):
-        a, b = int(a), int(b)
-        o.apply_q2(oa.coupling_gate(jv, ef.fields[a], ef.fields[b], deg[a], deg[b], cpl), a, b)
-    mg = oa.mixing_gate(mix)
-    for i in range(ef.nodes_number):
-        o.apply_q1(mg, i)
-    dt = time.perf_counter() - t0
+    with threadpool_limits(limits=threads):
+        t0 = time.perf_counter()
+        for (a, b), jv in zip(ef.coupled_spin_pairs, ef.coupling_amplitudes):
+            a, b = int(a), int(b)
+            o.apply_q2(oa.coupling_gate(jv, ef.fields[a], ef.fields[b], deg[a], deg[b], cpl), a, b)
+        mg = oa.mixing_gate(mix)
+        for i in range(ef.nodes_number):
+            o.apply_q1(mg, i)
+        dt = time.perf_counter() - t0
     sweeps = sum(o.bp_iterations)
     eu = sweeps * 2 * ef.edges_number
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": eu / dt, "unit": "edge-updates/s", "cores": threads, "kind": "port",
-            "sample": "one Trotter step of the same instance on the oracle (NumPy complex128), started from the "
-                      "GPU state after the timed steps: regauge (%d BP sweeps) + %d simple updates + %d one-site "
-                      "gates, %.1f s" % (sweeps, ef.edges_number, ef.nodes_number, dt),
+            "cpu": cpu_model(), "host_cores_usable": usable_cores(), "dtype": "c128",
+            "sample": "one Trotter step of the same instance on the oracle (NumPy complex128, %d BLAS threads), "
+                      "started from the GPU state halfway through the run (full bond spectra): regauge (%d BP "
+                      "sweeps) + %d simple updates + %d one-site gates, %.1f s" % (
+                          threads, sweeps, ef.edges_number, ef.nodes_number, dt),
             "seconds": dt, "gates_per_s": ef.edges_number / dt}
 
 
@@ -750,8 +920,8 @@ def exact_cpu_baseline(args, n=22):
     for q in range(n):
         ex.apply_q1(mg, q)
     dt = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": 2 ** n / dt, "unit": "amplitude-layers/s", "cores": threads, "kind": "port",
+    threads = usable_cores()
+    return {"value": 2 ** n / dt, "unit": "amplitude-layers/s", "cores": threads, "kind": "port", "cpu": cpu_model(),
             "sample": "one annealing layer (%d coupling gates, %d mixing gates) of a %d-qubit random 3-regular "
                       "instance on the oracle's state vector (NumPy complex128 tensordot), %.1f s"
                       % (ef.edges_number, n, n, dt),

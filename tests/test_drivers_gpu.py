@@ -115,13 +115,47 @@ def test_eagle_dynamics_matches_oracle(colour_layers):
     assert np.abs(np.asarray(z) - np.asarray(z_o)).max() < 1e-8, (z, z_o)
 
 
-def test_eagle_chi32_complex64_runs():
-    """The C2 shape (127 qubits, chi=32, complex64 bulk, three colour layers
-    each followed by a regauge) for a few steps: finite, bounded <Z>."""
+def _eagle_fixture():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "eagle_chi32.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("mode", ["edge_order", "colour_layers"])
+def test_eagle_chi32_complex128_matches_oracle(mode):
+    """The 127-qubit kicked-Ising run at max_chi = 32 in complex128 against
+    the oracle emulator's <Z_62> trajectory (tests/golden/eagle_chi32.json,
+    written by tests/golden/make_eagle_chi32.py): the bonds reach 32 at step 5
+    and from step 6 on every simple update truncates a 64-value spectrum to
+    32 (the chi * phys = 64 storage path), in the reference's edge order and
+    in three colour layers with a regauge after each."""
     import torch
-    z = drivers.run_eagle_simulation(32, 1, 4, 1e-5, 0.8, is_infinite=False, dtype=torch.complex64,
-                                     colour_layers=True, max_belief_propagation_iterations=100)
-    assert len(z) == 5 and all(np.isfinite(z)) and all(abs(v) <= 1.0 + 1e-5 for v in z)
+    fx = _eagle_fixture()
+    want = fx[mode]["z62"]
+    steps = len(want) - 1
+    print(f"eagle chi=32 c128 {mode}, {steps} steps", flush=True)
+    z = drivers.run_eagle_simulation(32, 1, steps, fx["accuracy"], fx["theta"], is_infinite=False,
+                                     dtype=torch.complex128, colour_layers=(mode == "colour_layers"))
+    err = np.abs(np.asarray(z) - np.asarray(want))
+    print("max |dZ| per step", err, flush=True)
+    assert err.max() < 1e-6, (z, want)
+
+
+def test_eagle_chi32_complex64_vs_complex128_all_steps():
+    """The C2 configuration (127 qubits, chi=32, complex64 bulk, three colour
+    layers each followed by a regauge, 20 steps): <Z_62> within 1e-5 of the
+    complex128 run at every step."""
+    import torch
+    runs = {}
+    for dtype in (torch.complex64, torch.complex128):
+        print(f"eagle chi=32 {dtype}, 20 steps", flush=True)
+        runs[dtype] = np.asarray(drivers.run_eagle_simulation(32, 1, 20, 1e-5, 0.8, is_infinite=False, dtype=dtype,
+                                                              colour_layers=True))
+    err = np.abs(runs[torch.complex64] - runs[torch.complex128])
+    print("max |dZ| c64 vs c128:", float(err.max()), flush=True)
+    assert err.max() < 1e-5, err
 
 
 def test_annealing_instance_set_batched_step():

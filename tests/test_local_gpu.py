@@ -186,6 +186,40 @@ def test_simple_update_matches_oracle(dims1, dims2, i1, i2, accuracy, max_rank, 
     assert _rel(_theta(host(tensors[a]), host(tensors[b]), lam, i1, i2), _theta(o1, o2, ol, i1, i2)) < tol
 
 
+@pytest.mark.parametrize("dims1,dims2,i1,i2", [
+    ((32, 32, 32), (32, 32, 32), 2, 0),  # the degree-3 nodes of the 127-qubit heavy-hex lattice
+    ((32, 32), (32, 32, 32), 1, 0),  # a degree-2 bridge node against a degree-3 node
+    ((32, 32), (32, 32), 0, 1),
+])
+@pytest.mark.parametrize("max_rank", [32, 20])
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-9), (np.complex64, 2e-4)])
+def test_simple_update_chi32_matches_oracle(dims1, dims2, i1, i2, max_rank, dtype, tol):
+    """chi = 32 with a rank-2 gate: a 64 x 64 core (chi * phys = 64 storage),
+    truncated to max_rank, against the oracle's QR-based update
+    (tensor_ops.py:265-295,318-389; a 2048 x 64 QR per degree-3 side)."""
+    rng = np.random.default_rng(11)
+    t1, t2, lam1, lam2 = _su_case(rng, dims1, dims2, i1, i2)
+    g, a, b, leaves = _graph_for(dims1, dims2, i1, i2)
+    lambdas = {}
+    for node, lams, shared in ((a, lam1, i1), (b, lam2, i2)):
+        for k, lm in enumerate(lams):
+            lambdas[(a, b) if k == shared else leaves[(node, k)]] = lm
+    tensors = {a: t1.astype(dtype), b: t2.astype(dtype)}
+    # the kicked-Ising coupling RZZ(-pi/2) (operator-Schmidt rank 2)
+    gate = np.diag(np.exp(1j * np.pi / 4 * np.array([1, -1, -1, 1]))).reshape(2, 2, 2, 2)
+    tensors, lambdas, err, ent = g.apply2_to_vidal_canonical(a, b, tensors, lambdas, gate, 1e-10, None, max_rank)
+    h1, h2 = ot.decompose_gate(gate, 1e-10)
+    o1, o2, ol, oerr, oent = ot.simple_update(t1, t2, lam1, lam2, h1, h2, i1, i2, 1e-10, None, max_rank)
+    lam = host(lambdas[(a, b)])
+    assert lam.shape == ol.shape == (max_rank,)
+    assert tuple(host(tensors[a]).shape) == o1.shape and tuple(host(tensors[b]).shape) == o2.shape
+    assert np.abs(lam - ol).max() < tol * max(1.0, np.abs(ol).max())
+    assert oerr > 1e-3  # the cap truncates: the error is a real check
+    assert abs(float(host(err)) - oerr) < tol
+    assert abs(float(host(ent)) - oent) < tol
+    assert _rel(_theta(host(tensors[a]), host(tensors[b]), lam, i1, i2), _theta(o1, o2, ol, i1, i2)) < tol
+
+
 def test_truncate_matches_oracle():
     rng = np.random.default_rng(3)
     gp, go = dq.get_nd_lattice([2, 3], 2, 4), og.nd_lattice([2, 3], 2, 4)

@@ -252,3 +252,28 @@ def test_simple_update_planted_mistakes_are_caught():
     assert calls["n"] > 0
     assert abs(bad_err - want_err) > 1e-3
     assert err == pytest.approx(want_err, abs=1e-12) and ent == pytest.approx(want_ent, abs=1e-12)
+
+
+# ------------------------------------------------------- golden fixtures
+def _load_script(name):
+    import importlib.util
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)
+    spec = importlib.util.spec_from_file_location(name[:-3], path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_eagle_fixture_matches_the_oracle_on_its_first_steps():
+    """tests/golden/eagle_chi32.json holds the oracle's <Z_62> trajectory; its
+    first four steps (bonds up to 16, about a second of oracle work) are
+    recomputed here."""
+    import json
+    import os
+    mod = _load_script("make_eagle_chi32.py")
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "eagle_chi32.json")) as f:
+        fx = json.load(f)
+    for mode, colour in (("edge_order", False), ("colour_layers", True)):
+        z, _ = mod.run(4, colour)
+        assert np.abs(np.asarray(z) - np.asarray(fx[mode]["z62"][:5])).max() < 1e-12, mode
