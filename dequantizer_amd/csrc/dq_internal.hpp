@@ -67,6 +67,23 @@ __device__ __forceinline__ cx<R> cdiv(cx<R> a, cx<R> b) {
 }
 
 typedef cx<double> cd;
+// Complex doubles and ints in LDS, for device functions the compiler does not
+// inline: through a generic pointer their LDS accesses become flat memory
+// instructions (slower, and waited on with the vector-memory counter).
+typedef __attribute__((address_space(3))) cd lds_cd;
+typedef __attribute__((address_space(3))) int lds_int;
+
+// Element load / store of a complex double through a pointer in any address
+// space (an LDS-qualified struct has no usable copy constructor or assignment).
+template <typename P>
+__device__ __forceinline__ cd cld(const P* p, long long i) {
+  return mk<double>(p[i].re, p[i].im);
+}
+template <typename P>
+__device__ __forceinline__ void cst(P* p, long long i, cd v) {
+  p[i].re = v.re;
+  p[i].im = v.im;
+}
 
 // Graph descriptor passed by value to kernels (all pointers are device memory).
 struct DevGraph {

@@ -612,16 +612,16 @@ __device__ inline double wave_nuclear_norm_n(cd* G) {
 // Block-parallel fixed-size one-sided Jacobi SVD (N = 32, 256 threads): the
 // 16 pairs of a round run on 16-lane groups (4 per wave), 2 rows per lane, one
 // block barrier per round.  Same outputs as jacobi_svd.  scal: N doubles.
-template <int N>
-__device__ inline void block_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, int* perm, double* scal) {
+template <int N, typename P>
+__device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int* perm, double* scal) {
   constexpr int NP = N / 2;
   constexpr int LPP = 256 / NP;  // 16 lanes per pair for N = 32
   constexpr int RPL = N / LPP;   // rows per lane
   static_assert(LPP <= 64 && RPL >= 1, "block SVD sizing");
   const int k = threadIdx.x / LPP, sub = threadIdx.x % LPP;
-  for (int i = threadIdx.x; i < N * N; i += blockDim.x) V[i] = mk<double>((i / N) == (i % N) ? 1.0 : 0.0, 0.0);
+  for (int i = threadIdx.x; i < N * N; i += blockDim.x) cst(V, i, mk<double>((i / N) == (i % N) ? 1.0 : 0.0, 0.0));
   double f = 0.0;
-  for (int i = threadIdx.x; i < N * N; i += blockDim.x) f += norm2(G[i]);
+  for (int i = threadIdx.x; i < N * N; i += blockDim.x) f += norm2(cld(G, i));
   f = block_sum(f, scal);
   const double abs_tol = 1e-32 * f;
   for (int sweep = 0; sweep < 60; ++sweep) {
@@ -632,10 +632,10 @@ __device__ inline void block_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, in
       cd x[RPL], y[RPL], vx[RPL], vy[RPL];
 #pragma unroll
       for (int t = 0; t < RPL; ++t) {
-        x[t] = G[p * N + sub + t * LPP];
-        y[t] = G[q * N + sub + t * LPP];
-        vx[t] = V[p * N + sub + t * LPP];
-        vy[t] = V[q * N + sub + t * LPP];
+        x[t] = cld(G, p * N + sub + t * LPP);
+        y[t] = cld(G, q * N + sub + t * LPP);
+        vx[t] = cld(V, p * N + sub + t * LPP);
+        vy[t] = cld(V, q * N + sub + t * LPP);
       }
       double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
 #pragma unroll
@@ -656,11 +656,11 @@ __device__ inline void block_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, in
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
           const cd ey = e * y[t];
-          G[p * N + sub + t * LPP] = mk<double>(c * x[t].re - s * ey.re, c * x[t].im - s * ey.im);
-          G[q * N + sub + t * LPP] = mk<double>(s * x[t].re + c * ey.re, s * x[t].im + c * ey.im);
+          cst(G, p * N + sub + t * LPP, mk<double>(c * x[t].re - s * ey.re, c * x[t].im - s * ey.im));
+          cst(G, q * N + sub + t * LPP, mk<double>(s * x[t].re + c * ey.re, s * x[t].im + c * ey.im));
           const cd evy = e * vy[t];
-          V[p * N + sub + t * LPP] = mk<double>(c * vx[t].re - s * evy.re, c * vx[t].im - s * evy.im);
-          V[q * N + sub + t * LPP] = mk<double>(s * vx[t].re + c * evy.re, s * vx[t].im + c * evy.im);
+          cst(V, p * N + sub + t * LPP, mk<double>(c * vx[t].re - s * evy.re, c * vx[t].im - s * evy.im));
+          cst(V, q * N + sub + t * LPP, mk<double>(s * vx[t].re + c * evy.re, s * vx[t].im + c * evy.im));
         }
       }
       __syncthreads();
@@ -669,7 +669,7 @@ __device__ inline void block_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, in
   }
   if (threadIdx.x < N) {
     double nrm = 0.0;
-    for (int i = 0; i < N; ++i) nrm += norm2(G[threadIdx.x * N + i]);
+    for (int i = 0; i < N; ++i) nrm += norm2(cld(G, threadIdx.x * N + i));
     scal[threadIdx.x] = sqrt(nrm);
   }
   __syncthreads();
@@ -684,7 +684,7 @@ __device__ inline void block_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, in
   for (int t = threadIdx.x; t < N * N; t += blockDim.x) {
     const int j = t / N, i = t % N;
     const double sg = sigma[j];
-    const cd g = G[perm[j] * N + i];
+    const cd g = cld(G, perm[j] * N + i);
     U[j * N + i] = sg > 0.0 ? mk<double>(g.re / sg, g.im / sg) : mk<double>(0.0, 0.0);
   }
   __syncthreads();
@@ -783,6 +783,357 @@ __device__ inline void pivoted_cholesky(cd* G, int n, cd* R, int* piv, double* s
     if (m >= i) R[i * n + piv[m]] = m == i ? conj(G[piv[i] * n + piv[i]]) : conj(G[piv[m] * n + piv[i]]);
   }
   __syncthreads();
+}
+
+// Value of lane `src` in every lane (a scalar read of a vector register).
+__device__ __forceinline__ double readlane_d(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Pivoted Cholesky R^H R = G of an n x n Hermitian PSD G (n <= 32, row-major
+// in LDS) on ONE wavefront, left-looking, with no block barrier: the same
+// factorisation, pivot rule (largest remaining diagonal, ties to the smaller
+// index) and stopping tolerance (n 2.3e-16 max diag) as pivoted_cholesky.
+// Lane (i, h) = (lane % 32, lane / 32) owns row i; the remaining diagonal
+// lives in registers, so the pivot search is a DPP argmax and two lane reads.
+// Step k computes column k of L for every row i not chosen yet,
+//   L[i][k] = (G[i][j] - sum_{m<k} L[i][m] conj(L[j][m])) / L[j][k],
+// the two lane halves summing alternate m.  Column k is stored over row j of
+// G (a chosen pivot's row is never read again; G[i][j] = conj(G[j][i]) is
+// read from row j before it is overwritten), so row piv[m] of G ends as
+// column m of L.  Outputs: piv (n ints: the pivots, then the unchosen
+// indices ascending) and R (n x n row-major, any memory) with
+// R[m][c] = conj(L[c][m]) for m < rank, zero rows after.  Returns the rank.
+// Call with all 64 lanes of one wave.
+template <typename P, typename I>
+__device__ __attribute__((noinline)) int wave_pivoted_cholesky(P* G, int n, cd* R, I* piv) {
+  const int lane = threadIdx.x & 63;
+  const int i = lane & 31, h = lane >> 5;
+  const bool row = i < n;
+  double di = row ? cld(G, i * n + i).re : 0.0;
+  double mx = h == 0 && row ? di : 0.0;
+  mx = fmax(mx, dpp_move<0xB1>(mx));
+  mx = fmax(mx, dpp_move<0x4E>(mx));
+  mx = fmax(mx, dpp_move<0x141>(mx));
+  mx = fmax(mx, dpp_move<0x140>(mx));
+  mx = fmax(readlane_d(mx, 0), readlane_d(mx, 16));
+  const double tol = n * 2.3e-16 * mx;
+  unsigned done = 0u;
+  int k = 0;
+  auto take = [](double& best, int& j, double ob, int oj) {
+    if (ob > best || (ob == best && oj < j)) {
+      best = ob;
+      j = oj;
+    }
+  };
+  for (; k < n; ++k) {
+    const bool mine = row && !((done >> i) & 1u);
+    double best = h == 0 && mine ? di : -1.0;
+    int j = h == 0 && mine ? i : n;
+    take(best, j, dpp_move<0xB1>(best), dpp_move_int<0xB1>(j));
+    take(best, j, dpp_move<0x4E>(best), dpp_move_int<0x4E>(j));
+    take(best, j, dpp_move<0x141>(best), dpp_move_int<0x141>(j));
+    take(best, j, dpp_move<0x140>(best), dpp_move_int<0x140>(j));
+    {
+      const double b1 = readlane_d(best, 16);
+      const int j1 = __builtin_amdgcn_readlane(j, 16);
+      best = readlane_d(best, 0);
+      j = __builtin_amdgcn_readlane(j, 0);
+      take(best, j, b1, j1);
+    }
+    if (!(best > tol)) break;
+    const double inv = rsqrt_nr(best);
+    const double lkk = best * inv;
+    // sum over the earlier columns, alternate m on the two halves, two chains
+    // (lanes past the matrix, i >= n, read row 0's entry and discard it)
+    const int ii = row ? i : 0;
+    cd a0 = mk<double>(0.0, 0.0), a1 = a0;
+    int m = h;
+    for (; m + 2 < k; m += 4) {
+      const int p0 = piv[m], p1 = piv[m + 2];
+      fma_acc(a0, cld(G, p0 * n + ii), conj(cld(G, p0 * n + j)));
+      fma_acc(a1, cld(G, p1 * n + ii), conj(cld(G, p1 * n + j)));
+    }
+    if (m < k) {
+      const int p0 = piv[m];
+      fma_acc(a0, cld(G, p0 * n + ii), conj(cld(G, p0 * n + j)));
+    }
+    cd acc = a0 + a1;
+    acc.re += __shfl_xor(acc.re, 32, 64);
+    acc.im += __shfl_xor(acc.im, 32, 64);
+    const cd gij = row ? conj(cld(G, j * n + i)) : mk<double>(0.0, 0.0);
+    cd l = mk<double>(0.0, 0.0);
+    if (i == j)
+      l = mk<double>(lkk, 0.0);
+    else if (mine)
+      l = inv * (gij - acc);
+    // every lane has read row j (above, in program order) before it is written
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (h == 0 && row) cst(G, j * n + i, l);
+    if (lane == 0) piv[k] = j;
+    if (mine && i != j) di -= norm2(l);
+    done |= 1u << j;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (lane == 0) {
+    int q = k;
+    for (int c = 0; c < n; ++c)
+      if (!((done >> c) & 1u)) piv[q++] = c;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (int m = h; m < n; m += 2) {
+    if (row) R[m * n + i] = m < k ? conj(cld(G, piv[m] * n + i)) : mk<double>(0.0, 0.0);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  return k;
+}
+
+// Householder QR with column pivoting, C P = Q R, of an N x N matrix (N even,
+// N <= 32, ROW-major in LDS, overwritten) on ONE wavefront, for the
+// preconditioned one-sided Jacobi SVD (Drmac & Veselic, "New fast and accurate
+// Jacobi SVD algorithm I", SIAM J. Matrix Anal. Appl. 29 (2008)): Jacobi sweeps
+// on R^H converge in far fewer sweeps than on C.  Lane (c, h) = (lane % 32,
+// lane / 32) works on rows h N/2 .. h N/2 + N/2 - 1 of column c, in place in
+// LDS (kept out of registers: the caller runs near its register limit; the
+// row-major layout puts the 32 column lanes on consecutive words).  Step
+// k takes the remaining column of largest norm (ties to the smaller index;
+// norms recomputed from the rows >= k every step), reflects it onto beta e_k
+// with H = I - u u^H, |u|^2 = 2 (beta = -e^{i arg x_k} |x|, as householder_qr),
+// and applies H to the other remaining columns.  Outputs: Hk (N x N
+// column-major, column k = u_k, zero above row k), X = R^H (column-major,
+// X[r N + m] = conj(R[r][m]), R upper triangular in pivot order) and piv
+// (column m of C P is column piv[m] of C).  Q = H_0 H_1 ... H_{N-1}.
+// The same pivoted QR (C P = Q R, same pivot rule, reflectors and outputs as
+// wave_qrcp_rh) for N = 32 on the whole 256-thread block: thread (c, g) =
+// (tid / 8, tid % 8) keeps rows 4g .. 4g + 3 of column c in registers for the
+// whole factorisation, column norms and reflector products are sums over the 8
+// lanes of a column (DPP moves, no cross-wave traffic), and a step costs two
+// block barriers: one after every wave has posted its best remaining column
+// (and each column its row-k entry), one after the pivot column has posted its
+// reflector.  C (row-major) is only read.  Call with all 256 threads.
+template <typename P, typename I>
+__device__ inline void block_qrcp_rh32(const P* C, P* Hk, P* X, I* piv) {
+  constexpr int N = 32;
+  __shared__ double cand_v[4];
+  __shared__ int cand_j[4];
+  __shared__ cd xk_col[N];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int c = t >> 3, g = t & 7, r0 = 4 * g;
+  cd x[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) x[q] = cld(C, (r0 + q) * N + c);
+  unsigned done = 0u;
+  int mypos = N;
+  auto take = [](double& best, int& j, double ob, int oj) {
+    if (ob > best || (ob == best && oj < j)) {
+      best = ob;
+      j = oj;
+    }
+  };
+  for (int k = 0; k < N; ++k) {
+    const bool mine = !((done >> c) & 1u);
+    double part = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) part += r0 + q >= k ? norm2(x[q]) : 0.0;
+    const double nrm2 = group_sum<8>(part);
+    // this column's row-k entry, for the pivot's alpha
+    if (g == (k >> 2)) {
+      cd v = x[0];
+      if ((k & 3) == 1) v = x[1];
+      if ((k & 3) == 2) v = x[2];
+      if ((k & 3) == 3) v = x[3];
+      cst(xk_col, c, v);
+    }
+    // argmax over the wave's 8 columns: the two columns of each 16-lane row
+    // meet by a row mirror, then the four rows by lane reads
+    double best = mine ? nrm2 : -1.0;
+    int j = mine ? c : N;
+    take(best, j, dpp_move<0x140>(best), dpp_move_int<0x140>(j));
+    {
+      double bv = readlane_d(best, 0);
+      int bj = __builtin_amdgcn_readlane(j, 0);
+#pragma unroll
+      for (int rw = 1; rw < 4; ++rw) take(bv, bj, readlane_d(best, 16 * rw), __builtin_amdgcn_readlane(j, 16 * rw));
+      if (lane == 0) {
+        cand_v[w] = bv;
+        cand_j[w] = bj;
+      }
+    }
+    __syncthreads();
+    best = cand_v[0];
+    j = cand_j[0];
+#pragma unroll
+    for (int q = 1; q < 4; ++q) take(best, j, cand_v[q], cand_j[q]);
+    const cd alpha = cld(xk_col, j);
+    const double nrm = best > 0.0 ? sqrt(best) : 0.0;
+    const double aa = sqrt(norm2(alpha));
+    cd beta = mk<double>(0.0, 0.0);
+    double scale = 0.0;
+    if (best > 0.0) {
+      beta = aa > 0.0 ? mk<double>(-alpha.re / aa * nrm, -alpha.im / aa * nrm) : mk<double>(-nrm, 0.0);
+      scale = sqrt(2.0) / sqrt(2.0 * nrm * (nrm + aa));
+    }
+    if (c == j) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = r0 + q;
+        cst(Hk, k * N + r, r >= k ? scale * (r == k ? x[q] - beta : x[q]) : mk<double>(0.0, 0.0));
+        x[q] = r < k ? x[q] : (r == k ? beta : mk<double>(0.0, 0.0));
+      }
+      mypos = k;
+    }
+    __syncthreads();
+    if (mine && c != j && best > 0.0) {
+      cd u[4];
+      double dr = 0.0, di = 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        u[q] = cld(Hk, k * N + r0 + q);
+        dr += u[q].re * x[q].re + u[q].im * x[q].im;
+        di += u[q].re * x[q].im - u[q].im * x[q].re;
+      }
+      // u is zero above row k, so every row takes the same update
+      const cd dot = mk<double>(group_sum<8>(dr), group_sum<8>(di));
+#pragma unroll
+      for (int q = 0; q < 4; ++q) x[q] = x[q] - u[q] * dot;
+    }
+    done |= 1u << j;
+    if (t == 0) piv[k] = j;
+  }
+  // X = R^H: column r of X holds conj(R[r][m]) at row m, R[r][m] = row r of
+  // column piv[m], zero below the diagonal of R
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = r0 + q;
+    cst(X, r * N + mypos, r <= mypos ? conj(x[q]) : mk<double>(0.0, 0.0));
+  }
+  __syncthreads();
+}
+
+template <int N, typename P, typename I>
+__device__ inline void wave_qrcp_rh(P* C, P* Hk, P* X, I* piv) {
+  static_assert(N % 8 == 0 && N <= 32, "wave QR sizing");
+  constexpr int HALF = N / 2;
+  const int lane = threadIdx.x & 63;
+  const int c = lane & 31, h = lane >> 5;
+  const bool col = c < N;
+  const int cc = col ? c : 0;  // lanes past the matrix read column 0 and discard it
+  const int r0 = h * HALF;
+  unsigned done = 0u;
+  int mypos = N;
+  auto take = [](double& best, int& j, double ob, int oj) {
+    if (ob > best || (ob == best && oj < j)) {
+      best = ob;
+      j = oj;
+    }
+  };
+#pragma unroll 1
+  for (int k = 0; k < N; ++k) {
+    const bool mine = col && !((done >> c) & 1u);
+    // rows are loaded unconditionally and masked, so the loads issue back to
+    // back instead of one branch (and one LDS round trip) per row
+    // batches of 4 rows bound the registers held (the caller runs near its
+    // limit)
+    double s = 0.0;
+#pragma unroll 1
+    for (int b = 0; b < HALF; b += 4) {
+#pragma unroll
+      for (int t = b; t < b + 4; ++t) {
+        const double v = norm2(cld(C, (r0 + t) * N + cc));
+        s += r0 + t >= k ? v : 0.0;
+      }
+    }
+    s += __shfl_xor(s, 32, 64);
+    double best = h == 0 && mine ? s : -1.0;
+    int j = h == 0 && mine ? c : N;
+    take(best, j, dpp_move<0xB1>(best), dpp_move_int<0xB1>(j));
+    take(best, j, dpp_move<0x4E>(best), dpp_move_int<0x4E>(j));
+    take(best, j, dpp_move<0x141>(best), dpp_move_int<0x141>(j));
+    take(best, j, dpp_move<0x140>(best), dpp_move_int<0x140>(j));
+    {
+      const double b1 = readlane_d(best, 16);
+      const int j1 = __builtin_amdgcn_readlane(j, 16);
+      best = readlane_d(best, 0);
+      j = __builtin_amdgcn_readlane(j, 0);
+      take(best, j, b1, j1);
+    }
+    const cd alpha = cld(C, k * N + j);
+    const double nrm = best > 0.0 ? sqrt(best) : 0.0;
+    const double aa = sqrt(norm2(alpha));
+    cd beta = mk<double>(0.0, 0.0);
+    double scale = 0.0;
+    if (best > 0.0) {
+      beta = aa > 0.0 ? mk<double>(-alpha.re / aa * nrm, -alpha.im / aa * nrm) : mk<double>(-nrm, 0.0);
+      scale = sqrt(2.0) / sqrt(2.0 * nrm * (nrm + aa));
+    }
+    // every lane has read column j (norms, alpha) before it is rewritten
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (c == j) {
+      // the reflector, and the pivot column becomes beta e_k from row k down
+#pragma unroll 1
+      for (int b = 0; b < HALF; b += 4) {
+#pragma unroll
+        for (int t = b; t < b + 4; ++t) {
+          const int r = r0 + t;
+          const cd x = cld(C, r * N + j);
+          cst(Hk, k * N + r, r >= k ? scale * (r == k ? x - beta : x) : mk<double>(0.0, 0.0));
+          cst(C, r * N + j, r < k ? x : (r == k ? beta : mk<double>(0.0, 0.0)));
+        }
+      }
+      mypos = k;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (mine && c != j && best > 0.0) {
+      double dr = 0.0, di = 0.0;
+#pragma unroll 1
+      for (int b = 0; b < HALF; b += 4) {
+#pragma unroll
+        for (int t = b; t < b + 4; ++t) {
+          const cd u = cld(Hk, k * N + r0 + t), x = cld(C, (r0 + t) * N + c);
+          dr += u.re * x.re + u.im * x.im;
+          di += u.re * x.im - u.im * x.re;
+        }
+      }
+      // lanes c and c + 32 hold the two halves of one column and take this
+      // branch together
+      dr += __shfl_xor(dr, 32, 64);
+      di += __shfl_xor(di, 32, 64);
+      const cd dot = mk<double>(dr, di);
+      // u is zero above row k, so every row takes the same update
+#pragma unroll 1
+      for (int b = 0; b < HALF; b += 4) {
+#pragma unroll
+        for (int t = b; t < b + 4; ++t) {
+          const int r = r0 + t;
+          cst(C, r * N + c, cld(C, r * N + c) - cld(Hk, k * N + r) * dot);
+        }
+      }
+    }
+    done |= 1u << j;
+    if (lane == 0) piv[k] = j;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  // X = R^H: column r of X holds conj(R[r][m]) at row m, R[r][m] = row r of
+  // column piv[m], zero below the diagonal of R
+  if (col) {
+#pragma unroll
+    for (int t = 0; t < HALF; ++t) {
+      const int r = r0 + t;
+      cst(X, r * N + mypos, r <= mypos ? conj(cld(C, r * N + c)) : mk<double>(0.0, 0.0));
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 }
 
 __device__ inline void pivoted_cholesky_swap(cd* G, int n, cd* R, int* piv, double* scal) {

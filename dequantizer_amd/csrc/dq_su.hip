@@ -489,6 +489,55 @@ __device__ void su_write_gamma(const DevGraph& g, cx<R>* t, int node, int slot, 
   }
 }
 
+// SVD of the 32 x 32 core C (row-major) preconditioned by a pivoted QR:
+// C P = Q R (block_qrcp_rh32), the one-sided Jacobi SVD of X = R^H,
+// X = U_x Sigma V_x^H (converging in a few sweeps where Jacobi on C needs about
+// eight), then C = (Q V_x) Sigma (P U_x)^H.  Outputs as block_jacobi_svd_n<32>
+// on C: sigma descending, U (column-major, sorted, zero columns for sigma = 0)
+// and V with perm, perm here being the identity.  Buffers (C, U, V in LDS):
+// U holds the reflectors until the end, X and V_x live in V and C, U_x in W
+// (any memory).
+__device__ __attribute__((noinline)) void su_core_svd_qr32(lds_cd* C, lds_cd* U, lds_cd* V, cd* W, double* sigma,
+                                                             int* perm, double* scal) {
+  __shared__ int qpiv[32];
+  block_qrcp_rh32(C, U, V, (lds_int*)qpiv);
+  block_jacobi_svd_n<32>(V, W, C, sigma, perm, scal);
+  // U_C[:, j] = Q V_x[:, perm[j]], Q = H_0 ... H_31 applied from H_31 down:
+  // 8 lanes per column, rows sub + 8 t
+  const int j = threadIdx.x >> 3, sub = threadIdx.x & 7;
+  const int pj = perm[j];
+  cd y[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) y[t] = cld(C, pj * 32 + sub + 8 * t);
+  for (int k = 31; k >= 0; --k) {
+    cd u[4];
+    double dr = 0.0, di = 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      u[t] = cld(U, k * 32 + sub + 8 * t);
+      dr += u[t].re * y[t].re + u[t].im * y[t].im;
+      di += u[t].re * y[t].im - u[t].im * y[t].re;
+    }
+    dr = group_sum<8>(dr);
+    di = group_sum<8>(di);
+    const cd dot = mk<double>(dr, di);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) y[t] = y[t] - u[t] * dot;
+  }
+  __syncthreads();  // every reflector and V_x read is done
+  const bool nz = sigma[j] > 0.0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int m = sub + 8 * t;
+    cst(U, j * 32 + m, nz ? y[t] : mk<double>(0.0, 0.0));
+    // V_C[:, j] = P U_x[:, j]: row qpiv[m] of V_C is row m of U_x
+    cst(V, j * 32 + qpiv[m], W[j * 32 + m]);
+  }
+  __syncthreads();
+  if (threadIdx.x < 32) perm[threadIdx.x] = threadIdx.x;
+  __syncthreads();
+}
+
 // Matrices of one gate (complex doubles): E (cp^2 or n^2; later U, then Z),
 // G (n^2; Gram, core, then W_a), V (n^2; right singular vectors, then W_b),
 // Ra, Rb (n^2 each), chunk (rows x cp, generic environment path only).
@@ -584,21 +633,45 @@ __global__ void __launch_bounds__(256, kMem == kSuCompact ? 2 : 1)
   else
     su_environment<R>(g, ta, na, ia, wla, E, chunk, kSuChunk);
   const double tra = su_gram(E, h1, chi, p, r, Gm, red);
-  pivoted_cholesky(Gm, n, Ra, perm, rot);
   SuTape tp(tape ? tape + (long long)blockIdx.x * tape_bytes : nullptr, chi * max_r);
-  if (tape) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) tp.piv_a()[i] = perm[i];
+  double trb;
+  if (kMem != kSuGlobal && n <= 32 && blockDim.x >= 128) {
+    // both Grams first (G_b in the V buffer, free until the SVD), then the
+    // two factorisations at once on waves 0 and 1, without block barriers
+    if (mfma_b)
+      su_environment_mfma<R>(g, tb, nb, ib, wlb, E);
+    else
+      su_environment<R>(g, tb, nb, ib, wlb, E, chunk, kSuChunk);
+    trb = su_gram(E, h2, chi, p, r, V, red);
+    const int w = threadIdx.x >> 6;
+    if (w == 0)
+      wave_pivoted_cholesky((lds_cd*)Gm, n, Ra, (lds_int*)perm);
+    else if (w == 1)
+      wave_pivoted_cholesky((lds_cd*)V, n, Rb, (lds_int*)(perm + 64));
     __syncthreads();
-  }
-  if (mfma_b)
-    su_environment_mfma<R>(g, tb, nb, ib, wlb, E);
-  else
-    su_environment<R>(g, tb, nb, ib, wlb, E, chunk, kSuChunk);
-  const double trb = su_gram(E, h2, chi, p, r, Gm, red);
-  pivoted_cholesky(Gm, n, Rb, perm, rot);
-  if (tape) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) tp.piv_b()[i] = perm[i];
-    __syncthreads();
+    if (tape) {
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        tp.piv_a()[i] = perm[i];
+        tp.piv_b()[i] = perm[64 + i];
+      }
+      __syncthreads();
+    }
+  } else {
+    pivoted_cholesky(Gm, n, Ra, perm, rot);
+    if (tape) {
+      for (int i = threadIdx.x; i < n; i += blockDim.x) tp.piv_a()[i] = perm[i];
+      __syncthreads();
+    }
+    if (mfma_b)
+      su_environment_mfma<R>(g, tb, nb, ib, wlb, E);
+    else
+      su_environment<R>(g, tb, nb, ib, wlb, E, chunk, kSuChunk);
+    trb = su_gram(E, h2, chi, p, r, Gm, red);
+    pivoted_cholesky(Gm, n, Rb, perm, rot);
+    if (tape) {
+      for (int i = threadIdx.x; i < n; i += blockDim.x) tp.piv_b()[i] = perm[i];
+      __syncthreads();
+    }
   }
   // core = R_a diag(lambda_rep) R_b^T, column-major.  R_b^T goes to the
   // free environment buffer first so that lanes (along j) read contiguous
@@ -629,8 +702,14 @@ __global__ void __launch_bounds__(256, kMem == kSuCompact ? 2 : 1)
       ci = su_mfma(a[ks].re, b[ks].im, ci);
       ci = su_mfma(a[ks].im, b[ks].re, ci);
     }
+    // column-major for the Jacobi SVD; row-major for the pivoted QR that
+    // preconditions it (its lanes walk the columns)
+    const bool row_major = !tape && kMem != kSuGlobal;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) C[(j0 + rr) * n + i0 + kq + 4 * v] = mk<double>(cr[v], ci[v]);
+    for (int v = 0; v < 4; ++v) {
+      const int i = i0 + kq + 4 * v, jc = j0 + rr;
+      C[row_major ? i * n + jc : jc * n + i] = mk<double>(cr[v], ci[v]);
+    }
   } else {
     for (int f = threadIdx.x; f < n * n; f += blockDim.x) {
       const int i = f / n, j = f % n;
@@ -640,8 +719,14 @@ __global__ void __launch_bounds__(256, kMem == kSuCompact ? 2 : 1)
     }
   }
   __syncthreads();
-  if (n == 32 && blockDim.x == 256) {
-    block_jacobi_svd_n<32>(C, U, Vs, sigma, perm, rot);
+  if (n == 32 && blockDim.x == 256 && !tape && kMem != kSuGlobal) {
+    su_core_svd_qr32((lds_cd*)C, (lds_cd*)U, (lds_cd*)Vs, chunk, sigma, perm, rot);
+  } else if (n == 32 && blockDim.x == 256) {
+    // the adjoint's record keeps the unpreconditioned factors it was built on
+    if (kMem != kSuGlobal)
+      block_jacobi_svd_n<32>((lds_cd*)C, U, (lds_cd*)Vs, sigma, perm, rot);
+    else
+      block_jacobi_svd_n<32>(C, U, Vs, sigma, perm, rot);
   } else if (n == 16) {
     if (threadIdx.x < 64) wave_jacobi_svd_n<16>(C, U, Vs, sigma, perm, rot);
     __syncthreads();
