@@ -10,8 +10,10 @@ gauge.  The first coupling layer is the identity (s = 0), so the bonds reach
 
 Stored: the instance (pairs, couplings, fields) so that the GPU test feeds
 identical inputs, and after the last step the per-spin <Z>, the per-edge
-<ZZ>, the BP energy, every gate's truncation error and every regauge's Vidal
-distance.  Run from the repository root:
+<ZZ>, the BP energy, the truncation error of every gate of the last
+TAIL_STEPS steps and every regauge's Vidal distance.  The committed file was
+made with 80 steps (truncation errors up to 5e-6 in its last steps).  Run from
+the repository root:
     python tests/golden/make_rr3_n1000.py [steps]
 """
 import os
@@ -30,6 +32,7 @@ from oracle import emulator as oe  # noqa: E402
 from oracle import graph as og  # noqa: E402
 
 N, SEED, CHI, DT, SCHEDULE, SWEEPS = 1000, 7, 16, 0.1, 200, 10
+TAIL_STEPS = 10  # steps whose per-gate truncation errors are stored
 
 
 def run(steps):
@@ -56,13 +59,15 @@ def run(steps):
     rho2 = np.array([emu.dens_q2(int(a), int(b)) for a, b in pairs])
     z = (rho1[:, 0, 0] - rho1[:, 1, 1]).real
     zz = (rho2[:, 0, 0] - rho2[:, 1, 1] - rho2[:, 2, 2] + rho2[:, 3, 3]).real
-    return dict(pairs=pairs, couplings=j, fields=h, steps=steps, z=z, zz=zz,
-                energy=oa.bp_energy(rho1, rho2, h, j), truncation_errors=np.asarray(emu.truncation_errors),
+    tail = TAIL_STEPS * len(pairs)
+    return dict(pairs=np.asarray(pairs, dtype=np.int32), couplings=j, fields=h, steps=steps, z=z, zz=zz,
+                energy=oa.bp_energy(rho1, rho2, h, j),
+                truncation_errors_tail=np.asarray(emu.truncation_errors)[-tail:], tail_steps=TAIL_STEPS,
                 vidal_distances=np.asarray(emu.vidal_distances_after_regauging, dtype=np.float64))
 
 
 if __name__ == "__main__":
-    steps = int(sys.argv[1]) if len(sys.argv) > 1 else 6
+    steps = int(sys.argv[1]) if len(sys.argv) > 1 else 80
     out = run(steps)
     np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "rr3_n1000_chi16.npz"), **out)
     print("energy", out["energy"])

@@ -2,7 +2,8 @@
 
 * The 1000-spin random 3-regular annealing run at chi = 16 (the instance of
   examples/quantum_annealing/quantum_annealing/src/quantum_annealer.py:90-152
-  at seed 7): the first six steps in complex128 and complex64 against the oracle's values
+  at seed 7): the first 80 steps (truncating from about step 45) in complex128
+  and complex64 against the oracle's values
   stored in tests/golden/rr3_n1000_chi16.npz (written by
   tests/golden/make_rr3_n1000.py), and the whole 200-step run in complex64
   against complex128 (the complex128 path is the one held to the oracle).
@@ -35,6 +36,18 @@ from dequantizer_amd.annealing import (Annealer, EnergyFunction, edge_colored, g
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
+@pytest.fixture(autouse=True)
+def _release_cached_memory():
+    """These tests hold tens of GB (the 4^3 lattice: 32 GiB of complex128
+    node tensors); blocks cached by earlier tests are returned first."""
+    import gc
+    gc.collect()
+    torch.cuda.empty_cache()
+    yield
+    gc.collect()
+    torch.cuda.empty_cache()
+
+
 def _z_zz(res):
     r1, r2 = res.density_matrices, res.two_site_density_matrices
     z = (r1[:, 0, 0] - r1[:, 1, 1]).real
@@ -61,9 +74,10 @@ def test_rr3_n1000_first_steps_match_oracle(dtype, tol):
     de = abs(res.bp_energy - float(fx["energy"])) / ef.nodes_number
     print(f"max |dZ| {dz:.2e}, max |dZZ| {dzz:.2e}, |dE|/N {de:.2e}", flush=True)
     assert dz < tol and dzz < tol and de < tol
-    assert len(res.truncation_errors) == len(fx["truncation_errors"])
-    assert np.abs(res.truncation_errors - fx["truncation_errors"]).max() < max(tol, 1e-9) * 10
-    assert fx["truncation_errors"].max() > 1e-6  # the run truncates
+    assert len(res.truncation_errors) == steps * ef.edges_number
+    tail = fx["truncation_errors_tail"]
+    assert np.abs(res.truncation_errors[-len(tail):] - tail).max() < max(tol, 1e-9) * 10
+    assert tail.max() > 1e-6  # the run truncates
     vd = np.asarray(res.vidal_distances_after_regauging, dtype=np.float64)
     assert np.abs(vd - fx["vidal_distances"]).max() < max(tol, 1e-9) * 10
 
