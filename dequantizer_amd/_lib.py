@@ -42,6 +42,7 @@ SIGNATURES = {
     "dq_lambdas_to_messages": (c_int, [vp, c_int, vp, vp, vp, vp]),
     "dq_vidal_gauge": (c_int, [vp, c_int, vp, vp, c_dbl, vp, vp, vp, vp]),
     "dq_vidal_distance": (c_int, [vp, c_int, vp, vp, c_dbl, vp, vp, vp]),
+    "dq_vidal_distance_noalloc": (c_int, [vp, c_int, vp, vp, c_dbl, vp, vp, vp]),
     "dq_symmetric_gauge": (c_int, [vp, c_int, vp, vp, c_dbl, vp, vp]),
     "dq_apply1": (c_int, [vp, c_int, vp, vp, c_int, vp, vp]),
     "dq_simple_update": (c_int, [vp, c_int, vp, vp, vp, c_int, vp, c_int, c_dbl, c_int, vp, vp, vp]),

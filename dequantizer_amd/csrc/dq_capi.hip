@@ -478,30 +478,83 @@ int dq_vidal_gauge_tape(dq_graph* g, int dtype, const void* T, const void* n2e, 
   return DQ_OK;
 }
 
+// Per-node workspace stride of the generic Vidal-distance kernels.
+static long long vidal_generic_ws_stride(const dq_graph* g) {
+  const long long cc = (long long)g->dev.chi * g->dev.chi;
+  return ((5 * cc * (long long)sizeof(cd) + ((long long)g->max_deg * g->dev.chi + 4 * g->dev.chi + 16) * 8 + 16) +
+          255) /
+         256 * 256;
+}
+
+// Workspace bytes of one Vidal-distance call on this handle, by kernel path.
+static size_t vidal_distance_ws_bytes(const dq_graph* g, int rb) {
+  if (bp_fast_applicable(rb, g->dev, g->min_deg, g->max_deg)) return (size_t)vidal_fast_ws_bytes(g->dev);
+  if (g->dev.chi == 32 && big_applicable(g->dev, g->max_deg))
+    return (size_t)vidal_big_ws_bytes(g->dev, g->max_deg, rb);
+  return (size_t)vidal_generic_ws_stride(g) * (g->dev.n_nodes > 0 ? g->dev.n_nodes : 1);
+}
+
+static hipError_t vidal_distance_launch(dq_graph* g, int rb, const void* gammas, const double* lam, double eps,
+                                        double* out, const int* inst_active, void* ws, hipStream_t s) {
+  if (bp_fast_applicable(rb, g->dev, g->min_deg, g->max_deg))
+    return launch_vidal_distance_fast(g->dev, gammas, lam, eps, g->d_all_nodes, g->d_zero_src, g->d_inst_edge_ptr,
+                                      g->n_inst, out, ws, inst_active, s);
+  if (g->dev.chi == 32 && big_applicable(g->dev, g->max_deg))
+    return launch_vidal_distance_big(rb, g->dev, g->max_deg, gammas, lam, eps, g->d_all_nodes, g->d_zero_src,
+                                     g->d_inst_edge_ptr, g->n_inst, out, ws, inst_active, s);
+  return launch_vidal_distance(rb, g->dev, gammas, lam, eps, g->node_dist, g->d_inst_node_ptr, g->d_inst_edge_ptr,
+                               g->n_inst, out, ws, vidal_generic_ws_stride(g), inst_active, s);
+}
+
 int dq_vidal_distance(dq_graph* g, int dtype, const void* gammas, const double* lam, double eps, double* out,
                       const int* inst_active, void* stream) {
   int rb, rc;
   if (!g || !out) return fail(DQ_ERR_INVALID, "NULL argument");
   if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
-  if (bp_fast_applicable(rb, g->dev, g->min_deg, g->max_deg)) {
-    if ((rc = ws_ensure(g, (size_t)vidal_fast_ws_bytes(g->dev))) != DQ_OK) return rc;
-    RET_LAUNCH(launch_vidal_distance_fast(g->dev, gammas, lam, eps, g->d_all_nodes, g->d_zero_src,
-                                          g->d_inst_edge_ptr, g->n_inst, out, g->ws, inst_active, S(stream)));
-    return DQ_OK;
+  if ((rc = ws_ensure(g, vidal_distance_ws_bytes(g, rb))) != DQ_OK) return rc;
+  RET_LAUNCH(vidal_distance_launch(g, rb, gammas, lam, eps, out, inst_active, g->ws, S(stream)));
+  return DQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch for the entry points that allocate nothing: a fixed array in device
+// memory, one copy per GPU (held until the process ends).  Host code never
+// passes the array's name to a kernel: its device address on the current GPU
+// comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[DQ_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mutex;
+static std::map<int, void*> g_scratch_ptr;
+
+static int scratch_on_current_device(void** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(DQ_ERR_HIP, "hipGetSymbolAddress returned NULL for the scratch array");
+    it = g_scratch_ptr.emplace(dev, ptr).first;
   }
-  if (g->dev.chi == 32 && big_applicable(g->dev, g->max_deg)) {
-    if ((rc = ws_ensure(g, (size_t)vidal_big_ws_bytes(g->dev, g->max_deg, rb))) != DQ_OK) return rc;
-    RET_LAUNCH(launch_vidal_distance_big(rb, g->dev, g->max_deg, gammas, lam, eps, g->d_all_nodes, g->d_zero_src,
-                                         g->d_inst_edge_ptr, g->n_inst, out, g->ws, inst_active, S(stream)));
-    return DQ_OK;
-  }
-  const long long cc = (long long)g->dev.chi * g->dev.chi;
-  const long long wsv =
-      ((5 * cc * (long long)sizeof(cd) + ((long long)g->max_deg * g->dev.chi + 4 * g->dev.chi + 16) * 8 + 16) + 255) /
-      256 * 256;
-  if ((rc = ws_ensure(g, (size_t)wsv * (g->dev.n_nodes > 0 ? g->dev.n_nodes : 1))) != DQ_OK) return rc;
-  RET_LAUNCH(launch_vidal_distance(rb, g->dev, gammas, lam, eps, g->node_dist, g->d_inst_node_ptr,
-                                   g->d_inst_edge_ptr, g->n_inst, out, g->ws, wsv, inst_active, S(stream)));
+  *out = it->second;
+  return DQ_OK;
+}
+
+int dq_vidal_distance_noalloc(dq_graph* g, int dtype, const void* gammas, const double* lam, double eps, double* out,
+                              const int* inst_active, void* stream) {
+  int rb, rc;
+  if (!g || !out || !gammas || !lam) return fail(DQ_ERR_INVALID, "NULL argument");
+  if ((rc = check_dtype(dtype, &rb)) != DQ_OK) return rc;
+  const size_t need = vidal_distance_ws_bytes(g, rb);
+  if (need > (size_t)DQ_NOALLOC_SCRATCH_BYTES)
+    return fail(DQ_ERR_INVALID,
+                "dq_vidal_distance_noalloc: this graph needs %zu bytes of scratch, above the fixed %lld "
+                "(use dq_vidal_distance)",
+                need, (long long)DQ_NOALLOC_SCRATCH_BYTES);
+  void* ws = nullptr;
+  if ((rc = scratch_on_current_device(&ws)) != DQ_OK) return rc;
+  RET_LAUNCH(vidal_distance_launch(g, rb, gammas, lam, eps, out, inst_active, ws, S(stream)));
   return DQ_OK;
 }
 

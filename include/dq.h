@@ -133,6 +133,21 @@ int dq_vidal_gauge(dq_graph* g, int dtype, const void* tensors, const void* n2e,
 int dq_vidal_distance(dq_graph* g, int dtype, const void* gammas, const double* lam, double eps, double* out,
                       const int* inst_active, void* stream);
 
+/* dq_vidal_distance without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Same arguments
+ * and results as dq_vidal_distance (the same kernels), but the kernels'
+ * scratch (bond Grams and per-bond distances) lives in a fixed array of
+ * DQ_NOALLOC_SCRATCH_BYTES in device memory, one per GPU, held until the
+ * process ends, instead of the handle's workspace.  Graphs whose scratch
+ * would exceed it are rejected with DQ_ERR_INVALID (on the complex64 chi=16
+ * degree-3 path: 2 * n_edges * 2056 + 256 bytes, about 16,300 edges).  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU (on
+ * different streams or threads) race. */
+#define DQ_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int dq_vidal_distance_noalloc(dq_graph* g, int dtype, const void* gammas, const double* lam, double eps, double* out,
+                              const int* inst_active, void* stream);
+
 /* Symmetric gauge.  Replaces get_symmetric_gauge_fixing_map
  * (mappings/symmetric_gauge.py:17-39) = canonicalize (tensor_ops.py:189-199). */
 int dq_symmetric_gauge(dq_graph* g, int dtype, const void* gammas, const double* lam, double eps, void* out,
