@@ -32,6 +32,11 @@ PEAK_HBM_GBS = 8000.0
 # reads half its bytes, WRITE_SIZE all): 2 x 38304.5 KiB read + 12000 KiB
 # written.
 TRAFFIC_BP_SWEEP_DEFAULT = 2 * 38304.5 * 1024 + 12000.0 * 1024
+# HBM bytes of one simple-update launch (500 gates, one edge-colour class) of
+# the default workload, from the same counter passes over a whole run (median
+# dispatch; the 256 MiB copy reads 131092.9 KiB by FETCH_SIZE and writes
+# 262144 KiB by WRITE_SIZE): 90680 KiB fetched (x 2) + 121300 KiB written.
+TRAFFIC_SU_LAUNCH_DEFAULT = 309879627
 
 
 def eu_flops(d: int, chi: int, p: int) -> int:
@@ -217,7 +222,9 @@ def main():
         "kernel": "simple update (k_su, one edge-colour class of gates per launch)",
         "achieved": su_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
         "frac": (su_tflops / PEAK_FP64_TFLOPS) if su_tflops else None,
-        "traffic": None,
+        "traffic": (TRAFFIC_SU_LAUNCH_DEFAULT if (args.nodes, args.chi) == (1000, 16) else None),
+        "traffic_unit": "B per launch",
+        "algorithmic_bytes_per_launch": su_gates_per_launch * 2 * 2 * 8 * chi ** d * p,
         "algorithmic_flops_per_launch": su_gates_per_launch * gate_flops,
         "algorithmic_flops_per_gate": gate_flops,
         "avg_launch_s": su_launch_s, "launches": int(len(su_durs)), "share_of_run": su_s / elapsed,

@@ -613,7 +613,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int t = 0; t < 4; ++t) G[threadIdx.x + 256 * t] = h[t];
   __syncthreads();
-  block_jacobi_svd_n<32>(G, U, V, sg, perm, scal);
+  block_jacobi_svd_n<32>((lds_cd*)G, U, (lds_cd*)V, sg, perm, scal);
   if (threadIdx.x == 0) {
     double nuc = 0.0;
     for (int k = 0; k < 32; ++k) nuc += sg[k];
