@@ -106,6 +106,11 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="Trotter steps of the forward+adjoint timing (0 skips it)")
     ap.add_argument("--knots", type=int, default=32, help="schedule knots of the forward+adjoint timing")
+    ap.add_argument("--adjoint-memory-gb", type=float, default=24.0,
+                    help="device memory for the adjoint's records and checkpoints (GiB)")
+    ap.add_argument("--adjoint-bp-tol", type=float, default=0.0,
+                    help="forward+adjoint timing with every regauge iterated to the BP fixed point (message "
+                         "residual below this) and the implicit adjoint; 0 = the fixed sweeps of --fixed-sweeps")
     ap.add_argument("--batch-instances", type=int, default=64,
                     help="instances of the batched annealing leg (the whole 200-step run of the random 3-regular "
                          "instance set, N in 100..1000, in one engine); 0 skips it")
@@ -601,13 +606,12 @@ def adjoint_timing(ef, args):
     """Energy of the annealing run and its gradient with respect to the
     schedule knots and every coupling (hand-written adjoint, complex128,
     synchronous BP, fixed sweeps per regauge): wall time of one forward and
-    of forward + adjoint on the same instance.  The forward + adjoint runs
-    twice: the first call takes the device memory for the forward records
-    (~1 GB per step) from the driver, which can stall for up to a second
-    while memory freed by earlier work is cleared; the second call reuses
-    the allocator's cached blocks.  ``seconds`` is the second call (a
-    gradient-based schedule search calls it repeatedly),
-    ``first_call_seconds`` the first."""
+    of forward + adjoint on the same instance, with the records and
+    checkpoints held to ``--adjoint-memory-gb`` (the backward recomputes all
+    segments but the last from their checkpoints).  The forward + adjoint
+    runs twice; ``seconds`` is the second call (a gradient-based schedule
+    search calls it repeatedly, and the first takes its device memory from
+    the driver), ``first_call_seconds`` the first."""
     import torch
     from dequantizer_amd import PRNGKey
     from dequantizer_amd.adjoint import AnnealingGradient
@@ -615,7 +619,9 @@ def adjoint_timing(ef, args):
     torch.cuda.empty_cache()  # blocks cached by the earlier legs
     t0 = time.perf_counter()
     run = AnnealingGradient(ef, PRNGKey(args.seed), chi=args.chi, n_steps=args.adjoint_steps, dt=args.dt,
-                            n_knots=args.knots, fixed_sweeps=args.fixed_sweeps or 10)
+                            n_knots=args.knots,
+                            fixed_sweeps=None if args.adjoint_bp_tol > 0 else (args.fixed_sweeps or 10),
+                            bp_tol=args.adjoint_bp_tol or 1e-12)
     torch.cuda.synchronize()
     setup = time.perf_counter() - t0
     knots = np.linspace(0.0, 1.0, args.knots)
@@ -627,16 +633,23 @@ def adjoint_timing(ef, args):
     e_fwd = run.value(knots)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    run.value_and_grad(knots)
+    run.value_and_grad(knots, memory_gb=args.adjoint_memory_gb)
     torch.cuda.synchronize()
+    run.forward_sweeps.clear()
+    run.adjoint_iterations.clear()
     t2 = time.perf_counter()
-    e, dk, dj = run.value_and_grad(knots)
+    e, dk, dj = run.value_and_grad(knots, memory_gb=args.adjoint_memory_gb)
     torch.cuda.synchronize()
     t3 = time.perf_counter()
     return {"seconds": t3 - t2, "first_call_seconds": t2 - t1, "forward_seconds": t1 - t0, "setup_seconds": setup,
             "steps": args.adjoint_steps,
             "knots": args.knots, "couplings": int(ef.edges_number), "dtype": "c128", "backward_cut": run.cut,
-            "sweeps_per_regauge": args.fixed_sweeps or 10, "energy": e, "energy_forward_only": e_fwd,
+            "sweeps_per_regauge": (f"to residual {args.adjoint_bp_tol:g} (implicit adjoint)" if args.adjoint_bp_tol > 0
+                                   else args.fixed_sweeps or 10),
+            "bp_sweeps_per_call": int(np.sum(run.forward_sweeps)) if args.adjoint_bp_tol > 0 else None,
+            "adjoint_iterations_per_call": int(np.sum(run.adjoint_iterations)) if args.adjoint_bp_tol > 0 else None,
+            "memory_budget_gb": args.adjoint_memory_gb,
+            "segment_steps": run.segment_steps, "record_gb_per_step": run.record_bytes() / 2 ** 30, "energy": e, "energy_forward_only": e_fwd,
             "grad_norm_knots": float(np.linalg.norm(dk)), "grad_norm_couplings": float(np.linalg.norm(dj)),
             "peak_device_memory_gb": torch.cuda.max_memory_allocated() / 1e9,
             "held_before_gb": held_gb,

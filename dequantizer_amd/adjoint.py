@@ -6,16 +6,24 @@ examples/quantum_annealing/quantum_annealing/src/quantum_annealer.py:104-141
 gates; final regauge and E = -sum h <Z> - sum J <ZZ>, energy_evaluation.py
 sign convention) with the schedule s(t) piecewise linear through ``knots``
 (coupling time dt s(t_k), mixing time dt (1 - s(t_k))).  It runs in
-complex128 with synchronous BP and a fixed number of sweeps per regauge, so
-the energy is a smooth function of (knots, J) and the adjoint is exact.
+complex128 with synchronous BP.  Each regauge either runs a fixed number of
+sweeps from the diag(lambda) messages, or iterates BP to its fixed point
+(message residual below ``bp_tol``), the loop of
+dequantizer/src/quantum_emulator/bp_quantum_emulator.py:283-296 driven to
+convergence.  Either way the energy is a smooth function of (knots, J).
 
-Reverse mode, per step from the last: the step is recomputed from its
-checkpoint with the forward kernels recording what their adjoints need (the
-simple-update and edge-SVD tapes, the sweep inputs, the states before every
-gate level), then the adjoint kernels run in reverse order:
-apply1 -> simple-update levels -> Vidal gauge -> BP sweeps -> diag(lambda)
-messages -> symmetric gauge.  Checkpoints are kept in device memory (one
-state per step).
+Reverse mode, per step from the last: the forward kernels record what their
+adjoints need (the simple-update and edge-SVD tapes, the sweep inputs or
+the fixed point, the states before every gate level), then the adjoint
+kernels run in reverse order: apply1 -> simple-update levels -> Vidal gauge
+-> BP -> symmetric gauge.  Through BP, fixed sweeps are walked back one by
+one and then into the diag(lambda) messages they started from; at a fixed
+point m* = F(T, m*) the adjoint is the implicit one, the reverse iteration
+mbar <- F_m^H mbar with every step's F_T^H mbar summed into the tensors'
+gradient, run at m* until mbar vanishes (nothing reaches the starting
+messages, which the fixed point does not depend on).  Records are kept for
+one segment of steps at a time; earlier segments are recomputed from
+checkpoints (``value_and_grad``).
 
 Coupling gates are diagonal, G = sum_q |q><q| (x) diag(g_q0, g_q1): this
 exact rank-2 split replaces the SVD split of _decompose_gate
@@ -24,7 +32,8 @@ the gate itself, so results agree up to the singular-vector gauge and the
 scale of the unnormalised lambdas (which the next regauge removes), and the
 gradient of g_{q s} is the (s, s) entry of the second half's gradient.
 """
-from typing import Optional, Sequence, Tuple
+import warnings
+from typing import Optional
 
 import numpy as np
 import torch
@@ -56,7 +65,9 @@ def knot_weights(n_knots: int, n_steps: int) -> np.ndarray:
 
 
 class AnnealingGradient:
-    """value_and_grad(knots, J) -> (E, dE/dknots, dE/dJ) for one instance.
+    """value_and_grad(knots, J) -> (E, dE/dknots, dE/dJ) for one instance
+    (``ef`` an EnergyFunction, ``key`` a PRNGKey) or for several run together
+    in one engine under the same schedule (``ef`` and ``key`` sequences).
 
     ``cut``: backward cutoff of the simple-update and Vidal-gauge adjoints.
     Singular values, Cholesky pivots and message eigenvalues below ``cut``
@@ -69,21 +80,40 @@ class AnnealingGradient:
     1000-spin, chi=16, 200-step run the usable window is about 3e-7..1e-6:
     at 1e-8 and below early-step gradients are rounding noise, at 1e-4 and
     above the frozen directions include the growing Schmidt values that carry
-    the early-schedule gradient."""
+    the early-schedule gradient.
 
-    def __init__(self, ef: EnergyFunction, key, chi: int, n_steps: int, dt: float = 0.1, n_knots: int = 32,
-                 fixed_sweeps: int = 10, traversal: str = "bfs", cut: float = 1e-6):
+    ``fixed_sweeps``: BP sweeps per regauge; None iterates each regauge to
+    the fixed point (mean Frobenius distance between successive message sets
+    below ``bp_tol``) and differentiates it implicitly, the adjoint iteration
+    stopping when its cotangent has fallen by ``bp_tol`` (both at most
+    ``max_sweeps``; hitting the cap warns).  ``forward_sweeps`` and
+    ``adjoint_iterations`` collect the counts of every regauge since they
+    were last cleared."""
+
+    def __init__(self, ef, key, chi: int, n_steps: int, dt: float = 0.1, n_knots: int = 32,
+                 fixed_sweeps: Optional[int] = 10, traversal: str = "bfs", cut: float = 1e-6,
+                 bp_tol: float = 1e-12, max_sweeps: int = 1000):
         if not 0.0 <= cut < 1.0:
             raise ValueError(f"cut {cut} outside [0, 1)")
+        self.batched = not isinstance(ef, EnergyFunction)
+        efs = list(ef) if self.batched else [ef]
+        keys = list(key) if self.batched else [key]
+        if not efs or len(keys) != len(efs):
+            raise ValueError(f"{len(efs)} energy functions and {len(keys)} keys")
         self.cut = float(cut)
-        self.ef = ef
-        self.n_steps, self.dt, self.n_knots, self.K = int(n_steps), float(dt), int(n_knots), int(fixed_sweeps)
+        self.ef, self.efs = efs[0], efs
+        self.n_inst = len(efs)
+        self.n_steps, self.dt, self.n_knots = int(n_steps), float(dt), int(n_knots)
+        self.K = None if fixed_sweeps is None else int(fixed_sweeps)
+        self.bp_tol, self.max_sweeps = float(bp_tol), int(max_sweeps)
+        self.forward_sweeps, self.adjoint_iterations = [], []
         self.W = knot_weights(self.n_knots, self.n_steps)
-        graph = tensor_graph_from(ef)
-        emu = BatchedBPEmulator([graph], [key], chi, ef.edges_number, 1e-5, 100, True, traversal,
+        graphs = [tensor_graph_from(e) for e in efs]
+        emu = BatchedBPEmulator(graphs, keys, chi, max(e.edges_number for e in efs), 1e-5, 100, True, traversal,
                                 dtype=torch.complex128, chi=chi, fixed_sweeps=self.K)
-        for i in range(ef.nodes_number):
-            emu.queue_q1(0, HADAMARD, i)
+        for inst, e in enumerate(efs):
+            for i in range(e.nodes_number):
+                emu.queue_q1(inst, HADAMARD, i)
         emu._flush()
         torch.cuda.synchronize()
         self.emu = emu
@@ -93,22 +123,32 @@ class AnnealingGradient:
         self.E, self.N = self.topo.n_edges, self.topo.n_nodes
         self.gam0 = emu.gam.clone()
         self.lam0 = emu.lam.clone()
-        plan = emu.make_plan([(0, int(a), int(b)) for a, b in ef.coupled_spin_pairs], [])
+        # coupled pairs of every instance, concatenated in instance order
+        plan = emu.make_plan([(inst, int(a), int(b)) for inst, e in enumerate(efs) for a, b in e.coupled_spin_pairs],
+                             [])
         self.launches = [(o2, c2) for (o2, c2, _o1, _c1) in plan.launches if c2]
         self.order2 = plan.order2  # launch position -> pair index
         meta = plan.meta.copy()
         meta[:, 2] = 2
         self.d_meta = torch.from_numpy(meta).to(DEVICE)
-        self.pairs = np.asarray(ef.coupled_spin_pairs, dtype=np.int64)
-        deg = np.asarray(ef.node_degrees, dtype=np.float64)
-        self.ha = ef.fields[self.pairs[:, 0]] / deg[self.pairs[:, 0]]
-        self.hb = ef.fields[self.pairs[:, 1]] / deg[self.pairs[:, 1]]
-        nidx = self.topo.node_index[0]
+        self.pairs = np.concatenate([np.asarray(e.coupled_spin_pairs, dtype=np.int64) for e in efs])
+        self.pair_inst = np.concatenate([np.full(e.edges_number, i) for i, e in enumerate(efs)])
+        self.spin_inst = np.concatenate([np.full(e.nodes_number, i) for i, e in enumerate(efs)])
+        self.fields = np.concatenate([np.asarray(e.fields, dtype=np.float64) for e in efs])
+        ha, hb = [], []
+        for e in efs:
+            pr = np.asarray(e.coupled_spin_pairs, dtype=np.int64)
+            deg = np.asarray(e.node_degrees, dtype=np.float64)
+            ha.append(e.fields[pr[:, 0]] / deg[pr[:, 0]])
+            hb.append(e.fields[pr[:, 1]] / deg[pr[:, 1]])
+        self.ha, self.hb = np.concatenate(ha), np.concatenate(hb)
         self.d_nodes_all = torch.arange(self.N, dtype=torch.int32, device=DEVICE)
-        self.rdm_nodes = torch.tensor([nidx[i] for i in range(ef.nodes_number)], dtype=torch.int32, device=DEVICE)
-        eidx = self.topo.edge_index[0]
-        self.rdm_edges = torch.tensor([eidx[(int(a), int(b))] for a, b in self.pairs], dtype=torch.int32,
-                                      device=DEVICE)
+        self.node_inst = np.asarray(self.topo.node_inst)  # instance of each engine node (mixing-gate order)
+        self.rdm_nodes = torch.tensor([self.topo.node_index[inst][i] for inst, e in enumerate(efs)
+                                       for i in range(e.nodes_number)], dtype=torch.int32, device=DEVICE)
+        self.rdm_edges = torch.tensor([self.topo.edge_index[inst][(int(a), int(b))] for inst, e in enumerate(efs)
+                                       for a, b in e.coupled_spin_pairs], dtype=torch.int32, device=DEVICE)
+        self.n_spins, self.n_pairs = len(self.spin_inst), len(self.pair_inst)
         L = _lib.lib()
         self.su_tape_bytes = int(L.dq_su_tape_bytes(self.h.ptr, 2))
         self.svd_tape_bytes = int(L.dq_edge_svd_tape_bytes(self.h.ptr)) * self.E
@@ -144,12 +184,17 @@ class AnnealingGradient:
         e2n = torch.zeros_like(n2e)
         _lib.check(L.dq_lambdas_to_messages(h.ptr, _lib.DQ_C128, _lib.ptr(lam), _lib.ptr(n2e), _lib.ptr(e2n), st))
         inputs = []
-        for _ in range(self.K):
-            inputs.append(e2n)
-            nn, ne = torch.empty_like(n2e), torch.empty_like(n2e)
-            _lib.check(L.dq_bp_sweep(h.ptr, _lib.DQ_C128, _lib.ptr(ts), _lib.ptr(n2e), _lib.ptr(e2n), _lib.ptr(nn),
-                                     _lib.ptr(ne), None, st))
-            n2e, e2n = nn, ne
+        if self.K is not None:
+            for _ in range(self.K):
+                inputs.append(e2n)
+                nn, ne = torch.empty_like(n2e), torch.empty_like(n2e)
+                _lib.check(L.dq_bp_sweep(h.ptr, _lib.DQ_C128, _lib.ptr(ts), _lib.ptr(n2e), _lib.ptr(e2n),
+                                         _lib.ptr(nn), _lib.ptr(ne), None, st))
+                n2e, e2n = nn, ne
+        else:
+            n2e, e2n, fixed_point, sweeps = self._converge(ts, n2e, e2n)
+            inputs = [fixed_point]
+            self.forward_sweeps.append(sweeps)
         gam_new = torch.empty_like(gam)
         lam_new = torch.empty_like(lam)
         X = torch.empty(2 * self.E * self.cc, dtype=torch.complex128, device=DEVICE)
@@ -158,6 +203,26 @@ class AnnealingGradient:
                                          _lib.ptr(lam_new), _lib.ptr(X), _lib.ptr(tape), st))
         rec = (gam, lam, ts, inputs, X, tape) if record else None
         return gam_new, lam_new, rec
+
+    def _converge(self, ts, n2e, e2n):
+        """Synchronous sweeps until the mean Frobenius distance between
+        successive message sets is below bp_tol; returns the last messages,
+        the last sweep's input e2n (the fixed point the adjoint linearises
+        at) and the sweep count."""
+        L, h, st = _lib.lib(), self.h, _lib.stream_handle()
+        dist = torch.empty(1, dtype=torch.float64, device=DEVICE)
+        for it in range(1, self.max_sweeps + 1):
+            nn, ne = torch.empty_like(n2e), torch.empty_like(n2e)
+            _lib.check(L.dq_bp_sweep(h.ptr, _lib.DQ_C128, _lib.ptr(ts), _lib.ptr(n2e), _lib.ptr(e2n), _lib.ptr(nn),
+                                     _lib.ptr(ne), None, st))
+            _lib.check(L.dq_messages_distance(h.ptr, _lib.DQ_C128, _lib.ptr(nn), _lib.ptr(ne), _lib.ptr(n2e),
+                                              _lib.ptr(e2n), _lib.ptr(dist), st))
+            prev, n2e, e2n = e2n, nn, ne
+            if float(dist) < self.bp_tol:
+                return n2e, e2n, prev, it
+        warnings.warn(f"BP did not reach the residual {self.bp_tol:g} in {self.max_sweeps} sweeps "
+                      f"(last {float(dist):.3g})")
+        return n2e, e2n, prev, self.max_sweeps
 
     def _step(self, gam, lam, k: int, knots, J, record: bool):
         L, h, st = _lib.lib(), self.h, _lib.stream_handle()
@@ -196,7 +261,7 @@ class AnnealingGradient:
         n_d = torch.zeros(2 * self.E * self.cc, dtype=torch.complex128, device=DEVICE)
         e_d = torch.zeros_like(n_d)
         _lib.check(L.dq_lambdas_to_messages(h.ptr, _lib.DQ_C128, _lib.ptr(lam), _lib.ptr(n_d), _lib.ptr(e_d), st))
-        n1, n2 = self.ef.nodes_number, self.ef.edges_number
+        n1, n2 = self.n_spins, self.n_pairs
         r1 = torch.empty((n1, 2, 2), dtype=torch.complex128, device=DEVICE)
         r2 = torch.empty((n2, 4, 4), dtype=torch.complex128, device=DEVICE)
         _lib.check(L.dq_rdm1(h.ptr, _lib.DQ_C128, _lib.ptr(ts), _lib.ptr(e_d), _lib.ptr(self.rdm_nodes), n1,
@@ -206,13 +271,30 @@ class AnnealingGradient:
         return (gam, lam, ts, rec), r1, r2
 
     def _energy(self, r1, r2, J):
+        """Energy of every instance (a float for a single one) and the <ZZ>
+        of every coupled pair."""
         z = (r1[:, 0, 0] - r1[:, 1, 1]).real
         zz = (r2[:, 0, 0] - r2[:, 1, 1] - r2[:, 2, 2] + r2[:, 3, 3]).real
-        return float(-np.dot(self.ef.fields, z) - np.dot(J, zz)), zz
+        e = (-np.bincount(self.spin_inst, self.fields * z, minlength=self.n_inst)
+             - np.bincount(self.pair_inst, J * zz, minlength=self.n_inst))
+        return (e if self.batched else float(e[0])), zz
 
-    def value(self, knots, J=None) -> float:
+    def _couplings(self, J) -> np.ndarray:
+        """Couplings of every pair, concatenated over instances (J: None for
+        the energy functions' own, an array, or one array per instance)."""
+        if J is None:
+            return np.concatenate([np.asarray(e.coupling_amplitudes, dtype=np.float64) for e in self.efs])
+        if self.batched:
+            J = np.concatenate([np.asarray(j, dtype=np.float64).reshape(-1) for j in J])
+        J = np.asarray(J, dtype=np.float64).reshape(-1)
+        if J.shape[0] != self.n_pairs:
+            raise ValueError(f"{J.shape[0]} couplings for {self.n_pairs} coupled pairs")
+        return J
+
+    def value(self, knots, J=None):
+        """E(knots, J): a float, or one energy per instance when batched."""
         knots = np.asarray(knots, dtype=np.float64)
-        J = np.asarray(self.ef.coupling_amplitudes if J is None else J, dtype=np.float64)
+        J = self._couplings(J)
         gam, lam = self.gam0.clone(), self.lam0.clone()
         for k in range(self.n_steps):
             gam, lam, _ = self._step(gam, lam, k, knots, J, False)
@@ -234,59 +316,116 @@ class AnnealingGradient:
         # update reading the sweep's input, e2n_i = swap(n2e_{i-1}); walk both
         # gradients back to the diag(lambda) messages of sweep 0
         e_bar = None
-        for i in range(self.K, 0, -1):
-            new_e = None
-            if n_bar is not None:
-                new_e = torch.empty_like(X)
-                _lib.check(L.dq_bp_sweep_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(ts), _lib.ptr(inputs[i - 1]),
-                                             _lib.ptr(n_bar), _lib.ptr(new_e), _lib.ptr(ts_bar), st))
-            new_n = None if e_bar is None else e_bar.view(self.E, 2, self.cc).flip(1).reshape(-1).contiguous()
-            n_bar, e_bar = new_n, new_e
+        if self.K is not None:
+            for i in range(self.K, 0, -1):
+                new_e = None
+                if n_bar is not None:
+                    new_e = torch.empty_like(X)
+                    _lib.check(L.dq_bp_sweep_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(ts), _lib.ptr(inputs[i - 1]),
+                                                 _lib.ptr(n_bar), _lib.ptr(new_e), _lib.ptr(ts_bar), st))
+                new_n = None if e_bar is None else e_bar.view(self.E, 2, self.cc).flip(1).reshape(-1).contiguous()
+                n_bar, e_bar = new_n, new_e
+        else:
+            # implicit adjoint at the fixed point: n2e = F(T, e2n), e2n =
+            # swap(n2e), so the cotangent of n2e maps to itself through
+            # swap(F_e2n^H .) while F_T^H . accumulates into ts_bar
+            fp = inputs[0]
+            ref = float(torch.linalg.vector_norm(n_bar))
+            its = 0
+            while its < self.max_sweeps and ref > 0.0:
+                e_in = torch.empty_like(X)
+                _lib.check(L.dq_bp_sweep_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(ts), _lib.ptr(fp), _lib.ptr(n_bar),
+                                             _lib.ptr(e_in), _lib.ptr(ts_bar), st))
+                n_bar = e_in.view(self.E, 2, self.cc).flip(1).reshape(-1).contiguous()
+                its += 1
+                if float(torch.linalg.vector_norm(n_bar)) <= self.bp_tol * ref:
+                    break
+            else:
+                if ref > 0.0:
+                    warnings.warn(f"BP adjoint did not fall by {self.bp_tol:g} in {self.max_sweeps} iterations")
+            self.adjoint_iterations.append(its)
+            n_bar = None
         gam_bar = torch.zeros_like(gam)
         lam_bar = torch.zeros_like(lam)
-        _lib.check(L.dq_lambdas_to_messages_vjp(h.ptr, _lib.DQ_C128, None if n_bar is None else _lib.ptr(n_bar),
-                                                None if e_bar is None else _lib.ptr(e_bar), _lib.ptr(lam_bar), st))
+        if n_bar is not None or e_bar is not None:
+            _lib.check(L.dq_lambdas_to_messages_vjp(h.ptr, _lib.DQ_C128,
+                                                    None if n_bar is None else _lib.ptr(n_bar),
+                                                    None if e_bar is None else _lib.ptr(e_bar), _lib.ptr(lam_bar),
+                                                    st))
         _lib.check(L.dq_symmetric_gauge_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(gam), _lib.ptr(lam), EPS, _lib.ptr(ts_bar),
                                             _lib.ptr(gam_bar), _lib.ptr(lam_bar), st))
         return gam_bar, lam_bar
 
-    def value_and_grad(self, knots, J=None, trace: Optional[list] = None,
-                       record_fraction: float = 0.6) -> Tuple[float, np.ndarray, np.ndarray]:
-        """trace (optional list) receives (step, |Gamma-bar|, |lambda-bar|) after
-        each step's adjoint.  The forward keeps the records of as many steps
-        (tapes, sweep inputs, states before every gate level; ~0.9 GB per step
-        at 1000 spins, chi=16) as fit in ``record_fraction`` of the free device
-        memory; the backward recomputes the other steps from their
-        checkpoints."""
+    def record_bytes(self) -> int:
+        """Device bytes of one step's forward record: the state before the
+        regauge, the symmetric tensors, the K sweep inputs, the gauge's X and
+        tape, the state before every gate level with the level's tape, the
+        state before the mixing gates and the gate halves."""
+        state = self.gam0.numel() * 16 + self.lam0.numel() * 8
+        msg = 2 * self.E * self.cc * 16
+        levels = len(self.launches) * state + sum(c2 for _, c2 in self.launches) * self.su_tape_bytes
+        sweeps = 2 if self.K is None else self.K + 1
+        return 3 * state + sweeps * msg + self.svd_tape_bytes + levels + self.E * 2 * 16 * 16
+
+    def segment_length(self, memory_gb: float) -> int:
+        """Steps per recomputation segment: the largest s for which the
+        checkpoints (one state per segment) and the records of one segment
+        fit in ``memory_gb``."""
+        state = self.gam0.numel() * 16 + self.lam0.numel() * 8
+        rec, budget, n = self.record_bytes(), memory_gb * 2 ** 30, self.n_steps
+        best = 0
+        for s in range(1, n + 1):
+            if -(-n // s) * state + s * rec <= budget:
+                best = s
+        if best == 0:
+            need = (n * state + rec) / 2 ** 30
+            raise ValueError(f"memory_gb={memory_gb} holds neither the checkpoints nor one step's records "
+                             f"({need:.1f} GB at one step per segment)")
+        return best
+
+    def value_and_grad(self, knots, J=None, trace: Optional[list] = None, memory_gb: float = 24.0,
+                       segment_steps: Optional[int] = None):
+        """(E, dE/dknots, dE/dJ).  Batched: E has one entry per instance,
+        dE/dknots is (instances, knots) (each instance's energy against the
+        shared schedule) and dE/dJ one array per instance, all from one
+        backward.  trace (optional list) receives (step, |Gamma-bar|,
+        |lambda-bar|) after each step's adjoint.
+
+        Memory is bounded by ``memory_gb`` (records and checkpoints; the
+        engine's own buffers come on top): the run is cut into segments of
+        ``segment_length(memory_gb)`` steps.  The forward keeps the state at
+        the start of each segment and the records of the last segment
+        (~1 GB per step at 1000 spins, chi=16); the backward recomputes each
+        earlier segment from its checkpoint with records on, then walks it
+        back.  Cost: one forward, one recomputation of all but the last
+        segment, one adjoint.  A budget that holds every step's records gives
+        one segment and no recomputation.  ``segment_steps`` sets the segment
+        length directly instead."""
         knots = np.asarray(knots, dtype=np.float64)
-        J = np.asarray(self.ef.coupling_amplitudes if J is None else J, dtype=np.float64)
+        J = self._couplings(J)
         L, h, st = _lib.lib(), self.h, _lib.stream_handle()
-        # forward with one checkpoint per step, and the records that fit
-        checkpoints = []
-        records = [None] * self.n_steps
-        # free device memory plus what torch's allocator holds cached but unused
-        free = torch.cuda.mem_get_info()[0] + torch.cuda.memory_reserved() - torch.cuda.memory_allocated()
-        budget = record_fraction * free
-        used, per_step = 0, None
+        n = self.n_steps
+        seg = self.segment_length(memory_gb) if segment_steps is None else int(segment_steps)
+        if not 1 <= seg <= n:
+            raise ValueError(f"segment_steps {seg} outside [1, {n}]")
+        last_start = ((n - 1) // seg) * seg
+        self.segment_steps = seg
+        checkpoints = {}
+        records = [None] * n
         gam, lam = self.gam0.clone(), self.lam0.clone()
-        for k in range(self.n_steps):
-            checkpoints.append((gam, lam))
-            keep = per_step is None or used + per_step <= budget
-            before = torch.cuda.memory_allocated()
+        for k in range(n):
+            if k % seg == 0 and k < last_start:
+                checkpoints[k] = (gam, lam)
+            keep = k >= last_start
             gam, lam, rec = self._step(gam.clone(), lam.clone(), k, knots, J, keep)
-            if keep:
-                grown = torch.cuda.memory_allocated() - before
-                if k >= 1:  # a step with a regauge
-                    per_step = max(per_step or 0, grown)
-                used += grown
-                records[k] = rec
+            records[k] = rec if keep else None
         (gf, lf, tsf, reg_f), r1, r2 = self._observables(gam, lam)
         r1h, r2h = r1.cpu().numpy(), r2.cpu().numpy()
         energy, zz = self._energy(r1h, r2h, J)
         # energy -> density matrices -> final symmetric tensors and lambdas
         rho1_bar = np.zeros_like(r1h)
-        rho1_bar[:, 0, 0] = -self.ef.fields
-        rho1_bar[:, 1, 1] = self.ef.fields
+        rho1_bar[:, 0, 0] = -self.fields
+        rho1_bar[:, 1, 1] = self.fields
         rho2_bar = np.zeros_like(r2h)
         for c, sgn in enumerate(_ZZ):
             rho2_bar[:, c, c] = -J * sgn
@@ -294,22 +433,27 @@ class AnnealingGradient:
         ts_bar = torch.zeros_like(tsf)
         lam_bar = torch.zeros_like(lf)
         _lib.check(L.dq_rdm1_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(tsf), _lib.ptr(lf), _lib.ptr(self.rdm_nodes),
-                                 self.ef.nodes_number, _lib.ptr(d1), _lib.ptr(ts_bar), _lib.ptr(lam_bar), st))
+                                 self.n_spins, _lib.ptr(d1), _lib.ptr(ts_bar), _lib.ptr(lam_bar), st))
         _lib.check(L.dq_rdm2_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(tsf), _lib.ptr(lf), _lib.ptr(self.rdm_edges),
-                                 self.ef.edges_number, _lib.ptr(d2), _lib.ptr(ts_bar), _lib.ptr(lam_bar), st))
+                                 self.n_pairs, _lib.ptr(d2), _lib.ptr(ts_bar), _lib.ptr(lam_bar), st))
         gam_bar = torch.zeros_like(gf)
         _lib.check(L.dq_symmetric_gauge_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(gf), _lib.ptr(lf), EPS, _lib.ptr(ts_bar),
                                             _lib.ptr(gam_bar), _lib.ptr(lam_bar), st))
         gam_bar, lam_bar = self._regauge_vjp(reg_f, gam_bar, lam_bar)
         dJ = -zz.copy()  # explicit -J <ZZ> term
-        dtau_c = np.zeros(self.n_steps)
-        dtau_m = np.zeros(self.n_steps)
-        for k in range(self.n_steps - 1, -1, -1):
+        # per instance: the instances share the schedule but not the state
+        dtau_c = np.zeros((self.n_inst, self.n_steps))
+        dtau_m = np.zeros((self.n_inst, self.n_steps))
+        for k in range(n - 1, -1, -1):
+            if records[k] is None:
+                # recompute the segment ending at step k from its checkpoint
+                start = (k // seg) * seg
+                g, l = checkpoints.pop(start)
+                for q in range(start, k + 1):
+                    g, l, records[q] = self._step(g.clone(), l.clone(), q, knots, J, True)
+                del g, l
             rec = records[k]
             records[k] = None
-            if rec is None:
-                gam_k, lam_k = checkpoints[k]
-                _, _, rec = self._step(gam_k.clone(), lam_k.clone(), k, knots, J, True)
             tau_c, tau_m = rec["tau"]
             # mixing gates
             mg_bar = torch.empty_like(rec["mix"])
@@ -317,28 +461,34 @@ class AnnealingGradient:
                                        self.N, _lib.ptr(rec["mix"]), _lib.ptr(gam_bar), _lib.ptr(mg_bar), st))
             c, s = np.cos(tau_m), np.sin(tau_m)
             dG = np.array([[-s, -1j * c], [-1j * c, -s]])
-            dtau_m[k] = float(np.real(np.vdot(mg_bar.sum(dim=0).cpu().numpy(), dG)))
+            mg = mg_bar.cpu().numpy()
+            per_node = np.real(np.einsum("nij,ij->n", np.conj(mg), dG))
+            dtau_m[:, k] = np.bincount(self.node_inst, per_node, minlength=self.n_inst)
             # coupling gates, levels in reverse
-            hb_all = torch.zeros((self.E, 2, 16), dtype=torch.complex128, device=DEVICE)
+            hb_all = torch.zeros((self.n_pairs, 2, 16), dtype=torch.complex128, device=DEVICE)
             for o2, c2, (pg, pl), tape in reversed(rec["levels"]):
                 _lib.check(L.dq_simple_update_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(pg), _lib.ptr(pl),
                                                   _lib.ptr(self.d_meta[o2:]), c2, _lib.ptr(rec["halves"][o2:]), 2,
                                                   EPS, self.cut, _lib.ptr(tape), _lib.ptr(gam_bar),
                                                   _lib.ptr(lam_bar),
                                                   _lib.ptr(hb_all[o2:]), st))
-            hb = np.empty((self.E, 2, 16), dtype=np.complex128)
+            hb = np.empty((self.n_pairs, 2, 16), dtype=np.complex128)
             hb[self.order2] = hb_all.cpu().numpy()
             gbar = np.stack([hb[:, 1, 0], hb[:, 1, 3], hb[:, 1, 4], hb[:, 1, 7]], axis=1)  # (E, 4): g_00, g_01, g_10, g_11
             g, theta = self._diag_phases(tau_c, J)
             # g_c = exp(-i tau theta_c): dg/dtau = -i theta g, dg/dJ = -i tau zz_c g
-            dtau_c[k] = float(np.real(np.sum(np.conj(gbar) * (-1j * theta * g))))
+            per_pair = np.real(np.sum(np.conj(gbar) * (-1j * theta * g), axis=1))
+            dtau_c[:, k] = np.bincount(self.pair_inst, per_pair, minlength=self.n_inst)
             dJ += np.real(np.sum(np.conj(gbar) * (-1j * tau_c * _ZZ[None, :] * g), axis=1))
             if rec["regauge"] is not None:
                 gam_bar, lam_bar = self._regauge_vjp(rec["regauge"], gam_bar, lam_bar)
             if trace is not None:
                 trace.append((k, float(torch.linalg.vector_norm(gam_bar)), float(torch.linalg.vector_norm(lam_bar))))
-        dknots = self.W.T @ (self.dt * dtau_c - self.dt * dtau_m)
-        return energy, dknots, dJ
+        dknots = (self.dt * dtau_c - self.dt * dtau_m) @ self.W
+        if not self.batched:
+            return energy, dknots[0], dJ
+        cuts = np.cumsum([e.edges_number for e in self.efs])[:-1]
+        return energy, dknots, np.split(dJ, cuts)
 
 
 class AnnealingEnergy(torch.autograd.Function):
@@ -348,6 +498,8 @@ class AnnealingEnergy(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, runner: AnnealingGradient, knots: torch.Tensor, J: torch.Tensor):
+        if runner.batched:
+            raise ValueError("AnnealingEnergy takes a single-instance runner")
         e, dk, dj = runner.value_and_grad(knots.detach().cpu().numpy(), J.detach().cpu().numpy())
         ctx.save_for_backward(torch.from_numpy(dk).to(knots), torch.from_numpy(dj).to(J))
         return torch.tensor(e, dtype=torch.float64, device=knots.device)
