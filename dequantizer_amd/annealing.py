@@ -289,8 +289,10 @@ class Annealer:
         self.steps_done += 1
 
     def finish(self, two_site: bool = True) -> List[QuantumAnnealingResults]:
-        rho1 = self.emu.all_rdm1().cpu().numpy()
-        rho2 = self.emu.all_rdm2().cpu().numpy() if two_site else None
+        d1 = self.emu.all_rdm1()
+        d2 = self.emu.all_rdm2() if two_site else None
+        rho1 = d1.cpu().numpy()
+        rho2 = d2.cpu().numpy() if two_site else None
         topo = self.emu.topo
         out = []
         for inst, ef in enumerate(self.efs):
