@@ -489,8 +489,7 @@ static long long vidal_generic_ws_stride(const dq_graph* g) {
 // Workspace bytes of one Vidal-distance call on this handle, by kernel path.
 static size_t vidal_distance_ws_bytes(const dq_graph* g, int rb) {
   if (bp_fast_applicable(rb, g->dev, g->min_deg, g->max_deg)) return (size_t)vidal_fast_ws_bytes(g->dev);
-  if (g->dev.chi == 32 && big_applicable(g->dev, g->max_deg))
-    return (size_t)vidal_big_ws_bytes(g->dev, g->max_deg, rb);
+  if (big_applicable(g->dev, g->max_deg)) return (size_t)vidal_big_ws_bytes(g->dev, g->max_deg, rb);
   return (size_t)vidal_generic_ws_stride(g) * (g->dev.n_nodes > 0 ? g->dev.n_nodes : 1);
 }
 
@@ -499,7 +498,7 @@ static hipError_t vidal_distance_launch(dq_graph* g, int rb, const void* gammas,
   if (bp_fast_applicable(rb, g->dev, g->min_deg, g->max_deg))
     return launch_vidal_distance_fast(g->dev, gammas, lam, eps, g->d_all_nodes, g->d_zero_src, g->d_inst_edge_ptr,
                                       g->n_inst, out, ws, inst_active, s);
-  if (g->dev.chi == 32 && big_applicable(g->dev, g->max_deg))
+  if (big_applicable(g->dev, g->max_deg))
     return launch_vidal_distance_big(rb, g->dev, g->max_deg, gammas, lam, eps, g->d_all_nodes, g->d_zero_src,
                                      g->d_inst_edge_ptr, g->n_inst, out, ws, inst_active, s);
   return launch_vidal_distance(rb, g->dev, gammas, lam, eps, g->node_dist, g->d_inst_node_ptr, g->d_inst_edge_ptr,

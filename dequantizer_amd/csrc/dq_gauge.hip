@@ -533,8 +533,11 @@ hipError_t launch_vidal_distance(int rb, const DevGraph& g, const void* gam, con
 
 // Per (edge, side): |R/tr(R) - ideal|_* with ideal = diag(R_kk/tr > eps)/count,
 // R Hermitised (it is a Gram matrix up to rounding).  4 waves per block.
+// R in complex64 (the matrix-core path) or either precision (the streaming
+// path of large node tensors).
+template <typename RT>
 __global__ void __launch_bounds__(256)
-    k_vidal_nuclear16(DevGraph g, const cx<float>* R, double eps, double* es_dist, const int* inst_active) {
+    k_vidal_nuclear16(DevGraph g, const cx<RT>* R, double eps, double* es_dist, const int* inst_active) {
   __shared__ cd A[4][256];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long long es = (long long)blockIdx.x * 4 + w;
@@ -542,7 +545,7 @@ __global__ void __launch_bounds__(256)
   const int e = (int)(es >> 1);
   if (inst_active && !inst_active[g.node_inst[g.edge_node[2 * e]]]) return;
   cd* G = A[w];
-  const cx<float>* r = R + es * 256;
+  const cx<RT>* r = R + es * 256;
   for (int i = lane; i < 256; i += 64) G[i] = cvt<double>(r[i]);
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -636,8 +639,9 @@ __global__ void __launch_bounds__(256)
 
 long long vidal_fast_ws_bytes(const DevGraph& g) { return 2LL * g.n_edges * (256 * 8 + 8) + 256; }
 
-// Vidal distance for chi = 32 on the streaming node update: the lambda^2-
-// weighted bond Grams are node updates from diag(lambda^2) messages.
+// Vidal distance for large node tensors (chi 16 or 32) on the streaming node
+// update: the lambda^2-weighted bond Grams are node updates from
+// diag(lambda^2) messages.
 long long vidal_big_ws_bytes(const DevGraph& g, int max_deg, int rb) {
   const long long cc = (long long)g.chi * g.chi;
   return big_ws_bytes(g, max_deg, g.n_nodes, rb) + 2 * (2LL * g.n_edges * cc * 2 * rb) + 2LL * g.n_edges * 8 + 512;
@@ -647,7 +651,7 @@ hipError_t launch_vidal_distance_big(int rb, const DevGraph& g, int max_deg, con
                                      double eps, const int* all_nodes, const uint8_t* zero_src,
                                      const int* inst_edge_ptr, int n_inst, double* inst_dist, void* ws,
                                      const int* inst_active, hipStream_t s) {
-  if (g.n_edges <= 0 || g.chi != 32) return hipErrorInvalidValue;
+  if (g.n_edges <= 0 || (g.chi != 16 && g.chi != 32)) return hipErrorInvalidValue;
   const long long cc = (long long)g.chi * g.chi, nm = 2LL * g.n_edges * cc;
   char* base = (char*)ws + (big_ws_bytes(g, max_deg, g.n_nodes, rb) + 255) / 256 * 256;
   void* msg = base;
@@ -661,12 +665,19 @@ hipError_t launch_vidal_distance_big(int rb, const DevGraph& g, int max_deg, con
   if (err != hipSuccess) return err;
   err = launch_bp_nodes_big(rb, g, max_deg, all_nodes, g.n_nodes, gam, msg, msg, R, zero_src, ws, inst_active, s);
   if (err != hipSuccess) return err;
-  if (rb == 4)
-    hipLaunchKernelGGL(k_vidal_nuclear32<float>, dim3((unsigned)(2LL * g.n_edges)), dim3(256), 0, s, g,
-                       (const cx<float>*)R, eps, es_dist, inst_active);
+  const unsigned n16 = (unsigned)((2LL * g.n_edges + 3) / 4), n32 = (unsigned)(2LL * g.n_edges);
+  if (g.chi == 16 && rb == 4)
+    hipLaunchKernelGGL(k_vidal_nuclear16<float>, dim3(n16), dim3(256), 0, s, g, (const cx<float>*)R, eps, es_dist,
+                       inst_active);
+  else if (g.chi == 16)
+    hipLaunchKernelGGL(k_vidal_nuclear16<double>, dim3(n16), dim3(256), 0, s, g, (const cx<double>*)R, eps, es_dist,
+                       inst_active);
+  else if (rb == 4)
+    hipLaunchKernelGGL(k_vidal_nuclear32<float>, dim3(n32), dim3(256), 0, s, g, (const cx<float>*)R, eps, es_dist,
+                       inst_active);
   else
-    hipLaunchKernelGGL(k_vidal_nuclear32<double>, dim3((unsigned)(2LL * g.n_edges)), dim3(256), 0, s, g,
-                       (const cx<double>*)R, eps, es_dist, inst_active);
+    hipLaunchKernelGGL(k_vidal_nuclear32<double>, dim3(n32), dim3(256), 0, s, g, (const cx<double>*)R, eps, es_dist,
+                       inst_active);
   err = hipGetLastError();
   if (err != hipSuccess) return err;
   if (n_inst > 0)
@@ -687,8 +698,8 @@ hipError_t launch_vidal_distance_fast(const DevGraph& g, const void* gam, const 
   (void)zero_src;
   hipError_t err = launch_bp_nodes_lam2_fast(g, all_nodes, g.n_nodes, gam, lam, R, inst_active, s);
   if (err != hipSuccess) return err;
-  hipLaunchKernelGGL(k_vidal_nuclear16, dim3((unsigned)((2LL * g.n_edges + 3) / 4)), dim3(256), 0, s, g, R, eps,
-                     es_dist, inst_active);
+  hipLaunchKernelGGL(k_vidal_nuclear16<float>, dim3((unsigned)((2LL * g.n_edges + 3) / 4)), dim3(256), 0, s, g, R,
+                     eps, es_dist, inst_active);
   err = hipGetLastError();
   if (err != hipSuccess) return err;
   if (n_inst > 0)

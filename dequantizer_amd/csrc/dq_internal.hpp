@@ -182,11 +182,39 @@ __device__ inline int find_slot(const DevGraph& g, int node, int e, int side) {
 // dst = src with matrix M applied on axis j of a tensor with `d` bonds of
 // size chi and a trailing phys axis: dst[o, a, in] = sum_b M[a*chi+b] src[o, b, in]
 // (transposed=true uses M[b*chi+a]).  Whole block cooperates.
+// For chi <= 32 each thread takes whole columns (all indices but axis j): the
+// chi values of a column are read once into registers and the chi outputs
+// written from them, so the tensor is read and written once per product
+// (the element loop below reads every input chi times).  Both forms sum over
+// b in the same order: the results are bitwise equal.
 template <typename R, typename MT>
 __device__ void mode_product(cx<R>* dst, const cx<R>* src, const cx<MT>* M, int chi, int d, int phys, int j,
                              bool transposed) {
   const long long inner = ipow(chi, d - 1 - j) * phys;
   const long long total = ipow(chi, d) * phys;
+  constexpr int kReg = 32;
+  if (chi <= kReg) {
+    const long long cols = total / chi;
+    for (long long c = threadIdx.x; c < cols; c += blockDim.x) {
+      const long long o = c / inner, i = c - o * inner;
+      const long long b0 = o * chi * inner + i;
+      cx<R> in[kReg];
+#pragma unroll
+      for (int b = 0; b < kReg; ++b)
+        if (b < chi) in[b] = src[b0 + b * inner];
+      for (int a = 0; a < chi; ++a) {
+        cx<R> acc = mk<R>(0, 0);
+#pragma unroll
+        for (int b = 0; b < kReg; ++b)
+          if (b < chi) {
+            const cx<MT> m = transposed ? M[(long long)b * chi + a] : M[(long long)a * chi + b];
+            fma_acc(acc, cvt<R>(m), in[b]);
+          }
+        dst[b0 + a * inner] = acc;
+      }
+    }
+    return;
+  }
   for (long long f = threadIdx.x; f < total; f += blockDim.x) {
     const long long in = f % inner;
     const long long t = f / inner;

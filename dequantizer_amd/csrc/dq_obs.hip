@@ -51,15 +51,20 @@ __global__ void __launch_bounds__(256)
   const cx<R>* ket = apply_incoming<R>(g, t, e2n, node, -1, w0, w1);
   cd* out = (cd*)(ws + (long long)blockIdx.x * wsb + 2 * S * (long long)sizeof(cx<double>));
   const long long rows = S / p;
-  for (int q = threadIdx.x; q < p * p; q += blockDim.x) {
+  // rho[s][s'] = sum over rows of ket[row, s] conj(t[row, s']): every thread
+  // sums a strided share of the rows, then one block reduction per entry
+  __shared__ double red[8];
+  for (int q = 0; q < p * p; ++q) {
     const int s = q / p, sp = q % p;
     double re = 0.0, im = 0.0;
-    for (long long a = 0; a < rows; ++a) {
+    for (long long a = threadIdx.x; a < rows; a += blockDim.x) {
       const cx<R> x = ket[a * p + s], y = t[a * p + sp];
       re += (double)x.re * y.re + (double)x.im * y.im;
       im += (double)x.im * y.re - (double)x.re * y.im;
     }
-    out[q] = mk<double>(re, im);
+    re = block_sum(re, red);
+    im = block_sum(im, red);
+    if (threadIdx.x == 0) out[q] = mk<double>(re, im);
   }
   __syncthreads();
   cd tr = mk<double>(0, 0);
@@ -68,20 +73,64 @@ __global__ void __launch_bounds__(256)
 }
 
 // E[(k s), (k' s')] = sum over the bonds of `node` other than `slot` of
-// ket[..k..s] * conj(t[..k'..s']).
+// ket[..k..s] * conj(t[..k'..s']).  Rows of the other bonds are staged through
+// LDS in tiles (their offsets decoded once per element), each thread keeping
+// up to 16 entries of E in registers (cp = chi * phys <= 64); wider shapes
+// take the element loop.
 template <typename R>
 __device__ void half_env(const DevGraph& g, const cx<R>* ket, const cx<R>* t, int node, int slot, cd* E) {
   const int d = g.deg[node], chi = g.chi, p = g.phys;
   const long long inner = ipow(chi, d - 1 - slot) * p;  // stride of bond `slot`
   const long long orows = ipow(chi, d - 1);              // other-bond combinations
+  const long long inner_b = inner / p;
   const int cp = chi * p;
+  constexpr int kRows = 16, kMaxCp = 64, kPer = kMaxCp * kMaxCp / 256;
+  if (cp <= kMaxCp && blockDim.x == 256) {
+    __shared__ cd xs[kRows * kMaxCp], ys[kRows * kMaxCp];
+    cd acc[kPer];
+#pragma unroll
+    for (int v = 0; v < kPer; ++v) acc[v] = mk<double>(0, 0);
+    for (long long o0 = 0; o0 < orows; o0 += kRows) {
+      const int nr = orows - o0 < kRows ? (int)(orows - o0) : kRows;
+      for (int f = threadIdx.x; f < nr * cp; f += blockDim.x) {
+        const int rr = f / cp, col = f - rr * cp;
+        const int k = col / p, s = col - k * p;
+        const long long o = o0 + rr;
+        const long long ob = o / inner_b, ib = o - ob * inner_b;
+        const long long idx = (ob * chi + k) * inner + ib * p + s;
+        xs[rr * cp + col] = cvt<double>(ket[idx]);
+        ys[rr * cp + col] = cvt<double>(t[idx]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int v = 0; v < kPer; ++v) {
+        const int q = threadIdx.x + 256 * v;
+        if (q < cp * cp) {
+          const int a = q / cp, b = q - a * cp;
+          double re = acc[v].re, im = acc[v].im;
+          for (int rr = 0; rr < nr; ++rr) {
+            const cd x = xs[rr * cp + a], y = ys[rr * cp + b];
+            re += x.re * y.re + x.im * y.im;
+            im += x.im * y.re - x.re * y.im;
+          }
+          acc[v] = mk<double>(re, im);
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int v = 0; v < kPer; ++v) {
+      const int q = threadIdx.x + 256 * v;
+      if (q < cp * cp) E[q] = acc[v];
+    }
+    return;
+  }
   for (int q = threadIdx.x; q < cp * cp; q += blockDim.x) {
     const int a = q / cp, b = q % cp;
     const int k = a / p, s = a % p, kp = b / p, sp = b % p;
     double re = 0.0, im = 0.0;
     for (long long o = 0; o < orows; ++o) {
       // o enumerates (outer bonds before slot, inner bonds after slot)
-      const long long inner_b = inner / p;
       const long long ob = o / inner_b, ib = o % inner_b;
       const long long i1 = (ob * chi + k) * inner + ib * p + s;
       const long long i2 = (ob * chi + kp) * inner + ib * p + sp;

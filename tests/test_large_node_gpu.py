@@ -86,3 +86,34 @@ def test_degree6_chi16_vidal_gauge_matches_oracle(star_case):
         r_o = om.one_side_density_matrix(node, t_o, m_o)
         r_p = om.one_side_density_matrix(node, t_p, m_p)
         assert np.abs(r_o - r_p).max() < 1e-9, node.id
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-9), (np.complex64, 2e-5)])
+def test_degree6_chi16_distance_and_density_matrices(star_case, dtype, tol):
+    """The other per-node operations at the 33.5 M-element centre against the
+    oracle: the Vidal distance (streaming bond Grams with diag(lambda^2)
+    messages) of arbitrary tensors and lambdas, where every bond term is far
+    from zero, and the one- and two-site density matrices from the BP
+    messages (tiled environments)."""
+    gp, tensors, msgs, ref = star_case
+    go = _star()[1]
+    trav_p, trav_o = list(gp.get_traversal_iterator()), go.traversal()
+    rng = np.random.default_rng(5)
+    lam = {}
+    for eid in go.edges:
+        v = np.sort(rng.uniform(0.05, 1.0, 16))[::-1]
+        lam[eid] = v / np.linalg.norm(v)
+    ten = {k: v.astype(dtype) for k, v in tensors.items()}
+    d_o = om.vidal_gauge_distance_map(trav_o, 1e-10)(tensors, lam)
+    d_p = float(dq.get_vidal_gauge_distance_map(trav_p, 1e-10)(ten, lam))
+    assert d_o > 0.1, d_o
+    assert abs(d_p - d_o) < tol * d_o, (d_p, d_o)
+    out = dq.get_belief_propagation_map(trav_p, True)(ten, {k: v.astype(dtype) for k, v in msgs.items()})
+    for nid in (0, 3):
+        r_p = host(dq.get_one_side_density_matrix(gp.nodes[nid], ten, out))
+        r_o = om.one_side_density_matrix(go.nodes[nid], tensors, ref)
+        assert np.abs(r_p - r_o).max() < tol, nid
+    for eid in ((0, 1), (0, 6)):
+        r_p = host(dq.get_two_side_density_matrix(gp.edges[eid], ten, out))
+        r_o = om.two_side_density_matrix(go.edges[eid], tensors, ref)
+        assert np.abs(r_p - r_o).max() < tol, eid
