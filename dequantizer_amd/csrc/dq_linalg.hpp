@@ -77,6 +77,31 @@ __device__ __forceinline__ bool jacobi_params(double app, double aqq, cd apq, do
   return true;
 }
 
+// The same rotation with a shorter dependent chain (two reciprocal square
+// roots in sequence instead of three transcendentals): with d = (aqq - app)/2,
+// m = |apq| and r = sqrt(d^2 + m^2), t = sign(d) m / (|d| + r), and
+// 1 + t^2 = 2r / (r + |d|), so c = sqrt((r + |d|) / (2r)) and
+// s = t c = sign(d) m / (2 r c).  Same thresholds; c^2 + s^2 = 1 to rounding.
+// The sign convention of d = 0 is that of tau = 0 above (t > 0).
+__device__ __forceinline__ bool jacobi_params_r(double app, double aqq, cd apq, double& c, double& s, double& er,
+                                                double& ei, double abs_tol) {
+  const double n2 = norm2(apq);
+  if (!(n2 > 1e-26 * fabs(app * aqq)) || !(n2 > abs_tol * abs_tol)) return false;
+  if (!(n2 > 1e-300)) return false;
+  const double inv = rsqrt_nr(n2);  // 1 / m, in parallel with the chain below
+  er = apq.re * inv;
+  ei = -apq.im * inv;
+  const double d = 0.5 * (aqq - app);
+  const double h = fma(d, d, n2);
+  const double y = rsqrt_nr(h);  // 1 / r
+  const double c2 = 0.5 * fma(h, y, fabs(d)) * y;
+  const double z = rsqrt_nr(c2);  // 1 / c
+  c = c2 * z;
+  const double sm = 0.5 * (n2 * inv) * y * z;
+  s = d < 0.0 ? -sm : sm;
+  return true;
+}
+
 // A (n x n row-major Hermitian) -> eigenvalues on the diagonal; V columns =
 // eigenvectors.  rot: 4*ceil(n/2) + 1 doubles; flag: one int.
 __device__ inline void jacobi_eigh(cd* A, cd* V, int n, double* rot, int* flag) {
@@ -625,7 +650,12 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
   f = block_sum(f, scal);
   const double abs_tol = 1e-32 * f;
   for (int sweep = 0; sweep < 60; ++sweep) {
-    int rotated = 0;
+    // rotated: any rotation this sweep; large: any with |g| above 1e-8 of
+    // sqrt(al be).  A sweep whose rotations were all below 1e-8 leaves
+    // off-diagonal terms of order 1e-16 (quadratic convergence), under the
+    // 1e-13 threshold of the rotation test: the sweep that would only
+    // confirm convergence is skipped.
+    int rotated = 0, large = 0;
     for (int r = 0; r < N - 1; ++r) {
       int p, q;
       rr_pair(r, k, N, p, q);
@@ -645,13 +675,11 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
         gr += x[t].re * y[t].re + x[t].im * y[t].im;
         gi += x[t].re * y[t].im - x[t].im * y[t].re;
       }
-      al = group_sum<LPP>(al);
-      be = group_sum<LPP>(be);
-      gr = group_sum<LPP>(gr);
-      gi = group_sum<LPP>(gi);
+      group_sum4<LPP>(al, be, gr, gi);
       double c, s, er, ei;
-      if (jacobi_params(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
+      if (jacobi_params_r(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
         rotated = 1;
+        if (gr * gr + gi * gi > 1e-16 * al * be) large = 1;
         const cd e = mk<double>(er, ei);
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
@@ -665,7 +693,7 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
       }
       __syncthreads();
     }
-    if (!__syncthreads_or(rotated)) break;
+    if (!__syncthreads_or(large)) break;
   }
   if (threadIdx.x < N) {
     double nrm = 0.0;
@@ -972,13 +1000,26 @@ __device__ inline void block_qrcp_rh32(const P* C, P* Hk, P* X, I* piv) {
 #pragma unroll
     for (int q = 1; q < 4; ++q) take(best, j, cand_v[q], cand_j[q]);
     const cd alpha = cld(xk_col, j);
-    const double nrm = best > 0.0 ? sqrt(best) : 0.0;
-    const double aa = sqrt(norm2(alpha));
+    // beta = -e^{i arg alpha} |x|, scale = 1 / sqrt(|x| (|x| + |alpha|)):
+    // reciprocal square roots refined by a Newton step (about an ulp, as in
+    // the Jacobi rotations) instead of correctly rounded square roots and
+    // divisions on the critical path of every step
+    // (columns with |x|^2 <= 1e-300 count as zero: no reflector)
+    const bool live = best > 1e-300;
     cd beta = mk<double>(0.0, 0.0);
     double scale = 0.0;
-    if (best > 0.0) {
-      beta = aa > 0.0 ? mk<double>(-alpha.re / aa * nrm, -alpha.im / aa * nrm) : mk<double>(-nrm, 0.0);
-      scale = sqrt(2.0) / sqrt(2.0 * nrm * (nrm + aa));
+    if (live) {
+      const double nrm = best * rsqrt_nr(best);
+      const double a2 = norm2(alpha);
+      double aa = 0.0;
+      if (a2 > 1e-300) {
+        const double ia = rsqrt_nr(a2);
+        aa = a2 * ia;
+        beta = mk<double>(-alpha.re * ia * nrm, -alpha.im * ia * nrm);
+      } else {
+        beta = mk<double>(-nrm, 0.0);
+      }
+      scale = rsqrt_nr(nrm * (nrm + aa));
     }
     if (c == j) {
 #pragma unroll
@@ -990,7 +1031,7 @@ __device__ inline void block_qrcp_rh32(const P* C, P* Hk, P* X, I* piv) {
       mypos = k;
     }
     __syncthreads();
-    if (mine && c != j && best > 0.0) {
+    if (mine && c != j && live) {
       cd u[4];
       double dr = 0.0, di = 0.0;
 #pragma unroll
