@@ -188,8 +188,12 @@ __global__ void __launch_bounds__(kBigThreads)
   if (inst_active && !inst_active[g.node_inst[node]]) return;
   const int d = g.deg[node];
   if (d < 1) return;
+  // the node's squared norm: the chunk sums split over the block, then one
+  // block reduction (uniform per block: the returns above depend on the node)
+  __shared__ double red[kBigThreads / 64];
   double s = 0.0;
-  for (int c = 0; c < chunks; ++c) s += part[(long long)k * chunks + c];
+  for (int c = threadIdx.x; c < chunks; c += blockDim.x) s += part[(long long)k * chunks + c];
+  s = block_sum(s, red);
   const R inv = (R)(1.0 / sqrt(s));
   const long long S = ipow(chi, d) * g.phys;
   const cx<R>* x = buf + (2LL * k + ((d - 1) & 1)) * smax;
@@ -203,8 +207,12 @@ static int big_col_blocks(long long cols) {
   long long b = (cols + kBigThreads - 1) / kBigThreads;
   return (int)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
 }
+// Gram partials per node: at least 512 columns per chunk, so that small node
+// tensors (degree 3 at chi 32: 2048 columns) write a few partial Grams
+// instead of one per 64-column tile; at most 256 chunks.
+constexpr int kBigGramCols = 512;
 static int big_gram_chunks(long long cols) {
-  long long b = (cols + kBigTile - 1) / kBigTile;
+  long long b = (cols + kBigGramCols - 1) / kBigGramCols;
   return (int)(b < 1 ? 1 : (b > 256 ? 256 : b));
 }
 

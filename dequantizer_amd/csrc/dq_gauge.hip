@@ -291,6 +291,140 @@ __global__ void __launch_bounds__(128)
   }
 }
 
+// The same decomposition for chi = 32, one edge per 256-thread block, four
+// 32 x 32 complex128 matrices in LDS (64 KiB: two blocks per CU).  Waves 0
+// and 1 diagonalise the two messages at once (wave-synchronous two-sided
+// Jacobi); the square roots are applied in factored form,
+//   G = Sf Sb^T = Qf (Df^1/2 Qf^H conj(Qb) Db^1/2) Qb^T,
+// as three small products; the block-level one-sided Jacobi takes the SVD of
+// G (its left vectors U go to the edge's slot of the workspace); then
+//   Xf = Qf (Df^-1/2 Qf^H U),  Xb = Qb (Db^-1/2 Qb^H conj(V)).
+// Same results as k_edge_svd to rounding (no adjoint tape: the generic kernel
+// records it).
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_edge_svd32(DevGraph g, const cx<R>* n2e, double eps, cd* X, double* lam, char* ws, long long wsb,
+                 const int* inst_active) {
+  constexpr int n = 32, cc = 1024;
+  __shared__ cd Mf[cc], Mb[cc], Qf[cc], Qb[cc];
+  __shared__ double wf[n], wb[n], sigma[n], scal[2 * n], rt[4][n], red[8];
+  __shared__ int perm[n];
+  const int e = blockIdx.x;
+  if (inst_active && !inst_active[g.edge_inst[e]]) return;
+  const int tid = threadIdx.x;
+  cd* U = (cd*)(ws + (long long)e * wsb);
+  const cx<R>* m0 = n2e + (2LL * e) * cc;
+  const cx<R>* m1 = n2e + (2LL * e + 1) * cc;
+  // Hermitian part (x + x^H) / 2, as the reference's eigh symmetrises its input
+  for (int q = tid; q < cc; q += blockDim.x) {
+    const int k = q / n, l = q % n;
+    const cd a = cvt<double>(m0[q]), at = cvt<double>(m0[l * n + k]);
+    const cd b = cvt<double>(m1[q]), bt = cvt<double>(m1[l * n + k]);
+    Mf[q] = mk<double>(0.5 * (a.re + at.re), 0.5 * (a.im - at.im));
+    Mb[q] = mk<double>(0.5 * (b.re + bt.re), 0.5 * (b.im - bt.im));
+  }
+  __syncthreads();
+  if (tid < 64)
+    wave_jacobi_eigh_n<n>(Mf, Qf);
+  else if (tid < 128)
+    wave_jacobi_eigh_n<n>(Mb, Qb);
+  __syncthreads();
+  if (tid < n) {
+    wf[tid] = Mf[tid * (n + 1)].re;
+    wb[tid] = Mb[tid * (n + 1)].re;
+  }
+  __syncthreads();
+  if (tid < 4 * n) {
+    const int which = tid / n, i = tid % n;
+    const double w = (which & 1) ? wb[i] : wf[i];
+    rt[which][i] = which < 2 ? sqrt(safe_truncate(w, eps)) : sqrt(safe_inverse(w, eps));
+  }
+  __syncthreads();
+  // P = Df^1/2 Qf^H conj(Qb) Db^1/2 -> Mf (Qf[k][i]: component k of eigenvector i)
+  for (int q = tid; q < cc; q += blockDim.x) {
+    const int i = q / n, j = q % n;
+    cd acc = mk<double>(0, 0);
+#pragma unroll 8
+    for (int k = 0; k < n; ++k) fma_acc(acc, conj(Qf[k * n + i]), conj(Qb[k * n + j]));
+    Mf[q] = (rt[0][i] * rt[1][j]) * acc;
+  }
+  __syncthreads();
+  // H = Qf P -> Mb
+  for (int q = tid; q < cc; q += blockDim.x) {
+    const int r = q / n, j = q % n;
+    cd acc = mk<double>(0, 0);
+#pragma unroll 8
+    for (int i = 0; i < n; ++i) fma_acc(acc, Qf[r * n + i], Mf[i * n + j]);
+    Mb[q] = acc;
+  }
+  __syncthreads();
+  // G = H Qb^T -> Mf, column-major (G[c * n + r]) for the Jacobi SVD
+  for (int q = tid; q < cc; q += blockDim.x) {
+    const int c = q / n, r = q % n;
+    cd acc = mk<double>(0, 0);
+#pragma unroll 8
+    for (int j = 0; j < n; ++j) fma_acc(acc, Mb[r * n + j], Qb[c * n + j]);
+    Mf[q] = acc;
+  }
+  __syncthreads();
+  block_jacobi_svd_n<32>((lds_cd*)Mf, U, (lds_cd*)Mb, sigma, perm, scal);
+  cd* V = Mb;
+  double nrm = 0.0;
+  for (int j = 0; j < n; ++j) nrm += sigma[j] * sigma[j];
+  nrm = sqrt(nrm);
+  // masked Frobenius norms of U and Vh
+  double pu = 0.0, pv = 0.0;
+  for (int q = tid; q < cc; q += blockDim.x) {
+    const int j = q / n, i = q % n;
+    const double lj = nrm > 0.0 ? sigma[j] / nrm : 0.0;
+    if (safe_truncate(lj, eps) > eps) {
+      pu += norm2(U[j * n + i]);
+      pv += norm2(V[perm[j] * n + i]);
+    }
+  }
+  const double nu = sqrt(block_sum(pu, red));
+  const double nv = sqrt(block_sum(pv, red));
+  for (int j = tid; j < n; j += blockDim.x) {
+    const double lj = nrm > 0.0 ? safe_truncate(sigma[j] / nrm, eps) : 0.0;
+    lam[(long long)e * n + j] = lj;
+  }
+  // T1 = Df^-1/2 Qf^H U -> Mf (G is no longer needed), then Xf = Qf T1 / nu;
+  // T2 = Db^-1/2 Qb^H conj(V) -> Mf, then Xb = Qb T2 / nv.  Columns of
+  // truncated singular values are zero.
+  for (int side = 0; side < 2; ++side) {
+    const cd* Q = side == 0 ? Qf : Qb;
+    const double* ri = rt[2 + side];
+    for (int q = tid; q < cc; q += blockDim.x) {
+      const int m = q / n, k = q % n;
+      cd acc = mk<double>(0, 0);
+      if (side == 0) {
+#pragma unroll 8
+        for (int i = 0; i < n; ++i) fma_acc(acc, conj(Q[i * n + m]), U[k * n + i]);
+      } else {
+        const int pk = perm[k];
+#pragma unroll 8
+        for (int i = 0; i < n; ++i) fma_acc(acc, conj(Q[i * n + m]), conj(V[pk * n + i]));
+      }
+      Mf[q] = ri[m] * acc;
+    }
+    __syncthreads();
+    cd* Xs = X + (2LL * e + side) * cc;
+    const double sc = 1.0 / (side == 0 ? nu : nv);
+    for (int q = tid; q < cc; q += blockDim.x) {
+      const int b = q / n, k = q % n;
+      const double lk = nrm > 0.0 ? safe_truncate(sigma[k] / nrm, eps) : 0.0;
+      cd acc = mk<double>(0, 0);
+      if (lk > eps) {
+#pragma unroll 8
+        for (int m = 0; m < n; ++m) fma_acc(acc, Q[b * n + m], Mf[m * n + k]);
+        acc = sc * acc;
+      }
+      Xs[q] = acc;
+    }
+    __syncthreads();
+  }
+}
+
 template <typename R>
 __global__ void __launch_bounds__(256)
     k_absorb(DevGraph g, const cx<R>* T, const cd* X, cx<R>* gam, char* ws, long long wsb, const int* inst_active) {
@@ -376,7 +510,7 @@ __global__ void __launch_bounds__(256)
     if (chi == 32 && blockDim.x == 256) {
       __shared__ double sg[32];
       __shared__ int pm[32];
-      block_jacobi_svd_n<32>(A, Us, Vs, sg, pm, rot);
+      block_jacobi_svd_n<32, cd, false>(A, nullptr, nullptr, sg, pm, rot);
       for (int k = 0; k < 32; ++k) dist += sg[k];
     } else if (chi == 16) {
       __shared__ double sg16[16];
@@ -480,6 +614,15 @@ hipError_t launch_edge_svd(int rb, const DevGraph& g, const void* n2e, double ep
     else
       hipLaunchKernelGGL(k_edge_svd16<double>, dim3(g.n_edges), dim3(128), 0, s, g, (const cx<double>*)n2e, eps, X,
                          lam, inst_active, (char*)tape);
+    return hipGetLastError();
+  }
+  if (g.chi == 32 && !tape && wsb >= 1024LL * (long long)sizeof(cd)) {
+    if (rb == 4)
+      hipLaunchKernelGGL(k_edge_svd32<float>, dim3(g.n_edges), dim3(256), 0, s, g, (const cx<float>*)n2e, eps, X, lam,
+                         (char*)ws, wsb, inst_active);
+    else
+      hipLaunchKernelGGL(k_edge_svd32<double>, dim3(g.n_edges), dim3(256), 0, s, g, (const cx<double>*)n2e, eps, X,
+                         lam, (char*)ws, wsb, inst_active);
     return hipGetLastError();
   }
   if (rb == 4)
@@ -587,11 +730,12 @@ __global__ void __launch_bounds__(256) k_lam2_diag_any(DevGraph g, const double*
 
 // The nuclear-norm term of one (edge, side) at chi = 32, one block each:
 // R Hermitised, the ideal subtracted, singular values by the block-level
-// one-sided Jacobi (sum of |eigenvalues| of a Hermitian matrix).
+// one-sided Jacobi without singular vectors (sum of |eigenvalues| of a
+// Hermitian matrix); 16 KiB of LDS per block.
 template <typename R>
 __global__ void __launch_bounds__(256)
     k_vidal_nuclear32(DevGraph g, const cx<R>* Rm, double eps, double* es_dist, const int* inst_active) {
-  __shared__ cd G[1024], U[1024], V[1024];
+  __shared__ cd G[1024];
   __shared__ double sg[32], scal[64];
   __shared__ int perm[32];
   const long long es = blockIdx.x;
@@ -616,7 +760,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int t = 0; t < 4; ++t) G[threadIdx.x + 256 * t] = h[t];
   __syncthreads();
-  block_jacobi_svd_n<32>((lds_cd*)G, U, (lds_cd*)V, sg, perm, scal);
+  block_jacobi_svd_n<32, lds_cd, false>((lds_cd*)G, nullptr, nullptr, sg, perm, scal);
   if (threadIdx.x == 0) {
     double nuc = 0.0;
     for (int k = 0; k < 32; ++k) nuc += sg[k];

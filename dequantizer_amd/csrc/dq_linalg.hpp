@@ -637,14 +637,17 @@ __device__ inline double wave_nuclear_norm_n(cd* G) {
 // Block-parallel fixed-size one-sided Jacobi SVD (N = 32, 256 threads): the
 // 16 pairs of a round run on 16-lane groups (4 per wave), 2 rows per lane, one
 // block barrier per round.  Same outputs as jacobi_svd.  scal: N doubles.
-template <int N, typename P>
+// kVectors = false: singular values only (sigma, perm); V is neither formed
+// nor read, U is not written (both may be null), G is overwritten.
+template <int N, typename P, bool kVectors = true>
 __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int* perm, double* scal) {
   constexpr int NP = N / 2;
   constexpr int LPP = 256 / NP;  // 16 lanes per pair for N = 32
   constexpr int RPL = N / LPP;   // rows per lane
   static_assert(LPP <= 64 && RPL >= 1, "block SVD sizing");
   const int k = threadIdx.x / LPP, sub = threadIdx.x % LPP;
-  for (int i = threadIdx.x; i < N * N; i += blockDim.x) cst(V, i, mk<double>((i / N) == (i % N) ? 1.0 : 0.0, 0.0));
+  if (kVectors)
+    for (int i = threadIdx.x; i < N * N; i += blockDim.x) cst(V, i, mk<double>((i / N) == (i % N) ? 1.0 : 0.0, 0.0));
   double f = 0.0;
   for (int i = threadIdx.x; i < N * N; i += blockDim.x) f += norm2(cld(G, i));
   f = block_sum(f, scal);
@@ -664,8 +667,10 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
       for (int t = 0; t < RPL; ++t) {
         x[t] = cld(G, p * N + sub + t * LPP);
         y[t] = cld(G, q * N + sub + t * LPP);
-        vx[t] = cld(V, p * N + sub + t * LPP);
-        vy[t] = cld(V, q * N + sub + t * LPP);
+        if (kVectors) {
+          vx[t] = cld(V, p * N + sub + t * LPP);
+          vy[t] = cld(V, q * N + sub + t * LPP);
+        }
       }
       double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
 #pragma unroll
@@ -686,9 +691,11 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
           const cd ey = e * y[t];
           cst(G, p * N + sub + t * LPP, mk<double>(c * x[t].re - s * ey.re, c * x[t].im - s * ey.im));
           cst(G, q * N + sub + t * LPP, mk<double>(s * x[t].re + c * ey.re, s * x[t].im + c * ey.im));
-          const cd evy = e * vy[t];
-          cst(V, p * N + sub + t * LPP, mk<double>(c * vx[t].re - s * evy.re, c * vx[t].im - s * evy.im));
-          cst(V, q * N + sub + t * LPP, mk<double>(s * vx[t].re + c * evy.re, s * vx[t].im + c * evy.im));
+          if (kVectors) {
+            const cd evy = e * vy[t];
+            cst(V, p * N + sub + t * LPP, mk<double>(c * vx[t].re - s * evy.re, c * vx[t].im - s * evy.im));
+            cst(V, q * N + sub + t * LPP, mk<double>(s * vx[t].re + c * evy.re, s * vx[t].im + c * evy.im));
+          }
         }
       }
       __syncthreads();
@@ -709,6 +716,7 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
     sigma[rank] = me;
   }
   __syncthreads();
+  if (!kVectors) return;
   for (int t = threadIdx.x; t < N * N; t += blockDim.x) {
     const int j = t / N, i = t % N;
     const double sg = sigma[j];
