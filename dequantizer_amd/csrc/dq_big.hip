@@ -7,15 +7,17 @@
 // level at once:
 //   mode product   dst = src x_axis M: one thread per column (all indices but
 //                  the axis), the chi values of the column in registers, M in
-//                  LDS; grid = column blocks x nodes.  Output slot idx of a
-//                  degree-d node takes the d - 1 products of the other axes
-//                  (product t acts on axis t, or t + 1 from idx on), ping-
-//                  ponging between two per-node buffers.
+//                  LDS; grid = column blocks x nodes.  The products of a node
+//                  update follow a plan that shares partial products between
+//                  output slots (big_plan: 16 products at degree 6 where each
+//                  slot on its own takes 5), in per-node buffers.
 //   Gram partial   conj(T) . U over a chunk of columns per block, columns
 //                  staged through LDS in tiles of 64, FP64 accumulation.
 //   Gram reduce    per node: sum of the chunks, division by the trace.
 // Each element of T and of the intermediates is read once per pass: the
 // launches are bound by HBM bandwidth.
+#include <vector>
+
 #include "dq_launch.hpp"
 
 namespace dq {
@@ -23,12 +25,56 @@ namespace dq {
 constexpr int kBigThreads = 256;
 constexpr int kBigTile = 64;  // Gram columns per LDS tile
 
-__host__ __device__ inline int big_axis(int t, int idx) { return t < idx ? t : t + 1; }
-
-template <typename R, int CHI, bool kAbsorb>
+// Absorb product t of a node (axis t): dst = src x_t X^T, ping-ponging
+// between the node's first two buffers (stride nb buffers per node).
+template <typename R, int CHI>
 __global__ void __launch_bounds__(kBigThreads)
-    k_big_mode_product(DevGraph g, const int* nodes, int t, int idx, const cx<R>* T, const cx<R>* n2e_old,
-                       const cx<R>* e2n_old, const cx<R>* n2e_new, const uint8_t* node_src, const cd* X,
+    k_big_absorb_product(DevGraph g, const int* nodes, int t, const cx<R>* T, const cd* X, cx<R>* buf, long long smax,
+                         int nb, const int* inst_active) {
+  constexpr int CC = CHI * CHI;
+  __shared__ cx<R> Ms[CC];
+  const int k = blockIdx.y;
+  const int node = nodes[k];
+  if (inst_active && !inst_active[g.node_inst[node]]) return;
+  const int d = g.deg[node];
+  const int axis = t;
+  if (axis >= d) return;
+  const int base = g.adj_ptr[node];
+  const long long S = ipow(CHI, d) * g.phys;
+  const long long inner = ipow(CHI, d - 1 - axis) * g.phys;
+  const long long cols = S / CHI;
+  const int e = g.adj_edge[base + axis], sd = g.adj_side[base + axis];
+  // Ms[a][b] = X[b][a]: the orthogonaliser's column index is the new bond
+  const cd* x = X + (2LL * e + sd) * CC;
+  for (int q = threadIdx.x; q < CC; q += blockDim.x) Ms[q] = cvt<R>(x[(q % CHI) * CHI + q / CHI]);
+  __syncthreads();
+  const cx<R>* src = t == 0 ? T + g.toff[node] : buf + ((long long)nb * k + ((t - 1) & 1)) * smax;
+  cx<R>* dst = buf + ((long long)nb * k + (t & 1)) * smax;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < cols;
+       c += (long long)gridDim.x * blockDim.x) {
+    const long long o = c / inner, i = c - o * inner;
+    const long long b0 = o * CHI * inner + i;
+    cx<R> in[CHI];
+#pragma unroll
+    for (int b = 0; b < CHI; ++b) in[b] = src[b0 + b * inner];
+#pragma unroll 2
+    for (int a = 0; a < CHI; ++a) {
+      cx<R> acc = mk<R>(0, 0);
+#pragma unroll
+      for (int b = 0; b < CHI; ++b) fma_acc(acc, Ms[a * CHI + b], in[b]);
+      dst[b0 + a * inner] = acc;
+    }
+  }
+}
+
+// One product of a node-update plan: buffer dst = (buffer src, or T when
+// src < 0) x_axis M_axis, buffers nb per node.  For a node of degree <= axis
+// the product is the identity (a copy): one plan serves every degree of a
+// level.
+template <typename R, int CHI>
+__global__ void __launch_bounds__(kBigThreads)
+    k_big_plan_product(DevGraph g, const int* nodes, int axis, int src_id, int dst_id, int nb, const cx<R>* T,
+                       const cx<R>* n2e_old, const cx<R>* e2n_old, const cx<R>* n2e_new, const uint8_t* node_src,
                        cx<R>* buf, long long smax, const int* inst_active) {
   constexpr int CC = CHI * CHI;
   __shared__ cx<R> Ms[CC];
@@ -36,27 +82,25 @@ __global__ void __launch_bounds__(kBigThreads)
   const int node = nodes[k];
   if (inst_active && !inst_active[g.node_inst[node]]) return;
   const int d = g.deg[node];
-  const int axis = kAbsorb ? t : big_axis(t, idx);
-  if (axis >= d || (!kAbsorb && idx >= d)) return;
-  const int base = g.adj_ptr[node];
   const long long S = ipow(CHI, d) * g.phys;
+  const cx<R>* src = src_id < 0 ? T + g.toff[node] : buf + ((long long)nb * k + src_id) * smax;
+  cx<R>* dst = buf + ((long long)nb * k + dst_id) * smax;
+  if (axis >= d) {
+    for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < S;
+         f += (long long)gridDim.x * blockDim.x)
+      dst[f] = src[f];
+    return;
+  }
+  const int base = g.adj_ptr[node];
   const long long inner = ipow(CHI, d - 1 - axis) * g.phys;
   const long long cols = S / CHI;
   const int e = g.adj_edge[base + axis], sd = g.adj_side[base + axis];
-  if (kAbsorb) {
-    // Ms[a][b] = X[b][a]: the orthogonaliser's column index is the new bond
-    const cd* x = X + (2LL * e + sd) * CC;
-    for (int q = threadIdx.x; q < CC; q += blockDim.x) Ms[q] = cvt<R>(x[(q % CHI) * CHI + q / CHI]);
-  } else {
-    // the incoming message, applied on the ket side: out[a] = sum_b M[a][b] in[b]
-    const int code = node_src ? node_src[base + axis] : 0;
-    const cx<R>* m = code == 0 ? e2n_old + (2LL * e + sd) * CC
-                               : (code == 1 ? n2e_new : n2e_old) + (2LL * e + 1 - sd) * CC;
-    for (int q = threadIdx.x; q < CC; q += blockDim.x) Ms[q] = m[q];
-  }
+  // the incoming message, applied on the ket side: out[a] = sum_b M[a][b] in[b]
+  const int code = node_src ? node_src[base + axis] : 0;
+  const cx<R>* m = code == 0 ? e2n_old + (2LL * e + sd) * CC
+                             : (code == 1 ? n2e_new : n2e_old) + (2LL * e + 1 - sd) * CC;
+  for (int q = threadIdx.x; q < CC; q += blockDim.x) Ms[q] = m[q];
   __syncthreads();
-  const cx<R>* src = t == 0 ? T + g.toff[node] : buf + (2LL * k + ((t - 1) & 1)) * smax;
-  cx<R>* dst = buf + (2LL * k + (t & 1)) * smax;
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < cols;
        c += (long long)gridDim.x * blockDim.x) {
     const long long o = c / inner, i = c - o * inner;
@@ -75,11 +119,12 @@ __global__ void __launch_bounds__(kBigThreads)
 }
 
 // Partial Gram of output slot idx over this block's columns:
-// part[(k * chunks + chunk)][kk * CHI + l] = sum conj(T[..kk..]) U[..l..].
+// part[(k * chunks + chunk)][kk * CHI + l] = sum conj(T[..kk..]) U[..l..],
+// U in buffer src_id (T itself when src_id < 0).
 template <typename R, int CHI>
 __global__ void __launch_bounds__(kBigThreads)
-    k_big_gram(DevGraph g, const int* nodes, int idx, const cx<R>* T, const cx<R>* buf, long long smax, cd* part,
-               const int* inst_active) {
+    k_big_gram(DevGraph g, const int* nodes, int idx, int src_id, int nb, const cx<R>* T, const cx<R>* buf,
+               long long smax, cd* part, const int* inst_active) {
   constexpr int CC = CHI * CHI, PP = CC / kBigThreads;
   static_assert(PP >= 1 && PP * kBigThreads == CC, "one or more (k, l) pairs per thread");
   __shared__ cx<R> Tt[kBigTile][CHI + 1], Ut[kBigTile][CHI + 1];
@@ -89,7 +134,7 @@ __global__ void __launch_bounds__(kBigThreads)
   const int d = g.deg[node];
   if (idx >= d) return;
   const cx<R>* t = T + g.toff[node];
-  const cx<R>* u = d >= 2 ? buf + (2LL * k + ((d - 2) & 1)) * smax : t;
+  const cx<R>* u = src_id < 0 ? t : buf + ((long long)nb * k + src_id) * smax;
   const long long S = ipow(CHI, d) * g.phys;
   const long long inner = ipow(CHI, d - 1 - idx) * g.phys;
   const long long cols = S / CHI;
@@ -203,6 +248,13 @@ __global__ void __launch_bounds__(kBigThreads)
 }
 
 // ------------------------------------------------------------- launchers
+// Blocks per node of the absorb's norm and scaling passes: at least 8192
+// elements per block (one block per 256 elements left 256 blocks and as many
+// partial sums per degree-3 node at chi 32).
+static int big_norm_blocks(long long smax) {
+  long long b = (smax + 8191) / 8192;
+  return (int)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+}
 static int big_col_blocks(long long cols) {
   long long b = (cols + kBigThreads - 1) / kBigThreads;
   return (int)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
@@ -220,12 +272,76 @@ bool big_applicable(const DevGraph& g, int max_deg) {
   return (g.chi == 16 || g.chi == 32) && ipow(g.chi, max_deg) * g.phys >= (1LL << 15);
 }
 
+// The node update of a degree-d node as a plan of products and Grams that
+// shares partial products between output slots: the slots are split in two
+// halves, the other half's messages are applied once for each, recursively.
+// Output slot idx still receives T with every other message applied, so the
+// result is that of pass_through_node (tensor_ops.py:36-60) with the products
+// in another order.  d = 6: 16 products (instead of 6 x 5), three buffers;
+// d = 3: 5 products (instead of 6), two buffers.
+namespace {
+struct BigOp {
+  bool gram;  // false: dst = src x_axis M; true: Gram of slot `axis` from src
+  int src, dst, axis;
+};
+struct BigPlanner {
+  std::vector<BigOp> ops;
+  std::vector<int> free_list;
+  int nbuf = 0;
+  int alloc() {
+    if (!free_list.empty()) {
+      const int b = free_list.back();
+      free_list.pop_back();
+      return b;
+    }
+    return nbuf++;
+  }
+  void release(int b) {
+    if (b >= 0) free_list.push_back(b);
+  }
+  int chain(int src, const std::vector<int>& axes) {
+    int cur = src;
+    for (int a : axes) {
+      const int dst = alloc();
+      ops.push_back({false, cur, dst, a});
+      if (cur != src) release(cur);
+      cur = dst;
+    }
+    return cur;
+  }
+  void solve(int src, const std::vector<int>& S) {
+    if (S.size() == 1) {
+      ops.push_back({true, src, -1, S[0]});
+      return;
+    }
+    const size_t h = S.size() / 2;
+    const std::vector<int> S1(S.begin(), S.begin() + h), S2(S.begin() + h, S.end());
+    const int x1 = chain(src, S2);
+    solve(x1, S1);
+    if (x1 != src) release(x1);
+    const int x2 = chain(src, S1);
+    solve(x2, S2);
+    if (x2 != src) release(x2);
+  }
+};
+BigPlanner big_plan(int d) {
+  BigPlanner p;
+  std::vector<int> S;
+  for (int i = 0; i < d; ++i) S.push_back(i);
+  if (d >= 1) p.solve(-1, S);
+  return p;
+}
+}  // namespace
+
 long long big_ws_bytes(const DevGraph& g, int max_deg, int count, int rb) {
   const long long smax = ipow(g.chi, max_deg) * g.phys;
   const long long cols = smax / g.chi;
   const long long part = (long long)count * big_gram_chunks(cols) * g.chi * g.chi * (long long)sizeof(cd);
-  const long long norm = (long long)count * big_col_blocks(smax) * (long long)sizeof(double);
-  return 2LL * count * smax * 2 * rb + (part > norm ? part : norm) + 256;
+  const long long norm = (long long)count * big_norm_blocks(smax) * (long long)sizeof(double);
+  // node-update plan buffers (the absorb uses two)
+  int nb = big_plan(max_deg).nbuf;
+  if (nb < 2) nb = 2;
+  return (long long)nb * count * smax * 2 * rb + (part > norm ? part : norm) + 256;
 }
 
 template <typename R, int CHI>
@@ -234,18 +350,23 @@ static hipError_t big_bp_level(const DevGraph& g, int max_deg, const int* nodes,
                                void* ws, const int* inst_active, hipStream_t s) {
   const long long smax = ipow(CHI, max_deg) * g.phys;
   const long long cols = smax / CHI;
+  const BigPlanner plan = big_plan(max_deg);
+  const int nb = plan.nbuf < 2 ? 2 : plan.nbuf;
   cx<R>* buf = (cx<R>*)ws;
-  cd* part = (cd*)((char*)ws + 2LL * count * smax * sizeof(cx<R>));
+  cd* part = (cd*)((char*)ws + (long long)nb * count * smax * sizeof(cx<R>));
   const int chunks = big_gram_chunks(cols);
-  for (int idx = 0; idx < max_deg; ++idx) {
-    for (int t = 0; t + 1 < max_deg; ++t)
-      hipLaunchKernelGGL((k_big_mode_product<R, CHI, false>), dim3(big_col_blocks(cols), count), dim3(kBigThreads), 0,
-                         s, g, nodes, t, idx, (const cx<R>*)T, (const cx<R>*)n2e_old, (const cx<R>*)e2n_old,
-                         (const cx<R>*)n2e_new, node_src, (const cd*)nullptr, buf, smax, inst_active);
-    hipLaunchKernelGGL((k_big_gram<R, CHI>), dim3(chunks, count), dim3(kBigThreads), 0, s, g, nodes, idx,
-                       (const cx<R>*)T, (const cx<R>*)buf, smax, part, inst_active);
-    hipLaunchKernelGGL((k_big_gram_reduce<R, CHI>), dim3(count), dim3(kBigThreads), 0, s, g, nodes, idx,
-                       (const cd*)part, chunks, (cx<R>*)n2e_new, inst_active);
+  const int pblocks = big_col_blocks(cols);
+  for (const BigOp& op : plan.ops) {
+    if (!op.gram) {
+      hipLaunchKernelGGL((k_big_plan_product<R, CHI>), dim3(pblocks, count), dim3(kBigThreads), 0, s, g, nodes,
+                         op.axis, op.src, op.dst, nb, (const cx<R>*)T, (const cx<R>*)n2e_old, (const cx<R>*)e2n_old,
+                         (const cx<R>*)n2e_new, node_src, buf, smax, inst_active);
+    } else {
+      hipLaunchKernelGGL((k_big_gram<R, CHI>), dim3(chunks, count), dim3(kBigThreads), 0, s, g, nodes, op.axis,
+                         op.src, nb, (const cx<R>*)T, (const cx<R>*)buf, smax, part, inst_active);
+      hipLaunchKernelGGL((k_big_gram_reduce<R, CHI>), dim3(count), dim3(kBigThreads), 0, s, g, nodes, op.axis,
+                         (const cd*)part, chunks, (cx<R>*)n2e_new, inst_active);
+    }
   }
   return hipGetLastError();
 }
@@ -273,10 +394,9 @@ static hipError_t big_absorb(const DevGraph& g, int max_deg, const int* nodes, i
   cx<R>* buf = (cx<R>*)ws;
   double* part = (double*)((char*)ws + 2LL * count * smax * sizeof(cx<R>));
   for (int t = 0; t < max_deg; ++t)
-    hipLaunchKernelGGL((k_big_mode_product<R, CHI, true>), dim3(big_col_blocks(cols), count), dim3(kBigThreads), 0,
-                       s, g, nodes, t, 0, (const cx<R>*)T, (const cx<R>*)nullptr, (const cx<R>*)nullptr,
-                       (const cx<R>*)nullptr, (const uint8_t*)nullptr, X, buf, smax, inst_active);
-  const int nb = big_col_blocks(smax);
+    hipLaunchKernelGGL((k_big_absorb_product<R, CHI>), dim3(big_col_blocks(cols), count), dim3(kBigThreads), 0, s, g,
+                       nodes, t, (const cx<R>*)T, X, buf, smax, 2, inst_active);
+  const int nb = big_norm_blocks(smax);
   hipLaunchKernelGGL((k_big_norm<R>), dim3(nb, count), dim3(kBigThreads), 0, s, g, nodes, (const cx<R>*)buf, smax,
                      CHI, part);
   hipLaunchKernelGGL((k_big_scale_out<R>), dim3(nb, count), dim3(kBigThreads), 0, s, g, nodes, (const cx<R>*)buf,

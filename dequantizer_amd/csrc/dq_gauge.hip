@@ -292,9 +292,9 @@ __global__ void __launch_bounds__(128)
 }
 
 // The same decomposition for chi = 32, one edge per 256-thread block, four
-// 32 x 32 complex128 matrices in LDS (64 KiB: two blocks per CU).  Waves 0
-// and 1 diagonalise the two messages at once (wave-synchronous two-sided
-// Jacobi); the square roots are applied in factored form,
+// 32 x 32 complex128 matrices in LDS (64 KiB: two blocks per CU).  The two
+// messages are diagonalised by the block-level one-sided Jacobi (below); the
+// square roots are applied in factored form,
 //   G = Sf Sb^T = Qf (Df^1/2 Qf^H conj(Qb) Db^1/2) Qb^T,
 // as three small products; the block-level one-sided Jacobi takes the SVD of
 // G (its left vectors U go to the edge's slot of the workspace); then
@@ -324,16 +324,70 @@ __global__ void __launch_bounds__(256)
     Mb[q] = mk<double>(0.5 * (b.re + bt.re), 0.5 * (b.im - bt.im));
   }
   __syncthreads();
-  if (tid < 64)
-    wave_jacobi_eigh_n<n>(Mf, Qf);
-  else if (tid < 128)
-    wave_jacobi_eigh_n<n>(Mb, Qb);
-  __syncthreads();
-  if (tid < n) {
-    wf[tid] = Mf[tid * (n + 1)].re;
-    wb[tid] = Mb[tid * (n + 1)].re;
+  // Eigendecompositions by the block-level one-sided Jacobi, one message
+  // after the other.  Read column-major, the row-major Hermitian M is
+  // conj(M); with conj(M) J = G, the columns of conj(J) are eigenvectors of M
+  // and the Rayleigh quotient j_p^H g_p its eigenvalue, sign included.  When
+  // a column's quotient falls short of its norm |g_p| (eigenvalues +w and -w
+  // sharing a singular subspace) the two-sided wave-level Jacobi redoes both.
+  __shared__ int bad;
+  if (tid == 0) bad = 0;
+  for (int side = 0; side < 2; ++side) {
+    cd* M = side == 0 ? Mf : Mb;
+    cd* Q = side == 0 ? Qf : Qb;
+    double* w = side == 0 ? wf : wb;
+    block_jacobi_svd_n<32, lds_cd, true, false>((lds_cd*)M, nullptr, (lds_cd*)Q, sigma, perm, scal);
+    if (tid < n) {
+      double re = 0.0, s2 = 0.0;
+#pragma unroll 8
+      for (int k = 0; k < n; ++k) {
+        const cd v = Q[tid * n + k], gk = M[tid * n + k];
+        re += v.re * gk.re + v.im * gk.im;
+        s2 += norm2(gk);
+      }
+      w[tid] = re;
+      const double nrm = sqrt(s2);
+      if (nrm > eps && fabs(re) < (1.0 - 1e-6) * nrm) bad = 1;
+    }
+    __syncthreads();
   }
-  __syncthreads();
+  if (bad) {
+    for (int q = tid; q < cc; q += blockDim.x) {
+      const int k = q / n, l = q % n;
+      const cd a = cvt<double>(m0[q]), at = cvt<double>(m0[l * n + k]);
+      const cd b = cvt<double>(m1[q]), bt = cvt<double>(m1[l * n + k]);
+      Mf[q] = mk<double>(0.5 * (a.re + at.re), 0.5 * (a.im - at.im));
+      Mb[q] = mk<double>(0.5 * (b.re + bt.re), 0.5 * (b.im - bt.im));
+    }
+    __syncthreads();
+    if (tid < 64)
+      wave_jacobi_eigh_n<n>(Mf, Qf);
+    else if (tid < 128)
+      wave_jacobi_eigh_n<n>(Mb, Qb);
+    __syncthreads();
+    if (tid < n) {
+      wf[tid] = Mf[tid * (n + 1)].re;
+      wb[tid] = Mb[tid * (n + 1)].re;
+    }
+    __syncthreads();
+  } else {
+    // Q[k][p] = conj(J[k][p]) = conj(V[p * n + k]): transpose and conjugate
+    cd tq[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int q = tid + 256 * t, side = q >> 10, r = q & 1023;
+      const cd* Q = side == 0 ? Qf : Qb;
+      tq[t] = conj(Q[(r % n) * n + r / n]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int q = tid + 256 * t, side = q >> 10, r = q & 1023;
+      cd* Q = side == 0 ? Qf : Qb;
+      Q[r] = tq[t];
+    }
+    __syncthreads();
+  }
   if (tid < 4 * n) {
     const int which = tid / n, i = tid % n;
     const double w = (which & 1) ? wb[i] : wf[i];

@@ -215,8 +215,9 @@ __device__ inline int find_slot(const DevGraph& g, int node, int e, int side) {
 // For chi <= 32 each thread takes whole columns (all indices but axis j): the
 // chi values of a column are read once into registers and the chi outputs
 // written from them, so the tensor is read and written once per product
-// (the element loop below reads every input chi times).  Both forms sum over
-// b in the same order: the results are bitwise equal.
+// (the element loop below reads every input chi times); M is staged in LDS
+// (as R) first.  Both forms sum over b in the same order: the results are
+// bitwise equal.
 template <typename R, typename MT>
 __device__ void mode_product(cx<R>* dst, const cx<R>* src, const cx<MT>* M, int chi, int d, int phys, int j,
                              bool transposed) {
@@ -224,6 +225,13 @@ __device__ void mode_product(cx<R>* dst, const cx<R>* src, const cx<MT>* M, int 
   const long long total = ipow(chi, d) * phys;
   constexpr int kReg = 32;
   if (chi <= kReg) {
+    __shared__ cx<R> Ms[kReg * kReg];
+    __syncthreads();  // earlier readers of Ms are done
+    for (int q = threadIdx.x; q < chi * chi; q += blockDim.x) {
+      const int a = q / chi, b = q - a * chi;
+      Ms[q] = cvt<R>(transposed ? M[(long long)b * chi + a] : M[q]);
+    }
+    __syncthreads();
     const long long cols = total / chi;
     for (long long c = threadIdx.x; c < cols; c += blockDim.x) {
       const long long o = c / inner, i = c - o * inner;
@@ -236,10 +244,7 @@ __device__ void mode_product(cx<R>* dst, const cx<R>* src, const cx<MT>* M, int 
         cx<R> acc = mk<R>(0, 0);
 #pragma unroll
         for (int b = 0; b < kReg; ++b)
-          if (b < chi) {
-            const cx<MT> m = transposed ? M[(long long)b * chi + a] : M[(long long)a * chi + b];
-            fma_acc(acc, cvt<R>(m), in[b]);
-          }
+          if (b < chi) fma_acc(acc, Ms[a * chi + b], in[b]);
         dst[b0 + a * inner] = acc;
       }
     }

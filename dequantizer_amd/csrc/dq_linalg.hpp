@@ -638,8 +638,11 @@ __device__ inline double wave_nuclear_norm_n(cd* G) {
 // 16 pairs of a round run on 16-lane groups (4 per wave), 2 rows per lane, one
 // block barrier per round.  Same outputs as jacobi_svd.  scal: N doubles.
 // kVectors = false: singular values only (sigma, perm); V is neither formed
-// nor read, U is not written (both may be null), G is overwritten.
-template <int N, typename P, bool kVectors = true>
+// nor read, U is not written (both may be null).  kU = false: V is formed
+// but U is not written (may be null).  G is left holding the rotated
+// columns G J in their unsorted order (column p of G J goes with column p of
+// J = V; sigma and perm give the sorted order).
+template <int N, typename P, bool kVectors = true, bool kU = true>
 __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int* perm, double* scal) {
   constexpr int NP = N / 2;
   constexpr int LPP = 256 / NP;  // 16 lanes per pair for N = 32
@@ -716,7 +719,7 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
     sigma[rank] = me;
   }
   __syncthreads();
-  if (!kVectors) return;
+  if (!kVectors || !kU) return;
   for (int t = threadIdx.x; t < N * N; t += blockDim.x) {
     const int j = t / N, i = t % N;
     const double sg = sigma[j];

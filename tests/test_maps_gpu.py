@@ -227,22 +227,27 @@ def test_vidal_distance_matrix_core_path_chi16(zero_tail):
     assert float(dq.get_vidal_gauge_distance_map(trav_p, 1e-10)(gam, lam)) < 1e-4
 
 
+@pytest.mark.parametrize("chi", [16, 32])
 @pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-10), (np.complex64, 2e-4)])
-def test_vidal_gauge_chi16_indefinite_messages(dtype, tol):
+def test_vidal_gauge_indefinite_messages(chi, dtype, tol):
     """chi=16 edges run the wave-level decomposition (two-sided Jacobi on each
-    message, one-sided Jacobi on the core).  Hermitian messages with negative
-    and clustered eigenvalues: lambdas and gauge-invariant products against
-    the oracle's eigh/SVD path."""
-    gp, go = regular_pair(10, 16, 4)
+    message, one-sided Jacobi on the core); chi=32 edges the block-level one
+    (one-sided Jacobi on each message with the eigenvalue signs from Rayleigh
+    quotients, the two-sided Jacobi when a pair +w, -w shares a singular
+    subspace).  Hermitian messages with negative, clustered and opposite
+    eigenvalues: lambdas and gauge-invariant products against the oracle's
+    eigh/SVD path."""
+    gp, go = regular_pair(10, chi, 4)
     rng = np.random.default_rng(17)
     tensors = random_tensors(gp, rng)
     msgs = {}
     for el in gp.get_traversal_iterator():
         for nb in el.neighbors:
-            q, _ = np.linalg.qr(rng.standard_normal((16, 16)) + 1j * rng.standard_normal((16, 16)))
-            w = rng.uniform(-0.2, 1.0, 16)
+            q, _ = np.linalg.qr(rng.standard_normal((chi, chi)) + 1j * rng.standard_normal((chi, chi)))
+            w = rng.uniform(-0.2, 1.0, chi)
             w[:3] = 0.5  # a threefold eigenvalue
             w[3] = 1e-13  # below eps
+            w[4], w[5] = 0.3, -0.3  # opposite eigenvalues
             m = (q * w) @ q.conj().T
             msgs[dq.MessageID(el.id, nb.id)] = m
     trav_p = list(gp.get_traversal_iterator())
