@@ -120,13 +120,24 @@ __global__ void __launch_bounds__(kBigThreads)
 
 // Partial Gram of output slot idx over this block's columns:
 // part[(k * chunks + chunk)][kk * CHI + l] = sum conj(T[..kk..]) U[..l..],
-// U in buffer src_id (T itself when src_id < 0).
+// U in buffer src_id (T itself when src_id < 0).  Columns are staged through
+// LDS in tiles of 64 (loads along the columns, coalesced); the products run
+// on FP64 matrix cores (v_mfma_f64_16x16x4f64: exact products and FP64
+// sums of the values converted from R, as the FMA form), four real MFMAs per
+// complex k-step: Re += Tr Ur + Ti Ui, Im += Tr Ui - Ti Ur.  Operand layout:
+// lane l holds A[l % 16][l / 16] = conj T[column l / 16][kk = m0 + l % 16]
+// and B[l / 16][l % 16] = U[column l / 16][n0 + l % 16]; the result
+// D[l / 16 + 4 v][l % 16] stays in four accumulators.  chi 32: wave w owns the
+// 16 x 16 tile (w / 2, w % 2) over all columns; chi 16: the one tile, the
+// four waves taking a quarter of each column tile each, summed at the end.
+typedef double big_d4 __attribute__((ext_vector_type(4)));
+
 template <typename R, int CHI>
 __global__ void __launch_bounds__(kBigThreads)
     k_big_gram(DevGraph g, const int* nodes, int idx, int src_id, int nb, const cx<R>* T, const cx<R>* buf,
                long long smax, cd* part, const int* inst_active) {
-  constexpr int CC = CHI * CHI, PP = CC / kBigThreads;
-  static_assert(PP >= 1 && PP * kBigThreads == CC, "one or more (k, l) pairs per thread");
+  static_assert(CHI == 16 || CHI == 32, "matrix-core Gram for chi 16 and 32");
+  constexpr int CC = CHI * CHI;
   __shared__ cx<R> Tt[kBigTile][CHI + 1], Ut[kBigTile][CHI + 1];
   const int k = blockIdx.y;
   const int node = nodes[k];
@@ -138,9 +149,13 @@ __global__ void __launch_bounds__(kBigThreads)
   const long long S = ipow(CHI, d) * g.phys;
   const long long inner = ipow(CHI, d - 1 - idx) * g.phys;
   const long long cols = S / CHI;
-  double ar[PP], ai[PP];
-#pragma unroll
-  for (int v = 0; v < PP; ++v) ar[v] = ai[v] = 0.0;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int m0 = CHI == 32 ? 16 * (w >> 1) : 0, n0 = CHI == 32 ? 16 * (w & 1) : 0;
+  // k-steps of a column tile this wave takes: all 16 (chi 32) or 4 (chi 16)
+  constexpr int KS = CHI == 32 ? kBigTile / 4 : kBigTile / 16;
+  const int ks0 = CHI == 32 ? 0 : w * KS;
+  const int mi = lane & 15, kq = lane >> 4;
+  big_d4 gr = {0.0, 0.0, 0.0, 0.0}, gi = gr;
   for (long long c0 = (long long)blockIdx.x * kBigTile; c0 < cols; c0 += (long long)gridDim.x * kBigTile) {
     for (int q = threadIdx.x; q < kBigTile * CHI; q += blockDim.x) {
       const int r = q / kBigTile, cc = q % kBigTile;
@@ -157,24 +172,30 @@ __global__ void __launch_bounds__(kBigThreads)
     }
     __syncthreads();
 #pragma unroll
-    for (int v = 0; v < PP; ++v) {
-      const int pr = threadIdx.x + v * kBigThreads;
-      const int kk = pr / CHI, l = pr % CHI;
-      double sr = 0.0, si = 0.0;
-#pragma unroll 8
-      for (int cc = 0; cc < kBigTile; ++cc) {
-        const cx<R> a = Tt[cc][kk], b = Ut[cc][l];
-        sr += (double)a.re * b.re + (double)a.im * b.im;
-        si += (double)a.re * b.im - (double)a.im * b.re;
-      }
-      ar[v] += sr;
-      ai[v] += si;
+    for (int ks = 0; ks < KS; ++ks) {
+      const int c = 4 * (ks0 + ks) + kq;
+      const cx<R> a = Tt[c][m0 + mi], b = Ut[c][n0 + mi];
+      const double ar = (double)a.re, ai = (double)a.im, br = (double)b.re, bi = (double)b.im;
+      gr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, gr, 0, 0, 0);
+      gr = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, gr, 0, 0, 0);
+      gi = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, gi, 0, 0, 0);
+      gi = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, br, gi, 0, 0, 0);
     }
     __syncthreads();
   }
   cd* pp = part + ((long long)k * gridDim.x + blockIdx.x) * CC;
+  if (CHI == 32) {
 #pragma unroll
-  for (int v = 0; v < PP; ++v) pp[threadIdx.x + v * kBigThreads] = mk<double>(ar[v], ai[v]);
+    for (int v = 0; v < 4; ++v) pp[(m0 + kq + 4 * v) * CHI + n0 + mi] = mk<double>(gr[v], gi[v]);
+  } else {
+    // the four waves' partial tiles, summed through LDS
+    __shared__ cd red[4 * 256];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red[w * 256 + (kq + 4 * v) * 16 + mi] = mk<double>(gr[v], gi[v]);
+    __syncthreads();
+    for (int q = threadIdx.x; q < 256; q += blockDim.x)
+      pp[q] = red[q] + red[256 + q] + red[512 + q] + red[768 + q];
+  }
 }
 
 template <typename R, int CHI>
