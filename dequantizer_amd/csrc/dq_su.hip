@@ -538,6 +538,20 @@ __device__ __attribute__((noinline)) void su_core_svd_qr32(lds_cd* C, lds_cd* U,
   __syncthreads();
 }
 
+// SVD of the 64 x 64 core (column-major, in the workspace) with the core and
+// the right vectors staged in LDS: the block-level one-sided Jacobi (eight
+// lanes per column pair), outputs as jacobi_svd (U column-major and sorted,
+// V with perm).  scal: 64 doubles.
+__device__ __attribute__((noinline)) void su_core_svd64(const cd* C, cd* U, cd* Vs, double* sigma, int* perm,
+                                                          double* scal) {
+  __shared__ cd G64[64 * 64], V64[64 * 64];
+  for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) cst((lds_cd*)G64, i, C[i]);
+  __syncthreads();
+  block_jacobi_svd_n<64>((lds_cd*)G64, U, (lds_cd*)V64, sigma, perm, scal);
+  for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) Vs[i] = cld((lds_cd*)V64, i);
+  __syncthreads();
+}
+
 // Matrices of one gate (complex doubles): E (cp^2 or n^2; later U, then Z),
 // G (n^2; Gram, core, then W_a), V (n^2; right singular vectors, then W_b),
 // Ra, Rb (n^2 each), chunk (rows x cp, generic environment path only).
@@ -546,8 +560,11 @@ __device__ __attribute__((noinline)) void su_core_svd_qr32(lds_cd* C, lds_cd* U,
 //   kSuLds      all in LDS (n, cp <= 32; ~112 KiB of LDS, one block per CU);
 //   kSuCompact  E, G, V in LDS (48 KiB), Ra, Rb, chunk in the workspace
 //               (L2-resident): two blocks per CU.  For cp = 32, chi % 16 == 0
-//               (the matrix-core shapes), n <= 32.
-constexpr int kSuGlobal = 0, kSuLds = 1, kSuCompact = 2;
+//               (the matrix-core shapes), n <= 32;
+//   kSuWide     storage as kSuGlobal, the n = 64 core SVD (chi = 32 with rank-2
+//               gates, chi = 16 with rank-4 gates) by the block-level Jacobi
+//               on the core and V in LDS (128 KiB, one block per CU).
+constexpr int kSuGlobal = 0, kSuLds = 1, kSuCompact = 2, kSuWide = 3;
 constexpr int kSuChunk = 32;
 constexpr int kSuLdsN = 32;
 
@@ -556,6 +573,7 @@ __host__ __device__ inline long long su_e_cd(long long n, long long cp) { return
 int su_mem_mode(int chi, int phys, int max_r) {
   if (chi * phys == 32 && chi % 16 == 0 && chi * max_r <= kSuLdsN) return kSuCompact;
   if (chi * max_r <= kSuLdsN && chi * phys <= kSuLdsN) return kSuLds;
+  if (chi * max_r == 64) return kSuWide;
   return kSuGlobal;
 }
 
@@ -573,6 +591,7 @@ __global__ void __launch_bounds__(256, kMem == kSuCompact ? 2 : 1)
          int max_rank, double* err_out, double* ent_out, char* ws, long long wsb, char* tape, long long tape_bytes) {
   __shared__ cd sm[kMem == kSuLds ? (5 * kSuLdsN * kSuLdsN + kSuChunk * kSuLdsN + 64)
                                   : (kMem == kSuCompact ? 3 * kSuLdsN * kSuLdsN : 1)];
+  constexpr bool kGlob = kMem == kSuGlobal || kMem == kSuWide;  // matrices in the workspace
   __shared__ double red[8];
   __shared__ double rot[4 * 64 + 8];
   __shared__ double sigma[128];
@@ -604,7 +623,7 @@ __global__ void __launch_bounds__(256, kMem == kSuCompact ? 2 : 1)
   const cd* h1 = halves + (long long)blockIdx.x * 2 * hstride;
   const cd* h2 = h1 + hstride;
   cd* wsp = (cd*)(ws + (long long)blockIdx.x * wsb);
-  cd* E = kMem == kSuGlobal ? wsp : sm;
+  cd* E = kGlob ? wsp : sm;
   cd* Gm = E + su_e_cd(n, cp);
   cd* V = Gm + n * n;
   cd* Ra = kMem == kSuCompact ? wsp : V + n * n;
@@ -635,7 +654,7 @@ __global__ void __launch_bounds__(256, kMem == kSuCompact ? 2 : 1)
   const double tra = su_gram(E, h1, chi, p, r, Gm, red);
   SuTape tp(tape ? tape + (long long)blockIdx.x * tape_bytes : nullptr, chi * max_r);
   double trb;
-  if (kMem != kSuGlobal && n <= 32 && blockDim.x >= 128) {
+  if (!kGlob && n <= 32 && blockDim.x >= 128) {
     // both Grams first (G_b in the V buffer, free until the SVD), then the
     // two factorisations at once on waves 0 and 1, without block barriers
     if (mfma_b)
@@ -704,7 +723,7 @@ __global__ void __launch_bounds__(256, kMem == kSuCompact ? 2 : 1)
     }
     // column-major for the Jacobi SVD; row-major for the pivoted QR that
     // preconditions it (its lanes walk the columns)
-    const bool row_major = !tape && kMem != kSuGlobal;
+    const bool row_major = !tape && !kGlob;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int i = i0 + kq + 4 * v, jc = j0 + rr;
@@ -719,11 +738,11 @@ __global__ void __launch_bounds__(256, kMem == kSuCompact ? 2 : 1)
     }
   }
   __syncthreads();
-  if (n == 32 && blockDim.x == 256 && !tape && kMem != kSuGlobal) {
+  if (n == 32 && blockDim.x == 256 && !tape && !kGlob) {
     su_core_svd_qr32((lds_cd*)C, (lds_cd*)U, (lds_cd*)Vs, chunk, sigma, perm, rot);
   } else if (n == 32 && blockDim.x == 256) {
     // the adjoint's record keeps the unpreconditioned factors it was built on
-    if (kMem != kSuGlobal)
+    if (!kGlob)
       block_jacobi_svd_n<32>((lds_cd*)C, U, (lds_cd*)Vs, sigma, perm, rot);
     else
       block_jacobi_svd_n<32>(C, U, Vs, sigma, perm, rot);
@@ -733,6 +752,8 @@ __global__ void __launch_bounds__(256, kMem == kSuCompact ? 2 : 1)
   } else if (n < 32) {
     if (threadIdx.x < 64) wave_jacobi_svd(C, n, n, U, Vs, sigma, perm, rot);
     __syncthreads();
+  } else if (kMem == kSuWide && n == 64 && blockDim.x == 256) {
+    if constexpr (kMem == kSuWide) su_core_svd64(C, U, Vs, sigma, perm, rot);
   } else {
     jacobi_svd(C, n, n, n, U, n, Vs, sigma, rot, flag, perm);
   }
@@ -871,6 +892,8 @@ hipError_t launch_su(int rb, const DevGraph& g, void* T, double* lam, const SuGa
       DQ_SU_LAUNCH(float, kSuCompact);
     else if (mode == kSuLds)
       DQ_SU_LAUNCH(float, kSuLds);
+    else if (mode == kSuWide)
+      DQ_SU_LAUNCH(float, kSuWide);
     else
       DQ_SU_LAUNCH(float, kSuGlobal);
   } else {
@@ -878,6 +901,8 @@ hipError_t launch_su(int rb, const DevGraph& g, void* T, double* lam, const SuGa
       DQ_SU_LAUNCH(double, kSuCompact);
     else if (mode == kSuLds)
       DQ_SU_LAUNCH(double, kSuLds);
+    else if (mode == kSuWide)
+      DQ_SU_LAUNCH(double, kSuWide);
     else
       DQ_SU_LAUNCH(double, kSuGlobal);
   }
