@@ -131,6 +131,7 @@ __global__ void __launch_bounds__(kBigThreads)
 // 16 x 16 tile (w / 2, w % 2) over all columns; chi 16: the one tile, the
 // four waves taking a quarter of each column tile each, summed at the end.
 typedef double big_d4 __attribute__((ext_vector_type(4)));
+typedef float big_f4 __attribute__((ext_vector_type(4)));
 
 template <typename R, int CHI>
 __global__ void __launch_bounds__(kBigThreads)
@@ -196,6 +197,152 @@ __global__ void __launch_bounds__(kBigThreads)
     for (int q = threadIdx.x; q < 256; q += blockDim.x)
       pp[q] = red[q] + red[256 + q] + red[512 + q] + red[768 + q];
   }
+}
+
+// The last product of an output slot fused with its Gram, chi = 16, for
+// levels whose nodes all have the plan's degree:
+//   G[k][l] += sum_{x_j} conj(T[.., x_j, .., k@idx, ..]) (M_j P)[.., x_j, .., l@idx, ..]
+// summed over every other index c'.  A tile holds the 16 x 16 (x_j, x_idx)
+// slices of P and T for W = 8 consecutive c' (c' enumerates the remaining
+// indices, the contiguous ones fastest, so the loads run along memory) in
+// LDS; wave w takes slices 2w and 2w + 1.  Both products run on matrix
+// cores: U = M_j P_w in FP32 (complex64 data, as the separate product), whose
+// result registers (row 4 (l / 16) + v, column l % 16) are exactly the B
+// operands of the Gram's four k-steps over a (rows v, v + 4, v + 8, v + 12),
+// the Gram in FP64 with A = conj(T_w)^T.  U never goes to memory: one pass
+// reads P and T where the separate product and Gram read P, write U, then
+// read U and T.  complex64 only (the complex128 levels keep the separate
+// launches, which run faster there).
+template <typename R>
+__global__ void __launch_bounds__(kBigThreads)
+    k_big_prod_gram16(DevGraph g, const int* nodes, int j, int idx, int src_id, int nb, const cx<R>* T,
+                      const cx<R>* n2e_old, const cx<R>* e2n_old, const cx<R>* n2e_new, const uint8_t* node_src,
+                      const cx<R>* buf, long long smax, cd* part, const int* inst_active) {
+  constexpr int CHI = 16, CC = CHI * CHI, W = 8;
+  __shared__ cx<R> Pt[W][CHI][CHI + 1], Tt[W][CHI][CHI + 1];
+  __shared__ cd red[4 * CC];
+  const int k = blockIdx.y;
+  const int node = nodes[k];
+  if (inst_active && !inst_active[g.node_inst[node]]) return;
+  const int d = g.deg[node];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, mi = lane & 15, kq = lane >> 4;
+  const long long S = ipow(CHI, d) * g.phys;
+  const int a_hi = j < idx ? j : idx, a_lo = j < idx ? idx : j;
+  const long long n_bot = ipow(CHI, d - 1 - a_lo) * g.phys;  // contiguous indices after both axes
+  const long long n_mid = ipow(CHI, a_lo - a_hi - 1);
+  const long long st_lo = n_bot, st_mid = CHI * n_bot, st_hi = n_mid * CHI * n_bot, st_top = CHI * st_hi;
+  const long long ncp = S / CC, ntiles = ncp / W;
+  const cx<R>* t = T + g.toff[node];
+  const cx<R>* P = src_id < 0 ? t : buf + ((long long)nb * k + src_id) * smax;
+  // M_j[a][b] as the A operand of the product's four k-steps: lane l holds
+  // M[l % 16][4 ks + l / 16]
+  cd mv[4];
+  {
+    const int base = g.adj_ptr[node];
+    const int e = g.adj_edge[base + j], sd = g.adj_side[base + j];
+    const int code = node_src ? node_src[base + j] : 0;
+    const cx<R>* m = code == 0 ? e2n_old + (2LL * e + sd) * CC
+                               : (code == 1 ? n2e_new : n2e_old) + (2LL * e + 1 - sd) * CC;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) mv[ks] = cvt<double>(m[mi * CHI + 4 * ks + kq]);
+  }
+  // the next tile's elements are loaded into registers while the current
+  // tile is multiplied (8 of P and 8 of T per thread)
+  // Index arithmetic in 32 bits (the host fuses only tensors below 2^31
+  // elements): the (x_hi, x_lo) part of each of the thread's PER elements is
+  // fixed; when the contiguous run n_bot holds whole tiles the c' part is one
+  // offset per tile.
+  constexpr int PER = W * CC / kBigThreads;
+  cx<R> np[PER], nt[PER];
+  unsigned offc[PER];
+  int wwv[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int q = tid + u * kBigThreads;
+    wwv[u] = q % W;
+    const int rest = q / W, xl = rest % CHI, xh = rest / CHI;
+    offc[u] = (unsigned)(xh * st_hi + xl * st_lo);
+  }
+  const unsigned nbot = (unsigned)n_bot, nmid = (unsigned)n_mid, smid = (unsigned)st_mid, stop = (unsigned)st_top;
+  const bool whole = nbot % W == 0;
+  auto load_tile = [&](long long tile) {
+    const unsigned cp0 = (unsigned)(tile * W);
+    if (whole) {
+      const unsigned t2 = cp0 / nbot, base = (t2 / nmid) * stop + (t2 % nmid) * smid + cp0 % nbot;
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const unsigned off = base + offc[u] + wwv[u];
+        np[u] = P[off];
+        nt[u] = t[off];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const unsigned cp = cp0 + wwv[u];
+        const unsigned t2 = cp / nbot;
+        const unsigned off = (t2 / nmid) * stop + (t2 % nmid) * smid + cp % nbot + offc[u];
+        np[u] = P[off];
+        nt[u] = t[off];
+      }
+    }
+  };
+  // two slices per wave, each with its own accumulators (four independent
+  // chains of matrix-core steps in each phase)
+  big_d4 gr0 = {0.0, 0.0, 0.0, 0.0}, gi0 = gr0, gr1 = gr0, gi1 = gr0;
+  if (blockIdx.x < ntiles) load_tile(blockIdx.x);
+  for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int q = tid + u * kBigThreads;
+      const int ww = q % W, rest = q / W, xl = rest % CHI, xh = rest / CHI;
+      const int xj = j < idx ? xh : xl, xi = j < idx ? xl : xh;
+      Pt[ww][xj][xi] = np[u];
+      Tt[ww][xj][xi] = nt[u];
+    }
+    __syncthreads();
+    if (tile + gridDim.x < ntiles) load_tile(tile + gridDim.x);
+    const int s0 = 2 * w, s1 = 2 * w + 1;
+    // U = M_j P_w in single precision (v_mfma_f32_16x16x4f32, as the
+    // separate product's FP32 FMAs): lane l gets rows 4 (l / 16) + v of U,
+    // column l % 16 -- for each v a Gram k-step over rows {v, v + 4, v + 8,
+    // v + 12}, with A = conj(T_w[4 (l / 16) + v][l % 16]).
+    big_f4 ur0 = {0.f, 0.f, 0.f, 0.f}, ui0 = ur0, ur1 = ur0, ui1 = ur0;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const cx<R> p0 = Pt[s0][4 * ks + kq][mi], p1 = Pt[s1][4 * ks + kq][mi];
+      const float mr = (float)mv[ks].re, mim = (float)mv[ks].im;
+      const float p0r = (float)p0.re, p0i = (float)p0.im, p1r = (float)p1.re, p1i = (float)p1.im;
+      ur0 = __builtin_amdgcn_mfma_f32_16x16x4f32(mr, p0r, ur0, 0, 0, 0);
+      ur1 = __builtin_amdgcn_mfma_f32_16x16x4f32(mr, p1r, ur1, 0, 0, 0);
+      ui0 = __builtin_amdgcn_mfma_f32_16x16x4f32(mr, p0i, ui0, 0, 0, 0);
+      ui1 = __builtin_amdgcn_mfma_f32_16x16x4f32(mr, p1i, ui1, 0, 0, 0);
+      ur0 = __builtin_amdgcn_mfma_f32_16x16x4f32(-mim, p0i, ur0, 0, 0, 0);
+      ur1 = __builtin_amdgcn_mfma_f32_16x16x4f32(-mim, p1i, ur1, 0, 0, 0);
+      ui0 = __builtin_amdgcn_mfma_f32_16x16x4f32(mim, p0r, ui0, 0, 0, 0);
+      ui1 = __builtin_amdgcn_mfma_f32_16x16x4f32(mim, p1r, ui1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double ur0v = ur0[v], ui0v = ui0[v], ur1v = ur1[v], ui1v = ui1[v];
+      const cx<R> t0 = Tt[s0][4 * kq + v][mi], t1 = Tt[s1][4 * kq + v][mi];
+      const double tr0 = (double)t0.re, ti0 = (double)t0.im, tr1 = (double)t1.re, ti1 = (double)t1.im;
+      gr0 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr0, ur0v, gr0, 0, 0, 0);
+      gr1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr1, ur1v, gr1, 0, 0, 0);
+      gi0 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr0, ui0v, gi0, 0, 0, 0);
+      gi1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tr1, ui1v, gi1, 0, 0, 0);
+      gr0 = __builtin_amdgcn_mfma_f64_16x16x4f64(ti0, ui0v, gr0, 0, 0, 0);
+      gr1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ti1, ui1v, gr1, 0, 0, 0);
+      gi0 = __builtin_amdgcn_mfma_f64_16x16x4f64(-ti0, ur0v, gi0, 0, 0, 0);
+      gi1 = __builtin_amdgcn_mfma_f64_16x16x4f64(-ti1, ur1v, gi1, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const big_d4 gr = gr0 + gr1, gi = gi0 + gi1;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) red[w * CC + (kq + 4 * v) * CHI + mi] = mk<double>(gr[v], gi[v]);
+  __syncthreads();
+  cd* pp = part + ((long long)k * gridDim.x + blockIdx.x) * CC;
+  for (int q = tid; q < CC; q += blockDim.x) pp[q] = red[q] + red[CC + q] + red[2 * CC + q] + red[3 * CC + q];
 }
 
 template <typename R, int CHI>
@@ -366,9 +513,9 @@ long long big_ws_bytes(const DevGraph& g, int max_deg, int count, int rb) {
 }
 
 template <typename R, int CHI>
-static hipError_t big_bp_level(const DevGraph& g, int max_deg, const int* nodes, int count, const void* T,
-                               const void* n2e_old, const void* e2n_old, void* n2e_new, const uint8_t* node_src,
-                               void* ws, const int* inst_active, hipStream_t s) {
+static hipError_t big_bp_level(const DevGraph& g, int min_deg, int max_deg, const int* nodes, int count,
+                               const void* T, const void* n2e_old, const void* e2n_old, void* n2e_new,
+                               const uint8_t* node_src, void* ws, const int* inst_active, hipStream_t s) {
   const long long smax = ipow(CHI, max_deg) * g.phys;
   const long long cols = smax / CHI;
   const BigPlanner plan = big_plan(max_deg);
@@ -377,7 +524,21 @@ static hipError_t big_bp_level(const DevGraph& g, int max_deg, const int* nodes,
   cd* part = (cd*)((char*)ws + (long long)nb * count * smax * sizeof(cx<R>));
   const int chunks = big_gram_chunks(cols);
   const int pblocks = big_col_blocks(cols);
-  for (const BigOp& op : plan.ops) {
+  // chi 16, every node of the plan's degree: a product whose result only
+  // feeds the next Gram runs fused with it
+  const bool fuse = sizeof(R) == 4 && CHI == 16 && min_deg == max_deg && max_deg >= 4 && smax < (1LL << 31);
+  for (size_t i = 0; i < plan.ops.size(); ++i) {
+    const BigOp& op = plan.ops[i];
+    if (fuse && !op.gram && i + 1 < plan.ops.size() && plan.ops[i + 1].gram && plan.ops[i + 1].src == op.dst) {
+      const int idx = plan.ops[i + 1].axis;
+      hipLaunchKernelGGL((k_big_prod_gram16<R>), dim3(chunks, count), dim3(kBigThreads), 0, s, g, nodes, op.axis,
+                         idx, op.src, nb, (const cx<R>*)T, (const cx<R>*)n2e_old, (const cx<R>*)e2n_old,
+                         (const cx<R>*)n2e_new, node_src, (const cx<R>*)buf, smax, part, inst_active);
+      hipLaunchKernelGGL((k_big_gram_reduce<R, CHI>), dim3(count), dim3(kBigThreads), 0, s, g, nodes, idx,
+                         (const cd*)part, chunks, (cx<R>*)n2e_new, inst_active);
+      ++i;
+      continue;
+    }
     if (!op.gram) {
       hipLaunchKernelGGL((k_big_plan_product<R, CHI>), dim3(pblocks, count), dim3(kBigThreads), 0, s, g, nodes,
                          op.axis, op.src, op.dst, nb, (const cx<R>*)T, (const cx<R>*)n2e_old, (const cx<R>*)e2n_old,
@@ -392,18 +553,18 @@ static hipError_t big_bp_level(const DevGraph& g, int max_deg, const int* nodes,
   return hipGetLastError();
 }
 
-hipError_t launch_bp_nodes_big(int rb, const DevGraph& g, int max_deg, const int* nodes, int count, const void* T,
-                               const void* n2e_old, const void* e2n_old, void* n2e_new, const uint8_t* node_src,
-                               void* ws, const int* inst_active, hipStream_t s) {
+hipError_t launch_bp_nodes_big(int rb, const DevGraph& g, int min_deg, int max_deg, const int* nodes, int count,
+                               const void* T, const void* n2e_old, const void* e2n_old, void* n2e_new,
+                               const uint8_t* node_src, void* ws, const int* inst_active, hipStream_t s) {
   if (count <= 0) return hipSuccess;
   if (rb == 4)
-    return g.chi == 16 ? big_bp_level<float, 16>(g, max_deg, nodes, count, T, n2e_old, e2n_old, n2e_new, node_src,
+    return g.chi == 16 ? big_bp_level<float, 16>(g, min_deg, max_deg, nodes, count, T, n2e_old, e2n_old, n2e_new, node_src,
                                                  ws, inst_active, s)
-                       : big_bp_level<float, 32>(g, max_deg, nodes, count, T, n2e_old, e2n_old, n2e_new, node_src,
+                       : big_bp_level<float, 32>(g, min_deg, max_deg, nodes, count, T, n2e_old, e2n_old, n2e_new, node_src,
                                                  ws, inst_active, s);
-  return g.chi == 16 ? big_bp_level<double, 16>(g, max_deg, nodes, count, T, n2e_old, e2n_old, n2e_new, node_src, ws,
+  return g.chi == 16 ? big_bp_level<double, 16>(g, min_deg, max_deg, nodes, count, T, n2e_old, e2n_old, n2e_new, node_src, ws,
                                                 inst_active, s)
-                     : big_bp_level<double, 32>(g, max_deg, nodes, count, T, n2e_old, e2n_old, n2e_new, node_src, ws,
+                     : big_bp_level<double, 32>(g, min_deg, max_deg, nodes, count, T, n2e_old, e2n_old, n2e_new, node_src, ws,
                                                 inst_active, s);
 }
 

@@ -325,8 +325,8 @@ static hipError_t bp_sweep_launch(dq_graph* g, int rb, const void* T, const void
   const long long wsb = (node_ws_bytes(g, rb) + 255) / 256 * 256;
   for (int l = 0; l < g->n_levels && e == hipSuccess; ++l) {
     const int b = g->level_ptr[l], cnt = g->level_ptr[l + 1] - b;
-    e = big ? launch_bp_nodes_big(rb, g->dev, g->max_deg, g->d_level_nodes + b, cnt, T, n2e_in, e2n_in, n2e_out,
-                                  g->d_node_src, g->ws, inst_active, s)
+    e = big ? launch_bp_nodes_big(rb, g->dev, g->min_deg, g->max_deg, g->d_level_nodes + b, cnt, T, n2e_in, e2n_in,
+                                  n2e_out, g->d_node_src, g->ws, inst_active, s)
             : launch_bp_nodes(rb, g->dev, g->d_level_nodes + b, cnt, T, n2e_in, e2n_in, n2e_out, g->d_node_src, g->ws,
                               wsb, inst_active, g->min_deg, g->max_deg, s);
   }
@@ -499,8 +499,8 @@ static hipError_t vidal_distance_launch(dq_graph* g, int rb, const void* gammas,
     return launch_vidal_distance_fast(g->dev, gammas, lam, eps, g->d_all_nodes, g->d_zero_src, g->d_inst_edge_ptr,
                                       g->n_inst, out, ws, inst_active, s);
   if (big_applicable(g->dev, g->max_deg))
-    return launch_vidal_distance_big(rb, g->dev, g->max_deg, gammas, lam, eps, g->d_all_nodes, g->d_zero_src,
-                                     g->d_inst_edge_ptr, g->n_inst, out, ws, inst_active, s);
+    return launch_vidal_distance_big(rb, g->dev, g->min_deg, g->max_deg, gammas, lam, eps, g->d_all_nodes,
+                                     g->d_zero_src, g->d_inst_edge_ptr, g->n_inst, out, ws, inst_active, s);
   return launch_vidal_distance(rb, g->dev, gammas, lam, eps, g->node_dist, g->d_inst_node_ptr, g->d_inst_edge_ptr,
                                g->n_inst, out, ws, vidal_generic_ws_stride(g), inst_active, s);
 }

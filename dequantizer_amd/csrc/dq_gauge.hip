@@ -845,7 +845,8 @@ long long vidal_big_ws_bytes(const DevGraph& g, int max_deg, int rb) {
   return big_ws_bytes(g, max_deg, g.n_nodes, rb) + 2 * (2LL * g.n_edges * cc * 2 * rb) + 2LL * g.n_edges * 8 + 512;
 }
 
-hipError_t launch_vidal_distance_big(int rb, const DevGraph& g, int max_deg, const void* gam, const double* lam,
+hipError_t launch_vidal_distance_big(int rb, const DevGraph& g, int min_deg, int max_deg, const void* gam,
+                                     const double* lam,
                                      double eps, const int* all_nodes, const uint8_t* zero_src,
                                      const int* inst_edge_ptr, int n_inst, double* inst_dist, void* ws,
                                      const int* inst_active, hipStream_t s) {
@@ -861,7 +862,8 @@ hipError_t launch_vidal_distance_big(int rb, const DevGraph& g, int max_deg, con
     hipLaunchKernelGGL(k_lam2_diag_any<double>, dim3(grid_for(nm, 256)), dim3(256), 0, s, g, lam, (cx<double>*)msg);
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) return err;
-  err = launch_bp_nodes_big(rb, g, max_deg, all_nodes, g.n_nodes, gam, msg, msg, R, zero_src, ws, inst_active, s);
+  err = launch_bp_nodes_big(rb, g, min_deg, max_deg, all_nodes, g.n_nodes, gam, msg, msg, R, zero_src, ws,
+                            inst_active, s);
   if (err != hipSuccess) return err;
   const unsigned n16 = (unsigned)((2LL * g.n_edges + 3) / 4), n32 = (unsigned)(2LL * g.n_edges);
   if (g.chi == 16 && rb == 4)

@@ -212,19 +212,20 @@ __device__ inline int find_slot(const DevGraph& g, int node, int e, int side) {
 // dst = src with matrix M applied on axis j of a tensor with `d` bonds of
 // size chi and a trailing phys axis: dst[o, a, in] = sum_b M[a*chi+b] src[o, b, in]
 // (transposed=true uses M[b*chi+a]).  Whole block cooperates.
-// For chi <= 32 each thread takes whole columns (all indices but axis j): the
-// chi values of a column are read once into registers and the chi outputs
-// written from them, so the tensor is read and written once per product
-// (the element loop below reads every input chi times); M is staged in LDS
-// (as R) first.  Both forms sum over b in the same order: the results are
-// bitwise equal.
+// For chi <= 32 and tensors of 4096 elements or more, each thread takes whole
+// columns (all indices but axis j): the chi values of a column are read once
+// into registers and the chi outputs written from them, so the tensor is read
+// and written once per product (the element loop below reads every input chi
+// times); M is staged in LDS (as R) first.  Small tensors keep the element
+// loop, which needs no block barrier.  Both forms sum over b in the same
+// order: the results are bitwise equal.
 template <typename R, typename MT>
 __device__ void mode_product(cx<R>* dst, const cx<R>* src, const cx<MT>* M, int chi, int d, int phys, int j,
                              bool transposed) {
   const long long inner = ipow(chi, d - 1 - j) * phys;
   const long long total = ipow(chi, d) * phys;
   constexpr int kReg = 32;
-  if (chi <= kReg) {
+  if (chi <= kReg && total >= 4096) {
     __shared__ cx<R> Ms[kReg * kReg];
     __syncthreads();  // earlier readers of Ms are done
     for (int q = threadIdx.x; q < chi * chi; q += blockDim.x) {

@@ -58,12 +58,12 @@ hipError_t launch_absorb_fast(const DevGraph& g, const void* T, const cd* X, voi
 bool big_applicable(const DevGraph& g, int max_deg);
 long long big_ws_bytes(const DevGraph& g, int max_deg, int count, int real_bytes);
 long long vidal_big_ws_bytes(const DevGraph& g, int max_deg, int real_bytes);
-hipError_t launch_vidal_distance_big(int real_bytes, const DevGraph& g, int max_deg, const void* gam,
+hipError_t launch_vidal_distance_big(int real_bytes, const DevGraph& g, int min_deg, int max_deg, const void* gam,
                                      const double* lam, double eps, const int* all_nodes, const uint8_t* zero_src,
                                      const int* inst_edge_ptr, int n_inst, double* inst_dist, void* ws,
                                      const int* inst_active, hipStream_t s);
-hipError_t launch_bp_nodes_big(int real_bytes, const DevGraph& g, int max_deg, const int* nodes, int count,
-                               const void* T, const void* n2e_old, const void* e2n_old, void* n2e_new,
+hipError_t launch_bp_nodes_big(int real_bytes, const DevGraph& g, int min_deg, int max_deg, const int* nodes,
+                               int count, const void* T, const void* n2e_old, const void* e2n_old, void* n2e_new,
                                const uint8_t* node_src, void* ws, const int* inst_active, hipStream_t s);
 hipError_t launch_absorb_big(int real_bytes, const DevGraph& g, int max_deg, const int* nodes, int count,
                              const void* T, const cd* X, void* gammas, void* ws, const int* inst_active,

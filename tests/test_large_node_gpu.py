@@ -117,3 +117,46 @@ def test_degree6_chi16_distance_and_density_matrices(star_case, dtype, tol):
         r_p = host(dq.get_two_side_density_matrix(gp.edges[eid], ten, out))
         r_o = om.two_side_density_matrix(go.edges[eid], tensors, ref)
         assert np.abs(r_p - r_o).max() < tol, eid
+
+
+def _uniform_pair(name):
+    """Graphs whose nodes all have one degree >= 4 at chi = 16: the streaming
+    node update fuses each output slot's last product with its Gram."""
+    gp, go = dq.TensorGraph(), og.TensorGraph()
+    if name == "torus4x4":  # degree 4, periodic 4 x 4 lattice
+        return dq.get_nd_lattice([4, 4], 2, 16, False), og.nd_lattice([4, 4], 2, 16, False)
+    for g in (gp, go):  # K7: degree 6, 33.5 M elements per node
+        for _ in range(7):
+            g.add_node(2)
+        for a in range(7):
+            for b in range(a + 1, 7):
+                g.add_edge((a, b), 16)
+    return gp, go
+
+
+@pytest.mark.parametrize("name", ["torus4x4", "k7"])
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-11), (np.complex64, 2e-5)])
+def test_uniform_degree_fused_node_update_matches_oracle(name, dtype, tol):
+    """One synchronous sweep on graphs of uniform degree 4 (every (product
+    axis, output slot) pair of the degree-4 plan, the contiguous and the
+    strided tile layouts) and 6 against the oracle."""
+    gp, go = _uniform_pair(name)
+    rng = np.random.default_rng(11)
+    tensors, msgs = {}, {}
+    for n in go.nodes.values():
+        shape = n.bond_shape + (2,)
+        scale = np.sqrt(2.0 * np.prod(shape))
+        tensors[n.id] = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / scale
+    for el in gp.get_traversal_iterator():
+        for nb in el.neighbors:
+            a = rng.standard_normal((16, 16)) + 1j * rng.standard_normal((16, 16))
+            m = a @ a.conj().T
+            msgs[dq.MessageID(el.id, nb.id)] = m / np.trace(m)
+    ref = om.belief_propagation_map(go.traversal(), True)(tensors, to_oracle_messages(msgs))
+    bp = dq.get_belief_propagation_map(list(gp.get_traversal_iterator()), True)
+    out = bp({k: v.astype(dtype) for k, v in tensors.items()}, {k: v.astype(dtype) for k, v in msgs.items()})
+    worst = 0.0
+    for k, v in ref.items():
+        got = host(out[dq.MessageID(k.src, k.dst)])
+        worst = max(worst, np.linalg.norm(got - v) / np.linalg.norm(v))
+    assert worst < tol, worst
