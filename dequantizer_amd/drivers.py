@@ -199,7 +199,7 @@ def run_eagle_simulation(max_chi: int, layers_per_regauging: int, trotter_steps:
 def run_eagle_simulations(thetas, max_chi: int, trotter_steps: int, accuracy: float, is_infinite: bool = False,
                           synchronous_update: bool = True, max_belief_propagation_iterations: Optional[int] = 100,
                           key=PRNGKey(42), traversal_type: str = "dfs", *, dtype=None,
-                          chi: Optional[int] = None) -> List[List[float]]:
+                          chi: Optional[int] = None, sweep_events: Optional[list] = None) -> List[List[float]]:
     """The kicked-Ising dynamics of ``run_eagle_simulation`` (three colour
     layers, a regauge after each) for several RX angles at once: one
     instance per angle in one batched engine, every launch covering all
@@ -213,6 +213,8 @@ def run_eagle_simulations(thetas, max_chi: int, trotter_steps: int, accuracy: fl
     n = len(thetas)
     emu = BatchedBPEmulator([lattice] * n, [key] * n, max_chi, 10 ** 9, accuracy, max_belief_propagation_iterations,
                             synchronous_update, traversal_type, dtype=dtype or torch.complex64, chi=chi)
+    # optional timing hook: (start, end, active-instance mask, sweeps) per BP sweep launch
+    emu.sweep_events = sweep_events
     traverser = list(lattice.get_traversal_iterator() or iter([]))
     edges = [el.id for el in traverser if isinstance(el, Edge)]
     colours = edge_colouring(lattice)
