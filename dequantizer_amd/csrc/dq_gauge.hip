@@ -564,7 +564,7 @@ __global__ void __launch_bounds__(256)
     if (chi == 32 && blockDim.x == 256) {
       __shared__ double sg[32];
       __shared__ int pm[32];
-      block_jacobi_svd_n<32, cd, false>(A, nullptr, nullptr, sg, pm, rot);
+      block_jacobi_svd_n<32, cd, false, false>(A, nullptr, nullptr, sg, pm, rot);
       for (int k = 0; k < 32; ++k) dist += sg[k];
     } else if (chi == 16) {
       __shared__ double sg16[16];
@@ -814,7 +814,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int t = 0; t < 4; ++t) G[threadIdx.x + 256 * t] = h[t];
   __syncthreads();
-  block_jacobi_svd_n<32, lds_cd, false>((lds_cd*)G, nullptr, nullptr, sg, perm, scal);
+  block_jacobi_svd_n<32, lds_cd, false, false>((lds_cd*)G, nullptr, nullptr, sg, perm, scal);
   if (threadIdx.x == 0) {
     double nuc = 0.0;
     for (int k = 0; k < 32; ++k) nuc += sg[k];

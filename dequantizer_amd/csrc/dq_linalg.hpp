@@ -637,19 +637,18 @@ __device__ inline double wave_nuclear_norm_n(cd* G) {
 // Block-parallel fixed-size one-sided Jacobi SVD (N = 32, 256 threads): the
 // 16 pairs of a round run on 16-lane groups (4 per wave), 2 rows per lane, one
 // block barrier per round.  Same outputs as jacobi_svd.  scal: N doubles.
-// kVectors = false: singular values only (sigma, perm); V is neither formed
-// nor read, U is not written (both may be null).  kU = false: V is formed
-// but U is not written (may be null).  G is left holding the rotated
+// kV = false: no right vectors (V is neither formed nor read; may be null);
+// kU = false: U is not written (may be null).  G is left holding the rotated
 // columns G J in their unsorted order (column p of G J goes with column p of
 // J = V; sigma and perm give the sorted order).
-template <int N, typename P, bool kVectors = true, bool kU = true>
+template <int N, typename P, bool kV = true, bool kU = true>
 __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int* perm, double* scal) {
   constexpr int NP = N / 2;
   constexpr int LPP = 256 / NP;  // 16 lanes per pair for N = 32
   constexpr int RPL = N / LPP;   // rows per lane
   static_assert(LPP <= 64 && RPL >= 1, "block SVD sizing");
   const int k = threadIdx.x / LPP, sub = threadIdx.x % LPP;
-  if (kVectors)
+  if (kV)
     for (int i = threadIdx.x; i < N * N; i += blockDim.x) cst(V, i, mk<double>((i / N) == (i % N) ? 1.0 : 0.0, 0.0));
   double f = 0.0;
   for (int i = threadIdx.x; i < N * N; i += blockDim.x) f += norm2(cld(G, i));
@@ -670,7 +669,7 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
       for (int t = 0; t < RPL; ++t) {
         x[t] = cld(G, p * N + sub + t * LPP);
         y[t] = cld(G, q * N + sub + t * LPP);
-        if (kVectors) {
+        if (kV) {
           vx[t] = cld(V, p * N + sub + t * LPP);
           vy[t] = cld(V, q * N + sub + t * LPP);
         }
@@ -694,7 +693,7 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
           const cd ey = e * y[t];
           cst(G, p * N + sub + t * LPP, mk<double>(c * x[t].re - s * ey.re, c * x[t].im - s * ey.im));
           cst(G, q * N + sub + t * LPP, mk<double>(s * x[t].re + c * ey.re, s * x[t].im + c * ey.im));
-          if (kVectors) {
+          if (kV) {
             const cd evy = e * vy[t];
             cst(V, p * N + sub + t * LPP, mk<double>(c * vx[t].re - s * evy.re, c * vx[t].im - s * evy.im));
             cst(V, q * N + sub + t * LPP, mk<double>(s * vx[t].re + c * evy.re, s * vx[t].im + c * evy.im));
@@ -719,7 +718,7 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
     sigma[rank] = me;
   }
   __syncthreads();
-  if (!kVectors || !kU) return;
+  if (!kU) return;
   for (int t = threadIdx.x; t < N * N; t += blockDim.x) {
     const int j = t / N, i = t % N;
     const double sg = sigma[j];
