@@ -670,19 +670,46 @@ def adjoint_timing(ef, args):
     e, dk, dj = run.value_and_grad(knots, memory_gb=args.adjoint_memory_gb)
     torch.cuda.synchronize()
     t3 = time.perf_counter()
-    return {"seconds": t3 - t2, "first_call_seconds": t2 - t1, "forward_seconds": t1 - t0, "setup_seconds": setup,
-            "steps": args.adjoint_steps,
-            "knots": args.knots, "couplings": int(ef.edges_number), "dtype": "c128", "backward_cut": run.cut,
-            "sweeps_per_regauge": (f"to residual {args.adjoint_bp_tol:g} (implicit adjoint)" if args.adjoint_bp_tol > 0
-                                   else args.fixed_sweeps or 10),
-            "bp_sweeps_per_call": int(np.sum(run.forward_sweeps)) if args.adjoint_bp_tol > 0 else None,
-            "adjoint_iterations_per_call": int(np.sum(run.adjoint_iterations)) if args.adjoint_bp_tol > 0 else None,
-            "memory_budget_gb": args.adjoint_memory_gb,
-            "segment_steps": run.segment_steps, "record_gb_per_step": run.record_bytes() / 2 ** 30, "energy": e, "energy_forward_only": e_fwd,
-            "grad_norm_knots": float(np.linalg.norm(dk)), "grad_norm_couplings": float(np.linalg.norm(dj)),
-            "peak_device_memory_gb": torch.cuda.max_memory_allocated() / 1e9,
-            "held_before_gb": held_gb,
-            "allocator_retries": torch.cuda.memory_stats().get("num_alloc_retries", 0) - retries0}
+    out = {"seconds": t3 - t2, "first_call_seconds": t2 - t1, "forward_seconds": t1 - t0, "setup_seconds": setup,
+           "steps": args.adjoint_steps,
+           "knots": args.knots, "couplings": int(ef.edges_number), "dtype": "c128", "backward_cut": run.cut,
+           "sweeps_per_regauge": (f"to residual {args.adjoint_bp_tol:g} (implicit adjoint)" if args.adjoint_bp_tol > 0
+                                  else args.fixed_sweeps or 10),
+           "bp_sweeps_per_call": int(np.sum(run.forward_sweeps)) if args.adjoint_bp_tol > 0 else None,
+           "adjoint_iterations_per_call": int(np.sum(run.adjoint_iterations)) if args.adjoint_bp_tol > 0 else None,
+           "memory_budget_gb": args.adjoint_memory_gb,
+           "segment_steps": run.segment_steps, "record_gb_per_step": run.record_bytes() / 2 ** 30, "energy": e,
+           "energy_forward_only": e_fwd,
+           "grad_norm_knots": float(np.linalg.norm(dk)), "grad_norm_couplings": float(np.linalg.norm(dj)),
+           "peak_device_memory_gb": torch.cuda.max_memory_allocated() / 1e9,
+           "held_before_gb": held_gb,
+           "allocator_retries": torch.cuda.memory_stats().get("num_alloc_retries", 0) - retries0,
+           "c64_forward": None}
+    if args.adjoint_bp_tol <= 0:
+        # the complex64 forward (the benchmark engine's precision) with the
+        # complex128 adjoint: time, and energy and gradients against the above
+        del run
+        gc.collect()
+        torch.cuda.empty_cache()
+        r64 = AnnealingGradient(ef, PRNGKey(args.seed), chi=args.chi, n_steps=args.adjoint_steps, dt=args.dt,
+                                n_knots=args.knots, fixed_sweeps=args.fixed_sweeps or 10,
+                                forward_dtype=torch.complex64)
+        r64.value_and_grad(knots, memory_gb=args.adjoint_memory_gb)
+        torch.cuda.synchronize()
+        t4 = time.perf_counter()
+        e64, dk64, dj64 = r64.value_and_grad(knots, memory_gb=args.adjoint_memory_gb)
+        torch.cuda.synchronize()
+        t5 = time.perf_counter()
+        out["c64_forward"] = {"seconds": t5 - t4, "forward_dtype": "c64", "adjoint_dtype": "c128",
+                              "segment_steps": r64.segment_steps,
+                              "record_gb_per_step": r64.record_bytes() / 2 ** 30, "energy": e64,
+                              "energy_abs_diff_vs_c128_forward": abs(e64 - e),
+                              "knot_grad_rel_diff": float(np.abs(dk64 - dk).max() / np.abs(dk).max()),
+                              "coupling_grad_rel_diff": float(np.abs(dj64 - dj).max() / np.abs(dj).max())}
+        del r64
+        gc.collect()
+        torch.cuda.empty_cache()
+    return out
 
 
 def plain_gpu_baseline(ann, ef, step_times, args, sample_sweeps=2, sample_gates=150):

@@ -49,6 +49,12 @@ _Z1 = np.array([1.0, 1.0, -1.0, -1.0])
 _Z2 = np.array([1.0, -1.0, 1.0, -1.0])
 
 
+def _wide(x: torch.Tensor) -> torch.Tensor:
+    """A forward record in complex128 for the adjoint kernels (a copy when
+    the forward ran in complex64)."""
+    return x if x.dtype != torch.complex64 else x.to(torch.complex128)
+
+
 def knot_weights(n_knots: int, n_steps: int) -> np.ndarray:
     """(n_steps, n_knots) linear-interpolation weights: s(t_k) = W @ knots at
     t_k = k / (n_steps - 1), knots at linspace(0, 1, n_knots)."""
@@ -88,11 +94,19 @@ class AnnealingGradient:
     stopping when its cotangent has fallen by ``bp_tol`` (both at most
     ``max_sweeps``; hitting the cap warns).  ``forward_sweeps`` and
     ``adjoint_iterations`` collect the counts of every regauge since they
-    were last cleared."""
+    were last cleared.
+
+    ``forward_dtype``: the precision of the forward's node tensors and
+    messages (torch.complex128, or torch.complex64 for the complex64 engine
+    of the annealing benchmark).  The adjoint always runs in complex128: the
+    records of a complex64 forward (states, sweep inputs, fixed points, in
+    complex64; the decompositions' tapes, in FP64 as always) are widened when
+    the backward reads them, so the gradient is that of the FP64 operations
+    evaluated along the complex64 trajectory."""
 
     def __init__(self, ef, key, chi: int, n_steps: int, dt: float = 0.1, n_knots: int = 32,
                  fixed_sweeps: Optional[int] = 10, traversal: str = "bfs", cut: float = 1e-6,
-                 bp_tol: float = 1e-12, max_sweeps: int = 1000):
+                 bp_tol: float = 1e-12, max_sweeps: int = 1000, forward_dtype: torch.dtype = torch.complex128):
         if not 0.0 <= cut < 1.0:
             raise ValueError(f"cut {cut} outside [0, 1)")
         self.batched = not isinstance(ef, EnergyFunction)
@@ -101,6 +115,10 @@ class AnnealingGradient:
         if not efs or len(keys) != len(efs):
             raise ValueError(f"{len(efs)} energy functions and {len(keys)} keys")
         self.cut = float(cut)
+        if forward_dtype not in (torch.complex64, torch.complex128):
+            raise ValueError(f"forward_dtype {forward_dtype} is neither complex64 nor complex128")
+        self.fdt = forward_dtype
+        self.fcode = _lib.dtype_code(forward_dtype)
         self.ef, self.efs = efs[0], efs
         self.n_inst = len(efs)
         self.n_steps, self.dt, self.n_knots = int(n_steps), float(dt), int(n_knots)
@@ -110,7 +128,7 @@ class AnnealingGradient:
         self.W = knot_weights(self.n_knots, self.n_steps)
         graphs = [tensor_graph_from(e) for e in efs]
         emu = BatchedBPEmulator(graphs, keys, chi, max(e.edges_number for e in efs), 1e-5, 100, True, traversal,
-                                dtype=torch.complex128, chi=chi, fixed_sweeps=self.K)
+                                dtype=forward_dtype, chi=chi, fixed_sweeps=self.K)
         for inst, e in enumerate(efs):
             for i in range(e.nodes_number):
                 emu.queue_q1(inst, HADAMARD, i)
@@ -178,17 +196,18 @@ class AnnealingGradient:
     # ---------------------------------------------------------- forward
     def _regauge(self, gam, lam, record: bool):
         L, h, st = _lib.lib(), self.h, _lib.stream_handle()
+        fc = self.fcode
         ts = torch.empty_like(gam)
-        _lib.check(L.dq_symmetric_gauge(h.ptr, _lib.DQ_C128, _lib.ptr(gam), _lib.ptr(lam), EPS, _lib.ptr(ts), st))
-        n2e = torch.zeros(2 * self.E * self.cc, dtype=torch.complex128, device=DEVICE)
+        _lib.check(L.dq_symmetric_gauge(h.ptr, fc, _lib.ptr(gam), _lib.ptr(lam), EPS, _lib.ptr(ts), st))
+        n2e = torch.zeros(2 * self.E * self.cc, dtype=self.fdt, device=DEVICE)
         e2n = torch.zeros_like(n2e)
-        _lib.check(L.dq_lambdas_to_messages(h.ptr, _lib.DQ_C128, _lib.ptr(lam), _lib.ptr(n2e), _lib.ptr(e2n), st))
+        _lib.check(L.dq_lambdas_to_messages(h.ptr, fc, _lib.ptr(lam), _lib.ptr(n2e), _lib.ptr(e2n), st))
         inputs = []
         if self.K is not None:
             for _ in range(self.K):
                 inputs.append(e2n)
                 nn, ne = torch.empty_like(n2e), torch.empty_like(n2e)
-                _lib.check(L.dq_bp_sweep(h.ptr, _lib.DQ_C128, _lib.ptr(ts), _lib.ptr(n2e), _lib.ptr(e2n),
+                _lib.check(L.dq_bp_sweep(h.ptr, fc, _lib.ptr(ts), _lib.ptr(n2e), _lib.ptr(e2n),
                                          _lib.ptr(nn), _lib.ptr(ne), None, st))
                 n2e, e2n = nn, ne
         else:
@@ -199,7 +218,7 @@ class AnnealingGradient:
         lam_new = torch.empty_like(lam)
         X = torch.empty(2 * self.E * self.cc, dtype=torch.complex128, device=DEVICE)
         tape = torch.empty(self.svd_tape_bytes, dtype=torch.uint8, device=DEVICE)
-        _lib.check(L.dq_vidal_gauge_tape(h.ptr, _lib.DQ_C128, _lib.ptr(ts), _lib.ptr(n2e), EPS, _lib.ptr(gam_new),
+        _lib.check(L.dq_vidal_gauge_tape(h.ptr, fc, _lib.ptr(ts), _lib.ptr(n2e), EPS, _lib.ptr(gam_new),
                                          _lib.ptr(lam_new), _lib.ptr(X), _lib.ptr(tape), st))
         rec = (gam, lam, ts, inputs, X, tape) if record else None
         return gam_new, lam_new, rec
@@ -213,9 +232,9 @@ class AnnealingGradient:
         dist = torch.empty(1, dtype=torch.float64, device=DEVICE)
         for it in range(1, self.max_sweeps + 1):
             nn, ne = torch.empty_like(n2e), torch.empty_like(n2e)
-            _lib.check(L.dq_bp_sweep(h.ptr, _lib.DQ_C128, _lib.ptr(ts), _lib.ptr(n2e), _lib.ptr(e2n), _lib.ptr(nn),
+            _lib.check(L.dq_bp_sweep(h.ptr, self.fcode, _lib.ptr(ts), _lib.ptr(n2e), _lib.ptr(e2n), _lib.ptr(nn),
                                      _lib.ptr(ne), None, st))
-            _lib.check(L.dq_messages_distance(h.ptr, _lib.DQ_C128, _lib.ptr(nn), _lib.ptr(ne), _lib.ptr(n2e),
+            _lib.check(L.dq_messages_distance(h.ptr, self.fcode, _lib.ptr(nn), _lib.ptr(ne), _lib.ptr(n2e),
                                               _lib.ptr(e2n), _lib.ptr(dist), st))
             prev, n2e, e2n = e2n, nn, ne
             if float(dist) < self.bp_tol:
@@ -238,7 +257,7 @@ class AnnealingGradient:
         for o2, c2 in self.launches:
             tape = torch.empty(c2 * self.su_tape_bytes, dtype=torch.uint8, device=DEVICE)
             pre = (gam.clone(), lam.clone()) if record else None
-            _lib.check(L.dq_simple_update_tape(h.ptr, _lib.DQ_C128, _lib.ptr(gam), _lib.ptr(lam),
+            _lib.check(L.dq_simple_update_tape(h.ptr, self.fcode, _lib.ptr(gam), _lib.ptr(lam),
                                                _lib.ptr(self.d_meta[o2:]), c2, _lib.ptr(halves[o2:]), 2, EPS,
                                                self.chi, _lib.ptr(err[o2:]), _lib.ptr(ent[o2:]), _lib.ptr(tape), st))
             if record:
@@ -247,7 +266,7 @@ class AnnealingGradient:
             rec["halves"] = halves
             rec["pre_mix"] = gam.clone()
         mg = torch.from_numpy(np.broadcast_to(self._mixing(tau_m), (self.N, 2, 2)).copy()).to(DEVICE)
-        _lib.check(L.dq_apply1(h.ptr, _lib.DQ_C128, _lib.ptr(gam), _lib.ptr(self.d_nodes_all), self.N, _lib.ptr(mg),
+        _lib.check(L.dq_apply1(h.ptr, self.fcode, _lib.ptr(gam), _lib.ptr(self.d_nodes_all), self.N, _lib.ptr(mg),
                                st))
         if record:
             rec["mix"] = mg
@@ -255,18 +274,19 @@ class AnnealingGradient:
 
     def _observables(self, gam, lam):
         L, h, st = _lib.lib(), self.h, _lib.stream_handle()
+        fc = self.fcode
         gam, lam, rec = self._regauge(gam, lam, True)
         ts = torch.empty_like(gam)
-        _lib.check(L.dq_symmetric_gauge(h.ptr, _lib.DQ_C128, _lib.ptr(gam), _lib.ptr(lam), EPS, _lib.ptr(ts), st))
-        n_d = torch.zeros(2 * self.E * self.cc, dtype=torch.complex128, device=DEVICE)
+        _lib.check(L.dq_symmetric_gauge(h.ptr, fc, _lib.ptr(gam), _lib.ptr(lam), EPS, _lib.ptr(ts), st))
+        n_d = torch.zeros(2 * self.E * self.cc, dtype=self.fdt, device=DEVICE)
         e_d = torch.zeros_like(n_d)
-        _lib.check(L.dq_lambdas_to_messages(h.ptr, _lib.DQ_C128, _lib.ptr(lam), _lib.ptr(n_d), _lib.ptr(e_d), st))
+        _lib.check(L.dq_lambdas_to_messages(h.ptr, fc, _lib.ptr(lam), _lib.ptr(n_d), _lib.ptr(e_d), st))
         n1, n2 = self.n_spins, self.n_pairs
         r1 = torch.empty((n1, 2, 2), dtype=torch.complex128, device=DEVICE)
         r2 = torch.empty((n2, 4, 4), dtype=torch.complex128, device=DEVICE)
-        _lib.check(L.dq_rdm1(h.ptr, _lib.DQ_C128, _lib.ptr(ts), _lib.ptr(e_d), _lib.ptr(self.rdm_nodes), n1,
+        _lib.check(L.dq_rdm1(h.ptr, fc, _lib.ptr(ts), _lib.ptr(e_d), _lib.ptr(self.rdm_nodes), n1,
                              _lib.ptr(r1), st))
-        _lib.check(L.dq_rdm2(h.ptr, _lib.DQ_C128, _lib.ptr(ts), _lib.ptr(e_d), _lib.ptr(self.rdm_edges), n2,
+        _lib.check(L.dq_rdm2(h.ptr, fc, _lib.ptr(ts), _lib.ptr(e_d), _lib.ptr(self.rdm_edges), n2,
                              _lib.ptr(r2), st))
         return (gam, lam, ts, rec), r1, r2
 
@@ -306,6 +326,7 @@ class AnnealingGradient:
         """Gradient of the state before a regauge from that after it."""
         L, h, st = _lib.lib(), self.h, _lib.stream_handle()
         gam, lam, ts, inputs, X, tape = rec
+        gam, ts, inputs = _wide(gam), _wide(ts), [_wide(x) for x in inputs]
         ts_bar = torch.zeros_like(ts)
         n_bar = torch.zeros(2 * self.E * self.cc, dtype=torch.complex128, device=DEVICE)
         X_bar = torch.empty_like(X)
@@ -361,8 +382,8 @@ class AnnealingGradient:
         regauge, the symmetric tensors, the K sweep inputs, the gauge's X and
         tape, the state before every gate level with the level's tape, the
         state before the mixing gates and the gate halves."""
-        state = self.gam0.numel() * 16 + self.lam0.numel() * 8
-        msg = 2 * self.E * self.cc * 16
+        state = self.gam0.numel() * self.gam0.element_size() + self.lam0.numel() * 8
+        msg = 2 * self.E * self.cc * self.gam0.element_size()
         levels = len(self.launches) * state + sum(c2 for _, c2 in self.launches) * self.su_tape_bytes
         sweeps = 2 if self.K is None else self.K + 1
         return 3 * state + sweeps * msg + self.svd_tape_bytes + levels + self.E * 2 * 16 * 16
@@ -371,7 +392,7 @@ class AnnealingGradient:
         """Steps per recomputation segment: the largest s for which the
         checkpoints (one state per segment) and the records of one segment
         fit in ``memory_gb``."""
-        state = self.gam0.numel() * 16 + self.lam0.numel() * 8
+        state = self.gam0.numel() * self.gam0.element_size() + self.lam0.numel() * 8
         rec, budget, n = self.record_bytes(), memory_gb * 2 ** 30, self.n_steps
         best = 0
         for s in range(1, n + 1):
@@ -420,6 +441,7 @@ class AnnealingGradient:
             gam, lam, rec = self._step(gam.clone(), lam.clone(), k, knots, J, keep)
             records[k] = rec if keep else None
         (gf, lf, tsf, reg_f), r1, r2 = self._observables(gam, lam)
+        gf, tsf = _wide(gf), _wide(tsf)
         r1h, r2h = r1.cpu().numpy(), r2.cpu().numpy()
         energy, zz = self._energy(r1h, r2h, J)
         # energy -> density matrices -> final symmetric tensors and lambdas
@@ -457,7 +479,8 @@ class AnnealingGradient:
             tau_c, tau_m = rec["tau"]
             # mixing gates
             mg_bar = torch.empty_like(rec["mix"])
-            _lib.check(L.dq_apply1_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(rec["pre_mix"]), _lib.ptr(self.d_nodes_all),
+            pre_mix = _wide(rec["pre_mix"])
+            _lib.check(L.dq_apply1_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(pre_mix), _lib.ptr(self.d_nodes_all),
                                        self.N, _lib.ptr(rec["mix"]), _lib.ptr(gam_bar), _lib.ptr(mg_bar), st))
             c, s = np.cos(tau_m), np.sin(tau_m)
             dG = np.array([[-s, -1j * c], [-1j * c, -s]])
@@ -467,6 +490,7 @@ class AnnealingGradient:
             # coupling gates, levels in reverse
             hb_all = torch.zeros((self.n_pairs, 2, 16), dtype=torch.complex128, device=DEVICE)
             for o2, c2, (pg, pl), tape in reversed(rec["levels"]):
+                pg = _wide(pg)
                 _lib.check(L.dq_simple_update_vjp(h.ptr, _lib.DQ_C128, _lib.ptr(pg), _lib.ptr(pl),
                                                   _lib.ptr(self.d_meta[o2:]), c2, _lib.ptr(rec["halves"][o2:]), 2,
                                                   EPS, self.cut, _lib.ptr(tape), _lib.ptr(gam_bar),

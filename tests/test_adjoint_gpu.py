@@ -431,6 +431,32 @@ def test_batched_gradient_matches_single_instance_runs(fixed_sweeps):
         assert np.allclose(dJ[i], dJ1, rtol=1e-9, atol=1e-11), (i, np.abs(dJ[i] - dJ1).max())
 
 
+@pytest.mark.parametrize("fixed_sweeps", [6, None])
+def test_complex64_forward_gradient_matches_complex128(fixed_sweeps):
+    """The forward in complex64 (the benchmark engine's precision) with the
+    complex128 adjoint reading its widened records: energy within the
+    complex64 rounding of the complex128 run and knot and coupling gradients
+    within 1e-4 relative of it, with fixed sweeps and with converged BP (the
+    complex64 fixed point at residual 1e-7)."""
+    import dequantizer_amd as dq
+    from dequantizer_amd.adjoint import AnnealingGradient
+    from dequantizer_amd.annealing import random_regular
+    ef = random_regular(12, 3, seed=9)
+    kw = dict(chi=8, n_steps=6, dt=0.3, n_knots=4, fixed_sweeps=fixed_sweeps)
+    knots = np.array([0.1, 0.35, 0.7, 0.95])
+    r128 = AnnealingGradient(ef, dq.PRNGKey(8), bp_tol=1e-13, **kw)
+    r64 = AnnealingGradient(ef, dq.PRNGKey(8), bp_tol=1e-7, forward_dtype=torch.complex64, **kw)
+    assert r64.gam0.dtype == torch.complex64
+    e1, dk1, dj1 = r128.value_and_grad(knots)
+    e2, dk2, dj2 = r64.value_and_grad(knots)
+    print("c64 forward: |dE|", abs(e2 - e1), "knots", np.abs(dk2 - dk1).max() / np.abs(dk1).max(),
+          "couplings", np.abs(dj2 - dj1).max() / np.abs(dj1).max(), flush=True)
+    assert abs(e2 - e1) < 1e-5 * ef.nodes_number, (e1, e2)
+    assert abs(e2 - r64.value(knots)) < 1e-10 * max(1.0, abs(e2))
+    assert np.abs(dk2 - dk1).max() < 1e-4 * np.abs(dk1).max(), (dk1, dk2)
+    assert np.abs(dj2 - dj1).max() < 1e-4 * np.abs(dj1).max(), np.abs(dj2 - dj1).max()
+
+
 def test_segmented_recomputation_matches_one_segment():
     """The backward recomputing segments of 1, 2 and 3 steps from their
     checkpoints returns the same energy, bit for bit, and the same gradients
