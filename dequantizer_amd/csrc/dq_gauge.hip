@@ -172,10 +172,13 @@ __global__ void __launch_bounds__(128)
     Mb[q] = mk<double>(0.5 * (b.re + bt.re), 0.5 * (b.im - bt.im));
   }
   __syncthreads();
+  // complex64 results without an adjoint record: the Jacobi iterations stop
+  // once the remaining rotations are below single-precision rounding
+  const bool loose = sizeof(R) == 4 && !tape;
   if (tid < 64)
-    wave_jacobi_eigh_n<n>(Mf, Qf);
+    wave_jacobi_eigh_n<n>(Mf, Qf, loose);
   else
-    wave_jacobi_eigh_n<n>(Mb, Qb);
+    wave_jacobi_eigh_n<n>(Mb, Qb, loose);
   __syncthreads();
   if (tid < n) {
     wf[tid] = Mf[tid * (n + 1)].re;
@@ -235,7 +238,7 @@ __global__ void __launch_bounds__(128)
   __syncthreads();
   cd* U = Mf;
   cd* V = Mb;
-  if (tid < 64) wave_jacobi_svd_n<n>(G, U, V, sigma, perm, scal);
+  if (tid < 64) wave_jacobi_svd_n<n>(G, U, V, sigma, perm, scal, loose);
   __syncthreads();
   double nrm = 0.0;
   for (int j = 0; j < n; ++j) nrm += sigma[j] * sigma[j];
@@ -764,7 +767,7 @@ __global__ void __launch_bounds__(256)
   for (int t = 0; t < 4; ++t) G[lane + 64 * t] = h[t];
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  const double nuc = wave_nuclear_norm_n<16>(G);
+  const double nuc = wave_nuclear_norm_n<16>(G, sizeof(RT) == 4);
   if (lane == 0) es_dist[es] = nuc;
 }
 

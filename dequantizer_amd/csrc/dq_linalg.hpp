@@ -190,9 +190,13 @@ __device__ inline void jacobi_eigh(cd* A, cd* V, int n, double* rot, int* flag) 
 // the same rotations and convergence rule as jacobi_eigh, with the round's
 // column and row updates separated by wave-level fences instead of block
 // barriers.  A -> eigenvalues on the diagonal; V columns = eigenvectors.
-// Call with all 64 lanes of one wave.
+// Call with all 64 lanes of one wave.  loose: stop after the first sweep
+// whose rotations all had |a_pq| below 1e-8 sqrt|a_pp a_qq|, leaving
+// off-diagonal terms of order 1e-16 (1e-8 at worst, for nearly equal
+// eigenvalues), below single-precision rounding; otherwise the sweeps run
+// until no pair rotates.
 template <int N>
-__device__ inline void wave_jacobi_eigh_n(cd* A, cd* V) {
+__device__ inline void wave_jacobi_eigh_n(cd* A, cd* V, bool loose = false) {
   constexpr int NP = N / 2;
   constexpr int LPP = 64 / NP;  // lanes per pair
   constexpr int RPL = N / LPP;  // rows (columns) per lane
@@ -206,16 +210,19 @@ __device__ inline void wave_jacobi_eigh_n(cd* A, cd* V) {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   for (int sweep = 0; sweep < 60; ++sweep) {
-    int rotated = 0;
+    int rotated = 0, large = 0;
     for (int r = 0; r < N - 1; ++r) {
       int p, q;
       rr_pair(r, k, N, p, q);
       double c = 1.0, s = 0.0, er = 1.0, ei = 0.0;
-      const bool rot = jacobi_params(A[p * N + p].re, A[q * N + q].re, A[p * N + q], c, s, er, ei, abs_tol);
+      const double app = A[p * N + p].re, aqq = A[q * N + q].re;
+      const cd apq = A[p * N + q];
+      const bool rot = jacobi_params_r(app, aqq, apq, c, s, er, ei, abs_tol);
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
       if (rot) {
         rotated = 1;
+        if (norm2(apq) > 1e-16 * fabs(app * aqq)) large = 1;
         const cd e = mk<double>(er, ei);
         // A <- A J and V <- V J (columns p, q)
 #pragma unroll
@@ -256,7 +263,7 @@ __device__ inline void wave_jacobi_eigh_n(cd* A, cd* V) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
-    if (!__any(rotated)) break;
+    if (!__any(loose ? large : rotated)) break;
   }
 }
 
@@ -472,8 +479,11 @@ __device__ inline void wave_jacobi_svd(cd* G, int m, int n, cd* U, cd* V, double
 // Fixed-size variant of wave_jacobi_svd for square N x N (N = 16 or 32):
 // loop bounds are compile-time constants, so each lane loads its rows of a
 // column pair in one burst, rotates them in registers and stores them back.
+// loose: stop after the first sweep whose rotations all had |g_pq| below
+// 1e-8 sqrt(|g_p|^2 |g_q|^2) (the rule of wave_jacobi_eigh_n).
 template <int N>
-__device__ inline void wave_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, int* perm, double* scal) {
+__device__ inline void wave_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, int* perm, double* scal,
+                                         bool loose = false) {
   constexpr int NP = N / 2;
   constexpr int LPP = 64 / NP;    // lanes per pair
   constexpr int RPL = N / LPP;    // rows per lane
@@ -487,7 +497,7 @@ __device__ inline void wave_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, int
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   for (int sweep = 0; sweep < 60; ++sweep) {
-    int rotated = 0;
+    int rotated = 0, large = 0;
     for (int r = 0; r < N - 1; ++r) {
       int p, q;
       rr_pair(r, k, N, p, q);
@@ -505,13 +515,11 @@ __device__ inline void wave_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, int
         gr += x[t].re * y[t].re + x[t].im * y[t].im;
         gi += x[t].re * y[t].im - x[t].im * y[t].re;
       }
-      al = group_sum<LPP>(al);
-      be = group_sum<LPP>(be);
-      gr = group_sum<LPP>(gr);
-      gi = group_sum<LPP>(gi);
+      group_sum4<LPP>(al, be, gr, gi);
       double c, s, er, ei;
-      if (jacobi_params(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
+      if (jacobi_params_r(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
         rotated = 1;
+        if (gr * gr + gi * gi > 1e-16 * al * be) large = 1;
         const cd e = mk<double>(er, ei);
         cd vx[RPL], vy[RPL];
 #pragma unroll
@@ -532,7 +540,7 @@ __device__ inline void wave_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, int
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
-    if (!__any(rotated)) break;
+    if (!__any(loose ? large : rotated)) break;
   }
   double nrm = 0.0;
   if (lane < N) {
@@ -572,9 +580,11 @@ __device__ inline void wave_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, int
 // Sum of the singular values (nuclear norm) of a square N x N matrix G
 // (column-major, overwritten) by the same wave-synchronous one-sided Jacobi
 // iteration, without accumulating singular vectors.  ONE wavefront; every
-// lane gets the result.
+// lane gets the result.  loose: the stopping rule of wave_jacobi_svd_n; the
+// column norms then differ from the singular values by terms of second order
+// in the remaining off-diagonal products.
 template <int N>
-__device__ inline double wave_nuclear_norm_n(cd* G) {
+__device__ inline double wave_nuclear_norm_n(cd* G, bool loose = false) {
   constexpr int NP = N / 2;
   constexpr int LPP = 64 / NP;
   constexpr int RPL = N / LPP;
@@ -587,7 +597,7 @@ __device__ inline double wave_nuclear_norm_n(cd* G) {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   for (int sweep = 0; sweep < 60; ++sweep) {
-    int rotated = 0;
+    int rotated = 0, large = 0;
     for (int r = 0; r < N - 1; ++r) {
       int p, q;
       rr_pair(r, k, N, p, q);
@@ -605,13 +615,11 @@ __device__ inline double wave_nuclear_norm_n(cd* G) {
         gr += x[t].re * y[t].re + x[t].im * y[t].im;
         gi += x[t].re * y[t].im - x[t].im * y[t].re;
       }
-      al = group_sum<LPP>(al);
-      be = group_sum<LPP>(be);
-      gr = group_sum<LPP>(gr);
-      gi = group_sum<LPP>(gi);
+      group_sum4<LPP>(al, be, gr, gi);
       double c, s, er, ei;
-      if (jacobi_params(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
+      if (jacobi_params_r(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
         rotated = 1;
+        if (gr * gr + gi * gi > 1e-16 * al * be) large = 1;
         const cd e = mk<double>(er, ei);
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
@@ -623,7 +631,7 @@ __device__ inline double wave_nuclear_norm_n(cd* G) {
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
-    if (!__any(rotated)) break;
+    if (!__any(loose ? large : rotated)) break;
   }
   double nrm = 0.0;
   if (lane < N) {
