@@ -126,18 +126,20 @@ __device__ __forceinline__ T wave_sum(T v) {
 
 // One DPP lane move of a double (two 32-bit moves).  Every source lane of
 // the controls used below lies in the destination's own 16-lane row, and the
-// callers run with the whole wave active.
+// callers run with the whole wave active.  bound_ctrl set: a source lane that
+// is disabled would read as 0, so the move needs no "old" operand and the
+// compiler writes a fresh register instead of first copying one.
 template <int kCtrl>
 __device__ __forceinline__ double dpp_move(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), kCtrl, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), kCtrl, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), kCtrl, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), kCtrl, 0xF, 0xF, true);
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
 template <int kCtrl>
 __device__ __forceinline__ int dpp_move_int(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xF, 0xF, false);
+  return __builtin_amdgcn_mov_dpp(v, kCtrl, 0xF, 0xF, true);
 }
 
 // Sum over aligned groups of L lanes (L = 1, 2, 4, 8, 16) by DPP moves inside
@@ -158,20 +160,10 @@ __device__ __forceinline__ double group_sum(double v) {
 
 // The same sum of four values at once, the DPP moves of each step issued
 // together (four independent chains instead of one chain four times as long).
-// Each move keeps the lane's own value as the "old" operand: every control
-// used reads a lane of the same row, so that operand is never taken.
-template <int kCtrl>
-__device__ __forceinline__ double dpp_move_self(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
-  const int mlo = __builtin_amdgcn_update_dpp(lo, lo, kCtrl, 0xF, 0xF, false);
-  const int mhi = __builtin_amdgcn_update_dpp(hi, hi, kCtrl, 0xF, 0xF, false);
-  return __longlong_as_double(((long long)mhi << 32) | (long long)(unsigned)mlo);
-}
 template <int kCtrl>
 __device__ __forceinline__ void group_step4(double& a, double& b, double& c, double& d) {
-  const double ma = dpp_move_self<kCtrl>(a), mb = dpp_move_self<kCtrl>(b);
-  const double mc = dpp_move_self<kCtrl>(c), md = dpp_move_self<kCtrl>(d);
+  const double ma = dpp_move<kCtrl>(a), mb = dpp_move<kCtrl>(b);
+  const double mc = dpp_move<kCtrl>(c), md = dpp_move<kCtrl>(d);
   a += ma;
   b += mb;
   c += mc;

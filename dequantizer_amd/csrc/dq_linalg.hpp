@@ -507,9 +507,13 @@ __device__ inline void wave_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, int
         x[t] = G[p * N + sub + t * LPP];
         y[t] = G[q * N + sub + t * LPP];
       }
-      double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+      // (the sums start from the first row's terms: adding them to 0.0
+      // would cost an instruction each, x + 0.0 not folding for x = -0.0)
+      double al = norm2(x[0]), be = norm2(y[0]);
+      double gr = x[0].re * y[0].re + x[0].im * y[0].im;
+      double gi = x[0].re * y[0].im - x[0].im * y[0].re;
 #pragma unroll
-      for (int t = 0; t < RPL; ++t) {
+      for (int t = 1; t < RPL; ++t) {
         al += norm2(x[t]);
         be += norm2(y[t]);
         gr += x[t].re * y[t].re + x[t].im * y[t].im;
@@ -607,9 +611,13 @@ __device__ inline double wave_nuclear_norm_n(cd* G, bool loose = false) {
         x[t] = G[p * N + sub + t * LPP];
         y[t] = G[q * N + sub + t * LPP];
       }
-      double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+      // (the sums start from the first row's terms: adding them to 0.0
+      // would cost an instruction each, x + 0.0 not folding for x = -0.0)
+      double al = norm2(x[0]), be = norm2(y[0]);
+      double gr = x[0].re * y[0].re + x[0].im * y[0].im;
+      double gi = x[0].re * y[0].im - x[0].im * y[0].re;
 #pragma unroll
-      for (int t = 0; t < RPL; ++t) {
+      for (int t = 1; t < RPL; ++t) {
         al += norm2(x[t]);
         be += norm2(y[t]);
         gr += x[t].re * y[t].re + x[t].im * y[t].im;
@@ -682,9 +690,13 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
           vy[t] = cld(V, q * N + sub + t * LPP);
         }
       }
-      double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
+      // (the sums start from the first row's terms: adding them to 0.0
+      // would cost an instruction each, x + 0.0 not folding for x = -0.0)
+      double al = norm2(x[0]), be = norm2(y[0]);
+      double gr = x[0].re * y[0].re + x[0].im * y[0].im;
+      double gi = x[0].re * y[0].im - x[0].im * y[0].re;
 #pragma unroll
-      for (int t = 0; t < RPL; ++t) {
+      for (int t = 1; t < RPL; ++t) {
         al += norm2(x[t]);
         be += norm2(y[t]);
         gr += x[t].re * y[t].re + x[t].im * y[t].im;
