@@ -657,8 +657,13 @@ __device__ inline double wave_nuclear_norm_n(cd* G, bool loose = false) {
 // kU = false: U is not written (may be null).  G is left holding the rotated
 // columns G J in their unsorted order (column p of G J goes with column p of
 // J = V; sigma and perm give the sorted order).
-template <int N, typename P, bool kV = true, bool kU = true>
+// kLoose: the sweeps stop after the first sweep whose rotations all had |g|
+// below 1e-4 sqrt(al be) instead of 1e-8: the off-diagonal terms left are of
+// order 1e-8 relative (quadratic convergence), below the rounding of
+// single-precision data, for factors that are rounded to single precision.
+template <int N, typename P, bool kV = true, bool kU = true, bool kLoose = false>
 __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int* perm, double* scal) {
+  constexpr double kLarge2 = kLoose ? 1e-8 : 1e-16;  // squared relative size of a "large" rotation
   constexpr int NP = N / 2;
   constexpr int LPP = 256 / NP;  // 16 lanes per pair for N = 32
   constexpr int RPL = N / LPP;   // rows per lane
@@ -672,7 +677,7 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
   const double abs_tol = 1e-32 * f;
   for (int sweep = 0; sweep < 60; ++sweep) {
     // rotated: any rotation this sweep; large: any with |g| above 1e-8 of
-    // sqrt(al be).  A sweep whose rotations were all below 1e-8 leaves
+    // sqrt(al be) (1e-4 with kLoose).  A sweep whose rotations were all below 1e-8 leaves
     // off-diagonal terms of order 1e-16 (quadratic convergence), under the
     // 1e-13 threshold of the rotation test: the sweep that would only
     // confirm convergence is skipped.
@@ -706,7 +711,7 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
       double c, s, er, ei;
       if (jacobi_params_r(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
         rotated = 1;
-        if (gr * gr + gi * gi > 1e-16 * al * be) large = 1;
+        if (gr * gr + gi * gi > kLarge2 * al * be) large = 1;
         const cd e = mk<double>(er, ei);
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
@@ -972,21 +977,25 @@ __device__ __attribute__((noinline)) int wave_pivoted_cholesky(P* G, int n, cd* 
 // wave_qrcp_rh) for N = 32 on the whole 256-thread block: thread (c, g) =
 // (tid / 8, tid % 8) keeps rows 4g .. 4g + 3 of column c in registers for the
 // whole factorisation, column norms and reflector products are sums over the 8
-// lanes of a column (DPP moves, no cross-wave traffic), and a step costs two
-// block barriers: one after every wave has posted its best remaining column
-// (and each column its row-k entry), one after the pivot column has posted its
-// reflector.  C (row-major) is only read.  Call with all 256 threads.
+// lanes of a column (DPP moves, no cross-wave traffic), and a step costs one
+// block barrier: before it every remaining column posts its rows to LDS and
+// every wave its best remaining column; after it every thread forms the rows
+// of the pivot's reflector it needs from the posted pivot column.  The posts
+// alternate between two buffers, C and X (a thread posting for step k + 1
+// has passed the barrier of step k, so every read of the buffer of step k - 1
+// is done).  C (row-major) is read first and then used as one of the two
+// buffers.  Call with all 256 threads.
 template <typename P, typename I>
-__device__ inline void block_qrcp_rh32(const P* C, P* Hk, P* X, I* piv) {
+__device__ inline void block_qrcp_rh32(P* C, P* Hk, P* X, I* piv) {
   constexpr int N = 32;
-  __shared__ double cand_v[4];
-  __shared__ int cand_j[4];
-  __shared__ cd xk_col[N];
+  __shared__ double cand_v[2][4];
+  __shared__ int cand_j[2][4];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int c = t >> 3, g = t & 7, r0 = 4 * g;
   cd x[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) x[q] = cld(C, (r0 + q) * N + c);
+  __syncthreads();  // C is read before it is posted to
   unsigned done = 0u;
   int mypos = N;
   auto take = [](double& best, int& j, double ob, int oj) {
@@ -996,18 +1005,15 @@ __device__ inline void block_qrcp_rh32(const P* C, P* Hk, P* X, I* piv) {
     }
   };
   for (int k = 0; k < N; ++k) {
+    P* pb = (k & 1) ? X : C;  // posted columns, column-major
     const bool mine = !((done >> c) & 1u);
     double part = 0.0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) part += r0 + q >= k ? norm2(x[q]) : 0.0;
     const double nrm2 = group_sum<8>(part);
-    // this column's row-k entry, for the pivot's alpha
-    if (g == (k >> 2)) {
-      cd v = x[0];
-      if ((k & 3) == 1) v = x[1];
-      if ((k & 3) == 2) v = x[2];
-      if ((k & 3) == 3) v = x[3];
-      cst(xk_col, c, v);
+    if (mine) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) cst(pb, c * N + r0 + q, x[q]);
     }
     // argmax over the wave's 8 columns: the two columns of each 16-lane row
     // meet by a row mirror, then the four rows by lane reads
@@ -1020,16 +1026,16 @@ __device__ inline void block_qrcp_rh32(const P* C, P* Hk, P* X, I* piv) {
 #pragma unroll
       for (int rw = 1; rw < 4; ++rw) take(bv, bj, readlane_d(best, 16 * rw), __builtin_amdgcn_readlane(j, 16 * rw));
       if (lane == 0) {
-        cand_v[w] = bv;
-        cand_j[w] = bj;
+        cand_v[k & 1][w] = bv;
+        cand_j[k & 1][w] = bj;
       }
     }
     __syncthreads();
-    best = cand_v[0];
-    j = cand_j[0];
+    best = cand_v[k & 1][0];
+    j = cand_j[k & 1][0];
 #pragma unroll
-    for (int q = 1; q < 4; ++q) take(best, j, cand_v[q], cand_j[q]);
-    const cd alpha = cld(xk_col, j);
+    for (int q = 1; q < 4; ++q) take(best, j, cand_v[k & 1][q], cand_j[k & 1][q]);
+    const cd alpha = cld(pb, j * N + k);
     // beta = -e^{i arg alpha} |x|, scale = 1 / sqrt(|x| (|x| + |alpha|)):
     // reciprocal square roots refined by a Newton step (about an ulp, as in
     // the Jacobi rotations) instead of correctly rounded square roots and
@@ -1051,22 +1057,26 @@ __device__ inline void block_qrcp_rh32(const P* C, P* Hk, P* X, I* piv) {
       }
       scale = rsqrt_nr(nrm * (nrm + aa));
     }
+    // this thread's rows of the reflector u = scale (x_j - beta e_k), zero above k
+    cd u[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = r0 + q;
+      const cd xj = cld(pb, j * N + r);
+      u[q] = r >= k ? scale * (r == k ? xj - beta : xj) : mk<double>(0.0, 0.0);
+    }
     if (c == j) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int r = r0 + q;
-        cst(Hk, k * N + r, r >= k ? scale * (r == k ? x[q] - beta : x[q]) : mk<double>(0.0, 0.0));
+        cst(Hk, k * N + r, u[q]);
         x[q] = r < k ? x[q] : (r == k ? beta : mk<double>(0.0, 0.0));
       }
       mypos = k;
-    }
-    __syncthreads();
-    if (mine && c != j && live) {
-      cd u[4];
+    } else if (mine && live) {
       double dr = 0.0, di = 0.0;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        u[q] = cld(Hk, k * N + r0 + q);
         dr += u[q].re * x[q].re + u[q].im * x[q].im;
         di += u[q].re * x[q].im - u[q].im * x[q].re;
       }
@@ -1078,6 +1088,7 @@ __device__ inline void block_qrcp_rh32(const P* C, P* Hk, P* X, I* piv) {
     done |= 1u << j;
     if (t == 0) piv[k] = j;
   }
+  __syncthreads();  // every read of the last posted buffer (possibly X) is done
   // X = R^H: column r of X holds conj(R[r][m]) at row m, R[r][m] = row r of
   // column piv[m], zero below the diagonal of R
 #pragma unroll

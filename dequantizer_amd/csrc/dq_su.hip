@@ -497,11 +497,16 @@ __device__ void su_write_gamma(const DevGraph& g, cx<R>* t, int node, int slot, 
 // and V with perm, perm here being the identity.  Buffers (C, U, V in LDS):
 // U holds the reflectors until the end, X and V_x live in V and C, U_x in W
 // (any memory).
+// single (factors that end in single precision): the sweeps stop at the
+// single-precision rule of block_jacobi_svd_n (kLoose).
 __device__ __attribute__((noinline)) void su_core_svd_qr32(lds_cd* C, lds_cd* U, lds_cd* V, cd* W, double* sigma,
-                                                             int* perm, double* scal) {
+                                                             int* perm, double* scal, bool single) {
   __shared__ int qpiv[32];
   block_qrcp_rh32(C, U, V, (lds_int*)qpiv);
-  block_jacobi_svd_n<32>(V, W, C, sigma, perm, scal);
+  if (single)
+    block_jacobi_svd_n<32, lds_cd, true, true, true>(V, W, C, sigma, perm, scal);
+  else
+    block_jacobi_svd_n<32>(V, W, C, sigma, perm, scal);
   // U_C[:, j] = Q V_x[:, perm[j]], Q = H_0 ... H_31 applied from H_31 down:
   // 8 lanes per column, rows sub + 8 t
   const int j = threadIdx.x >> 3, sub = threadIdx.x & 7;
@@ -739,7 +744,7 @@ __global__ void __launch_bounds__(256, kMem == kSuCompact ? 2 : 1)
   }
   __syncthreads();
   if (n == 32 && blockDim.x == 256 && !tape && !kGlob) {
-    su_core_svd_qr32((lds_cd*)C, (lds_cd*)U, (lds_cd*)Vs, chunk, sigma, perm, rot);
+    su_core_svd_qr32((lds_cd*)C, (lds_cd*)U, (lds_cd*)Vs, chunk, sigma, perm, rot, sizeof(R) == 4);
   } else if (n == 32 && blockDim.x == 256) {
     // the adjoint's record keeps the unpreconditioned factors it was built on
     if (!kGlob)
