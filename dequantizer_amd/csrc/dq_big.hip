@@ -345,6 +345,135 @@ __global__ void __launch_bounds__(kBigThreads)
   for (int q = tid; q < CC; q += blockDim.x) pp[q] = red[q] + red[CC + q] + red[2 * CC + q] + red[3 * CC + q];
 }
 
+// Two products of a node update in one pass, chi = 16, complex64 data:
+//   dst = src x_a M_a x_b M_b   (a != b),
+// over the tiles of k_big_prod_gram16: the 16 x 16 (x_a, x_b) slices S of src
+// for W = 8 consecutive c' (the remaining indices, contiguous ones fastest),
+// wave w taking slices 2w and 2w + 1.  Per slice on FP32 matrix cores (as the
+// separate products' FP32 FMAs): U1 = S M_b^T with A = S (lane l: S[l % 16]
+// [4 ks + l / 16]) and B = M_b^T (lane l: M_b[l % 16][4 ks + l / 16]); the
+// result registers hold U1 rows 4 (l / 16) + v, column l % 16, which are the
+// B operands of U = M_a U1 when the k-steps run over those rows (step v:
+// A = M_a[l % 16][4 (l / 16) + v]).  U goes back over the slice in LDS and
+// out along memory: one HBM pass where two separate products take two.
+template <typename R>
+__global__ void __launch_bounds__(kBigThreads)
+    k_big_prod2_16(DevGraph g, const int* nodes, int ax_a, int ax_b, int src_id, int dst_id, int nb, const cx<R>* T,
+                   const cx<R>* n2e_old, const cx<R>* e2n_old, const cx<R>* n2e_new, const uint8_t* node_src,
+                   cx<R>* buf, long long smax, const int* inst_active) {
+  constexpr int CHI = 16, CC = CHI * CHI, W = 8;
+  __shared__ cx<R> St[W][CHI][CHI + 1];  // [slice][x_a][x_b]
+  const int k = blockIdx.y;
+  const int node = nodes[k];
+  if (inst_active && !inst_active[g.node_inst[node]]) return;
+  const int d = g.deg[node];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, mi = lane & 15, kq = lane >> 4;
+  const long long S = ipow(CHI, d) * g.phys;
+  const int a_hi = ax_a < ax_b ? ax_a : ax_b, a_lo = ax_a < ax_b ? ax_b : ax_a;
+  const long long n_bot = ipow(CHI, d - 1 - a_lo) * g.phys;  // contiguous indices after both axes
+  const long long n_mid = ipow(CHI, a_lo - a_hi - 1);
+  const long long st_lo = n_bot, st_mid = CHI * n_bot, st_hi = n_mid * CHI * n_bot, st_top = CHI * st_hi;
+  const long long ncp = S / CC, ntiles = ncp / W;
+  const cx<R>* src = src_id < 0 ? T + g.toff[node] : buf + ((long long)nb * k + src_id) * smax;
+  cx<R>* dst = buf + ((long long)nb * k + dst_id) * smax;
+  // the incoming messages of both axes (ket side, as k_big_plan_product):
+  // mb[ks] = M_b[l % 16][4 ks + l / 16], ma[v] = M_a[l % 16][4 (l / 16) + v]
+  cd ma[4], mb[4];
+  {
+    const int base = g.adj_ptr[node];
+    auto msg = [&](int axis) {
+      const int e = g.adj_edge[base + axis], sd = g.adj_side[base + axis];
+      const int code = node_src ? node_src[base + axis] : 0;
+      return code == 0 ? e2n_old + (2LL * e + sd) * CC : (code == 1 ? n2e_new : n2e_old) + (2LL * e + 1 - sd) * CC;
+    };
+    const cx<R>* m_a = msg(ax_a);
+    const cx<R>* m_b = msg(ax_b);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      mb[v] = cvt<double>(m_b[mi * CHI + 4 * v + kq]);
+      ma[v] = cvt<double>(m_a[mi * CHI + 4 * kq + v]);
+    }
+  }
+  // each thread's PER elements of a tile: slice q % W, (x_hi, x_lo) from q / W
+  // (as k_big_prod_gram16; 32-bit offsets: the host fuses only tensors below
+  // 2^31 elements)
+  constexpr int PER = W * CC / kBigThreads;
+  cx<R> np[PER];
+  unsigned offc[PER];
+  int wwv[PER], xav[PER], xbv[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int q = tid + u * kBigThreads;
+    wwv[u] = q % W;
+    const int rest = q / W, xl = rest % CHI, xh = rest / CHI;
+    offc[u] = (unsigned)(xh * st_hi + xl * st_lo);
+    xav[u] = ax_a == a_hi ? xh : xl;
+    xbv[u] = ax_b == a_hi ? xh : xl;
+  }
+  const unsigned nbot = (unsigned)n_bot, nmid = (unsigned)n_mid, smid = (unsigned)st_mid, stop = (unsigned)st_top;
+  const bool whole = nbot % W == 0;
+  auto tile_off = [&](long long tile, int u) -> unsigned {
+    const unsigned cp = (unsigned)(tile * W) + (whole ? 0u : (unsigned)wwv[u]);
+    const unsigned t2 = cp / nbot;
+    return (t2 / nmid) * stop + (t2 % nmid) * smid + cp % nbot + offc[u] + (whole ? (unsigned)wwv[u] : 0u);
+  };
+  if (blockIdx.x < ntiles) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) np[u] = src[tile_off(blockIdx.x, u)];
+  }
+  const int s0 = 2 * w, s1 = 2 * w + 1;
+  for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) St[wwv[u]][xav[u]][xbv[u]] = np[u];
+    __syncthreads();
+    if (tile + gridDim.x < ntiles) {
+#pragma unroll
+      for (int u = 0; u < PER; ++u) np[u] = src[tile_off(tile + gridDim.x, u)];
+    }
+    // U1 = S M_b^T
+    big_f4 ur0 = {0.f, 0.f, 0.f, 0.f}, ui0 = ur0, ur1 = ur0, ui1 = ur0;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const cx<R> p0 = St[s0][mi][4 * ks + kq], p1 = St[s1][mi][4 * ks + kq];
+      const float br = (float)mb[ks].re, bi = (float)mb[ks].im;
+      const float p0r = (float)p0.re, p0i = (float)p0.im, p1r = (float)p1.re, p1i = (float)p1.im;
+      ur0 = __builtin_amdgcn_mfma_f32_16x16x4f32(p0r, br, ur0, 0, 0, 0);
+      ur1 = __builtin_amdgcn_mfma_f32_16x16x4f32(p1r, br, ur1, 0, 0, 0);
+      ui0 = __builtin_amdgcn_mfma_f32_16x16x4f32(p0r, bi, ui0, 0, 0, 0);
+      ui1 = __builtin_amdgcn_mfma_f32_16x16x4f32(p1r, bi, ui1, 0, 0, 0);
+      ur0 = __builtin_amdgcn_mfma_f32_16x16x4f32(-p0i, bi, ur0, 0, 0, 0);
+      ur1 = __builtin_amdgcn_mfma_f32_16x16x4f32(-p1i, bi, ur1, 0, 0, 0);
+      ui0 = __builtin_amdgcn_mfma_f32_16x16x4f32(p0i, br, ui0, 0, 0, 0);
+      ui1 = __builtin_amdgcn_mfma_f32_16x16x4f32(p1i, br, ui1, 0, 0, 0);
+    }
+    // U = M_a U1, k-steps over the U1 rows 4 (l / 16) + v each lane holds
+    big_f4 vr0 = {0.f, 0.f, 0.f, 0.f}, vi0 = vr0, vr1 = vr0, vi1 = vr0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const float ar = (float)ma[v].re, ai = (float)ma[v].im;
+      vr0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, ur0[v], vr0, 0, 0, 0);
+      vr1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, ur1[v], vr1, 0, 0, 0);
+      vi0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, ui0[v], vi0, 0, 0, 0);
+      vi1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, ui1[v], vi1, 0, 0, 0);
+      vr0 = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, ui0[v], vr0, 0, 0, 0);
+      vr1 = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, ui1[v], vr1, 0, 0, 0);
+      vi0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, ur0[v], vi0, 0, 0, 0);
+      vi1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, ur1[v], vi1, 0, 0, 0);
+    }
+    // over this wave's own slices (every read of them above is complete: the
+    // products depend on it), then out along memory once every slice is written
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      St[s0][4 * kq + v][mi] = mk<R>((R)vr0[v], (R)vi0[v]);
+      St[s1][4 * kq + v][mi] = mk<R>((R)vr1[v], (R)vi1[v]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u) dst[tile_off(tile, u)] = St[wwv[u]][xav[u]][xbv[u]];
+    __syncthreads();
+  }
+}
+
 template <typename R, int CHI>
 __global__ void __launch_bounds__(kBigThreads)
     k_big_gram_reduce(DevGraph g, const int* nodes, int idx, const cd* part, int chunks, cx<R>* n2e_new,
@@ -527,6 +656,47 @@ static hipError_t big_bp_level(const DevGraph& g, int min_deg, int max_deg, cons
   // chi 16, every node of the plan's degree: a product whose result only
   // feeds the next Gram runs fused with it
   const bool fuse = sizeof(R) == 4 && CHI == 16 && min_deg == max_deg && max_deg >= 4 && smax < (1LL << 31);
+  if (fuse && (max_deg == 6 || max_deg == 4)) {
+    // The messages in pairs {0,1}, {2,3}, {4,5}: slots a, b of a pair take
+    // Q = T x (every other pair), then their partner's message fused with the
+    // Gram.  Degree 6: Q_{01} = (T x23) x45, Q_{23} = (T x01) x45,
+    // Q_{45} = (T x01) x23 -- five two-product passes and six product-Gram
+    // passes (22 tensor reads and writes per node against 32 for the plan
+    // above, same 16 products); degree 4: Q_{01} = T x23, Q_{23} = T x01.
+    auto prod2 = [&](int a, int b, int src, int dst) {
+      hipLaunchKernelGGL((k_big_prod2_16<R>), dim3(chunks, count), dim3(kBigThreads), 0, s, g, nodes, a, b, src, dst,
+                         nb, (const cx<R>*)T, (const cx<R>*)n2e_old, (const cx<R>*)e2n_old, (const cx<R>*)n2e_new,
+                         node_src, buf, smax, inst_active);
+    };
+    auto pgram = [&](int j, int idx, int src) {
+      hipLaunchKernelGGL((k_big_prod_gram16<R>), dim3(chunks, count), dim3(kBigThreads), 0, s, g, nodes, j, idx, src,
+                         nb, (const cx<R>*)T, (const cx<R>*)n2e_old, (const cx<R>*)e2n_old, (const cx<R>*)n2e_new,
+                         node_src, (const cx<R>*)buf, smax, part, inst_active);
+      hipLaunchKernelGGL((k_big_gram_reduce<R, CHI>), dim3(count), dim3(kBigThreads), 0, s, g, nodes, idx,
+                         (const cd*)part, chunks, (cx<R>*)n2e_new, inst_active);
+    };
+    if (max_deg == 6) {
+      prod2(0, 1, -1, 0);  // buffer 0: T x01
+      prod2(2, 3, -1, 1);  // buffer 1: T x23
+      prod2(4, 5, 1, 2);   // buffer 2: T x2345
+      pgram(1, 0, 2);
+      pgram(0, 1, 2);
+      prod2(4, 5, 0, 1);  // buffer 1: T x0145
+      pgram(3, 2, 1);
+      pgram(2, 3, 1);
+      prod2(2, 3, 0, 2);  // buffer 2: T x0123
+      pgram(5, 4, 2);
+      pgram(4, 5, 2);
+    } else {
+      prod2(2, 3, -1, 0);  // T x23
+      pgram(1, 0, 0);
+      pgram(0, 1, 0);
+      prod2(0, 1, -1, 0);  // T x01
+      pgram(3, 2, 0);
+      pgram(2, 3, 0);
+    }
+    return hipGetLastError();
+  }
   for (size_t i = 0; i < plan.ops.size(); ++i) {
     const BigOp& op = plan.ops[i];
     if (fuse && !op.gram && i + 1 < plan.ops.size() && plan.ops[i + 1].gram && plan.ops[i + 1].src == op.dst) {
