@@ -191,11 +191,10 @@ __device__ inline void jacobi_eigh(cd* A, cd* V, int n, double* rot, int* flag) 
 // column and row updates separated by wave-level fences instead of block
 // barriers.  A -> eigenvalues on the diagonal; V columns = eigenvectors.
 // Call with all 64 lanes of one wave.  loose: stop after the first sweep
-// whose rotations all had |a_pq| below 1e-4 sqrt|a_pp a_qq|, leaving
-// off-diagonal terms of order 1e-8 relative (quadratic convergence; each
-// later rotation of a sweep moves a zeroed pair by the product of two such
-// terms), below single-precision rounding; otherwise the sweeps run until no
-// pair rotates.
+// whose rotations all had |a_pq| below 1e-8 sqrt|a_pp a_qq|, leaving
+// off-diagonal terms of order 1e-16 (1e-8 at worst, for nearly equal
+// eigenvalues), below single-precision rounding; otherwise the sweeps run
+// until no pair rotates.
 template <int N>
 __device__ inline void wave_jacobi_eigh_n(cd* A, cd* V, bool loose = false) {
   constexpr int NP = N / 2;
@@ -223,7 +222,7 @@ __device__ inline void wave_jacobi_eigh_n(cd* A, cd* V, bool loose = false) {
       __builtin_amdgcn_wave_barrier();
       if (rot) {
         rotated = 1;
-        if (norm2(apq) > 1e-8 * fabs(app * aqq)) large = 1;
+        if (norm2(apq) > 1e-16 * fabs(app * aqq)) large = 1;
         const cd e = mk<double>(er, ei);
         // A <- A J and V <- V J (columns p, q)
 #pragma unroll
@@ -481,7 +480,7 @@ __device__ inline void wave_jacobi_svd(cd* G, int m, int n, cd* U, cd* V, double
 // loop bounds are compile-time constants, so each lane loads its rows of a
 // column pair in one burst, rotates them in registers and stores them back.
 // loose: stop after the first sweep whose rotations all had |g_pq| below
-// 1e-4 sqrt(|g_p|^2 |g_q|^2) (the rule of wave_jacobi_eigh_n).
+// 1e-8 sqrt(|g_p|^2 |g_q|^2) (the rule of wave_jacobi_eigh_n).
 template <int N>
 __device__ inline void wave_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, int* perm, double* scal,
                                          bool loose = false) {
@@ -524,7 +523,7 @@ __device__ inline void wave_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, int
       double c, s, er, ei;
       if (jacobi_params_r(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
         rotated = 1;
-        if (gr * gr + gi * gi > 1e-8 * al * be) large = 1;
+        if (gr * gr + gi * gi > 1e-16 * al * be) large = 1;
         const cd e = mk<double>(er, ei);
         cd vx[RPL], vy[RPL];
 #pragma unroll
@@ -628,7 +627,7 @@ __device__ inline double wave_nuclear_norm_n(cd* G, bool loose = false) {
       double c, s, er, ei;
       if (jacobi_params_r(al, be, mk<double>(gr, gi), c, s, er, ei, abs_tol)) {
         rotated = 1;
-        if (gr * gr + gi * gi > 1e-8 * al * be) large = 1;
+        if (gr * gr + gi * gi > 1e-16 * al * be) large = 1;
         const cd e = mk<double>(er, ei);
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
