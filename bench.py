@@ -34,9 +34,9 @@ PEAK_HBM_GBS = 8000.0
 TRAFFIC_BP_SWEEP_DEFAULT = 2 * 38304.5 * 1024 + 12000.0 * 1024
 # HBM bytes of one simple-update launch (500 gates, one edge-colour class) of
 # the default workload, from the same counter passes over a whole run (median
-# dispatch; the 256 MiB copy reads 131092.9 KiB by FETCH_SIZE and writes
-# 262144 KiB by WRITE_SIZE): 90680 KiB fetched (x 2) + 121300 KiB written.
-TRAFFIC_SU_LAUNCH_DEFAULT = 309879627
+# dispatch; the 256 MiB copy reads 131089 KiB by FETCH_SIZE and writes
+# 262144 KiB by WRITE_SIZE): 84093.6 KiB fetched (x 2) + 112942 KiB written.
+TRAFFIC_SU_LAUNCH_DEFAULT = int((2 * 84093.6 + 112942.0) * 1024)
 
 
 def eu_flops(d: int, chi: int, p: int) -> int:
@@ -153,8 +153,10 @@ def main():
         ef = edge_colored(ef)
     schedule = get_scheduler(args.dt, linear_schedule, args.schedule_steps)
     t_build = time.perf_counter()
+    tuning = ""  # the autotuner runs once per process, in the first engine
     for k in range(args.warmup):
-        annealing_run(ef, schedule, args, torch.complex64)
+        ann_w, _ = annealing_run(ef, schedule, args, torch.complex64)
+        tuning = tuning or ann_w.emu.tuning_report
         torch.cuda.synchronize()
         print(f"bench: warmup run {k} done", file=sys.stderr, flush=True)
     t_build = time.perf_counter() - t_build
@@ -274,7 +276,7 @@ def main():
         "energy_per_spin_bp": res.bp_energy / ef.nodes_number,
         "magnetisation": float(z.mean()),
         "max_truncation_error": float(np.max(res.truncation_errors)),
-        "autotune": [ln for ln in emu.tuning_report.strip().split("\n") if ln],
+        "autotune": [ln for ln in (tuning or runs[0][0].emu.tuning_report).strip().split("\n") if ln],
         "setup_and_warmup_s": t_build,
         "roofline": dominant,
         "roofline_other": secondary,
