@@ -498,15 +498,16 @@ __device__ void su_write_gamma(const DevGraph& g, cx<R>* t, int node, int slot, 
 // U holds the reflectors until the end, X and V_x live in V and C, U_x in W
 // (any memory).
 // single (factors that end in single precision): the sweeps stop at the
-// single-precision rule of block_jacobi_svd_n (kLoose).
+// single-precision rule of block_jacobi_svd_n (kLoose).  The rounds follow
+// the block-cyclic ordering (kBlockOrder).
 __device__ __attribute__((noinline)) void su_core_svd_qr32(lds_cd* C, lds_cd* U, lds_cd* V, cd* W, double* sigma,
                                                              int* perm, double* scal, bool single) {
   __shared__ int qpiv[32];
   block_qrcp_rh32(C, U, V, (lds_int*)qpiv);
   if (single)
-    block_jacobi_svd_n<32, lds_cd, true, true, true>(V, W, C, sigma, perm, scal);
+    block_jacobi_svd_n<32, lds_cd, true, true, true, true>(V, W, C, sigma, perm, scal);
   else
-    block_jacobi_svd_n<32>(V, W, C, sigma, perm, scal);
+    block_jacobi_svd_n<32, lds_cd, true, true, false, true>(V, W, C, sigma, perm, scal);
   // U_C[:, j] = Q V_x[:, perm[j]], Q = H_0 ... H_31 applied from H_31 down:
   // 8 lanes per column, rows sub + 8 t
   const int j = threadIdx.x >> 3, sub = threadIdx.x & 7;
