@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(256)
     cd* M = side == 0 ? Mf : Mb;
     cd* Q = side == 0 ? Qf : Qb;
     double* w = side == 0 ? wf : wb;
-    block_jacobi_svd_n<32, lds_cd, true, false>((lds_cd*)M, nullptr, (lds_cd*)Q, sigma, perm, scal);
+    block_jacobi_svd_n<32, lds_cd, true, false, false, true>((lds_cd*)M, nullptr, (lds_cd*)Q, sigma, perm, scal);
     if (tid < n) {
       double re = 0.0, s2 = 0.0;
 #pragma unroll 8
@@ -424,7 +424,7 @@ __global__ void __launch_bounds__(256)
     Mf[q] = acc;
   }
   __syncthreads();
-  block_jacobi_svd_n<32>((lds_cd*)Mf, U, (lds_cd*)Mb, sigma, perm, scal);
+  block_jacobi_svd_n<32, lds_cd, true, true, false, true>((lds_cd*)Mf, U, (lds_cd*)Mb, sigma, perm, scal);
   cd* V = Mb;
   double nrm = 0.0;
   for (int j = 0; j < n; ++j) nrm += sigma[j] * sigma[j];
@@ -817,7 +817,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int t = 0; t < 4; ++t) G[threadIdx.x + 256 * t] = h[t];
   __syncthreads();
-  block_jacobi_svd_n<32, lds_cd, false, false>((lds_cd*)G, nullptr, nullptr, sg, perm, scal);
+  block_jacobi_svd_n<32, lds_cd, false, false, false, true>((lds_cd*)G, nullptr, nullptr, sg, perm, scal);
   if (threadIdx.x == 0) {
     double nuc = 0.0;
     for (int k = 0; k < 32; ++k) nuc += sg[k];

@@ -749,7 +749,7 @@ __global__ void __launch_bounds__(256, kMem == kSuCompact ? 2 : 1)
   } else if (n == 32 && blockDim.x == 256) {
     // the adjoint's record keeps the unpreconditioned factors it was built on
     if (!kGlob)
-      block_jacobi_svd_n<32>((lds_cd*)C, U, (lds_cd*)Vs, sigma, perm, rot);
+      block_jacobi_svd_n<32, lds_cd, true, true, false, true>((lds_cd*)C, U, (lds_cd*)Vs, sigma, perm, rot);
     else
       block_jacobi_svd_n<32>(C, U, Vs, sigma, perm, rot);
   } else if (n == 16) {
