@@ -661,17 +661,19 @@ __device__ inline double wave_nuclear_norm_n(cd* G, bool loose = false) {
 // below 1e-4 sqrt(al be) instead of 1e-8: the off-diagonal terms left are of
 // order 1e-8 relative (quadratic convergence), below the rounding of
 // single-precision data, for factors that are rounded to single precision.
-// kBlockOrder (N = 32): a block-cyclic ordering of the same 496 pairs per
-// sweep.  The columns form 8 blocks of 4; in outer round o (of 7) wave w owns
-// the block pair (P, Q) = rr_pair(o, w, 8) and takes its 16 cross pairs in 4
-// rounds (slot s of the wave pairs column 4P + s with 4Q + (s ^ t)), after
-// the 12 pairs inside P and inside Q in 3 rounds at o = 0.  A wave only
-// touches the 8 columns it owns within an outer round, so its rounds there are
-// separated by wave-level fences, and block barriers fall between outer
-// rounds: 7 per sweep instead of 31.
+// kBlockOrder (N = 32 or 64): a block-cyclic ordering of the same N (N - 1) / 2
+// pairs per sweep.  The columns form 8 blocks of S = N / 8 (the pairs a wave
+// takes per round); in outer round o (of 7) wave w owns the block pair
+// (P, Q) = rr_pair(o, w, 8) and takes its S^2 cross pairs in S rounds (slot s
+// of the wave pairs column S P + s with S Q + (s ^ t)), after the pairs inside
+// P and inside Q in S - 1 rounds at o = 0 (round m pairs a with a ^ m, the
+// first S / 2 slots in P, the others in Q).  A wave only touches the 2S
+// columns it owns within an outer round, so its rounds there are separated
+// by wave-level fences, and block barriers fall between outer rounds: 7 per
+// sweep instead of N - 1.
 template <int N, typename P, bool kV = true, bool kU = true, bool kLoose = false, bool kBlockOrder = false>
 __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int* perm, double* scal) {
-  static_assert(!kBlockOrder || N == 32, "the block ordering is sized for N = 32");
+  static_assert(!kBlockOrder || N == 32 || N == 64, "the block ordering is sized for N = 32 or 64");
   constexpr double kLarge2 = kLoose ? 1e-8 : 1e-16;  // squared relative size of a "large" rotation
   constexpr int NP = N / 2;
   constexpr int LPP = 256 / NP;  // 16 lanes per pair for N = 32
@@ -695,21 +697,25 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
       int p, q;
       bool block_sync = true;  // uniform over the block: depends on r only
       if (kBlockOrder) {
-        const int o = r < 7 ? 0 : 1 + (r - 7) / 4, i = r < 7 ? r : (r - 7) % 4;
-        const int s4 = k & 3;
+        constexpr int S = N / 8;  // slots (pairs) per wave and round
+        const int o = r < 2 * S - 1 ? 0 : 1 + (r - (2 * S - 1)) / S;
+        const int i = r < 2 * S - 1 ? r : (r - (2 * S - 1)) % S;
+        const int sl = k % S;
         int bp, bq;
-        rr_pair(o, k >> 2, 8, bp, bq);
-        if (o == 0 && i < 3) {
-          // pairs inside a block: (0,1)(2,3), (0,2)(1,3), (0,3)(1,2)
-          const int m = i + 1, blk = s4 < 2 ? bp : bq, j = s4 & 1;
-          const int a = m == 1 ? 2 * j : j;
-          p = 4 * blk + a;
-          q = 4 * blk + (a ^ m);
+        rr_pair(o, k / S, 8, bp, bq);
+        if (o == 0 && i < S - 1) {
+          // pairs inside a block: round m pairs a with a ^ m; slot j of a
+          // block takes the j-th a without the top bit of m
+          const int m = i + 1, blk = sl < S / 2 ? bp : bq, j = sl % (S / 2);
+          const int h = 31 - __builtin_clz(m);
+          const int a = ((j >> h) << (h + 1)) | (j & ((1 << h) - 1));
+          p = S * blk + a;
+          q = S * blk + (a ^ m);
         } else {
-          p = 4 * bp + s4;
-          q = 4 * bq + (s4 ^ (o == 0 ? i - 3 : i));
+          p = S * bp + sl;
+          q = S * bq + (sl ^ (o == 0 ? i - (S - 1) : i));
         }
-        block_sync = o == 0 ? i == 6 : i == 3;
+        block_sync = o == 0 ? i == 2 * S - 2 : i == S - 1;
       } else {
         rr_pair(r, k, N, p, q);
       }

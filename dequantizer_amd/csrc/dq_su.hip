@@ -553,7 +553,7 @@ __device__ __attribute__((noinline)) void su_core_svd64(const cd* C, cd* U, cd* 
   __shared__ cd G64[64 * 64], V64[64 * 64];
   for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) cst((lds_cd*)G64, i, C[i]);
   __syncthreads();
-  block_jacobi_svd_n<64>((lds_cd*)G64, U, (lds_cd*)V64, sigma, perm, scal);
+  block_jacobi_svd_n<64, lds_cd, true, true, false, true>((lds_cd*)G64, U, (lds_cd*)V64, sigma, perm, scal);
   for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) Vs[i] = cld((lds_cd*)V64, i);
   __syncthreads();
 }
