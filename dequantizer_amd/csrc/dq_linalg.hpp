@@ -1037,11 +1037,11 @@ __device__ inline void block_qrcp_rh32(P* C, P* Hk, P* X, I* piv) {
   __syncthreads();  // C is read before it is posted to
   unsigned done = 0u;
   int mypos = N;
+  // (largest norm, ties to the smaller index) as selects, without branches
   auto take = [](double& best, int& j, double ob, int oj) {
-    if (ob > best || (ob == best && oj < j)) {
-      best = ob;
-      j = oj;
-    }
+    const bool better = (ob > best) | ((ob == best) & (oj < j));  // (no short-circuit branches)
+    best = better ? ob : best;
+    j = better ? oj : j;
   };
   for (int k = 0; k < N; ++k) {
     P* pb = (k & 1) ? X : C;  // posted columns, column-major
@@ -1055,25 +1055,27 @@ __device__ inline void block_qrcp_rh32(P* C, P* Hk, P* X, I* piv) {
       for (int q = 0; q < 4; ++q) cst(pb, c * N + r0 + q, x[q]);
     }
     // argmax over the wave's 8 columns: the two columns of each 16-lane row
-    // meet by a row mirror, then the four rows by lane reads
+    // meet by a row mirror, then the rows by lane shuffles (xor 16, xor 32)
     double best = mine ? nrm2 : -1.0;
     int j = mine ? c : N;
     take(best, j, dpp_move<0x140>(best), dpp_move_int<0x140>(j));
-    {
-      double bv = readlane_d(best, 0);
-      int bj = __builtin_amdgcn_readlane(j, 0);
-#pragma unroll
-      for (int rw = 1; rw < 4; ++rw) take(bv, bj, readlane_d(best, 16 * rw), __builtin_amdgcn_readlane(j, 16 * rw));
-      if (lane == 0) {
-        cand_v[k & 1][w] = bv;
-        cand_j[k & 1][w] = bj;
-      }
+    take(best, j, __shfl_xor(best, 16, 64), __shfl_xor(j, 16, 64));
+    take(best, j, __shfl_xor(best, 32, 64), __shfl_xor(j, 32, 64));
+    if (lane == 0) {
+      cand_v[k & 1][w] = best;
+      cand_j[k & 1][w] = j;
     }
     __syncthreads();
-    best = cand_v[k & 1][0];
-    j = cand_j[k & 1][0];
-#pragma unroll
-    for (int q = 1; q < 4; ++q) take(best, j, cand_v[k & 1][q], cand_j[k & 1][q]);
+    {
+      // the four waves' candidates, loaded together and reduced as a tree
+      double b0 = cand_v[k & 1][0], b1 = cand_v[k & 1][1], b2 = cand_v[k & 1][2], b3 = cand_v[k & 1][3];
+      int j0 = cand_j[k & 1][0], j1 = cand_j[k & 1][1], j2 = cand_j[k & 1][2], j3 = cand_j[k & 1][3];
+      take(b0, j0, b1, j1);
+      take(b2, j2, b3, j3);
+      take(b0, j0, b2, j2);
+      best = b0;
+      j = j0;
+    }
     const cd alpha = cld(pb, j * N + k);
     // beta = -e^{i arg alpha} |x|, scale = 1 / sqrt(|x| (|x| + |alpha|)):
     // reciprocal square roots refined by a Newton step (about an ulp, as in
