@@ -650,6 +650,30 @@ __device__ inline double wave_nuclear_norm_n(cd* G, bool loose = false) {
   return wave_sum(nrm);
 }
 
+// Pair (p, q), p < q, of slot k in round r of the block-cyclic ordering of
+// block_jacobi_svd_n (kBlockOrder) for N = 32 or 64.
+template <int N>
+__device__ __forceinline__ void block_order_pair(int r, int k, int& p, int& q) {
+  constexpr int S = N / 8;  // slots (pairs) per wave and round
+  const int o = r < 2 * S - 1 ? 0 : 1 + (r - (2 * S - 1)) / S;
+  const int i = r < 2 * S - 1 ? r : (r - (2 * S - 1)) % S;
+  const int sl = k % S;
+  int bp, bq;
+  rr_pair(o, k / S, 8, bp, bq);
+  if (o == 0 && i < S - 1) {
+    // pairs inside a block: round m pairs a with a ^ m; slot j of a block
+    // takes the j-th a without the top bit of m
+    const int m = i + 1, blk = sl < S / 2 ? bp : bq, j = sl % (S / 2);
+    const int h = 31 - __builtin_clz(m);
+    const int a = ((j >> h) << (h + 1)) | (j & ((1 << h) - 1));
+    p = S * blk + a;
+    q = S * blk + (a ^ m);
+  } else {
+    p = S * bp + sl;
+    q = S * bq + (sl ^ (o == 0 ? i - (S - 1) : i));
+  }
+}
+
 // Block-parallel fixed-size one-sided Jacobi SVD (N = 32, 256 threads): the
 // 16 pairs of a round run on 16-lane groups (4 per wave), 2 rows per lane, one
 // block barrier per round.  Same outputs as jacobi_svd.  scal: N doubles.
@@ -686,6 +710,18 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
   for (int i = threadIdx.x; i < N * N; i += blockDim.x) f += norm2(cld(G, i));
   f = block_sum(f, scal);
   const double abs_tol = 1e-32 * f;
+  // N = 32 in the block ordering: the 31 x 16 pairs (p | q << 8) in a table
+  // built by the block before the first sweep, one LDS read per round
+  // instead of the index arithmetic
+  __shared__ unsigned short ptab[kBlockOrder && N == 32 ? (N - 1) * NP : 1];
+  if (kBlockOrder && N == 32) {
+    for (int e = threadIdx.x; e < (N - 1) * NP; e += blockDim.x) {
+      int tp, tq;
+      block_order_pair<N>(e / NP, e % NP, tp, tq);
+      ptab[e] = (unsigned short)(tp | (tq << 8));
+    }
+    __syncthreads();
+  }
   for (int sweep = 0; sweep < 60; ++sweep) {
     // rotated: any rotation this sweep; large: any with |g| above 1e-8 of
     // sqrt(al be) (1e-4 with kLoose).  A sweep whose rotations were all below 1e-8 leaves
@@ -700,20 +736,12 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
         constexpr int S = N / 8;  // slots (pairs) per wave and round
         const int o = r < 2 * S - 1 ? 0 : 1 + (r - (2 * S - 1)) / S;
         const int i = r < 2 * S - 1 ? r : (r - (2 * S - 1)) % S;
-        const int sl = k % S;
-        int bp, bq;
-        rr_pair(o, k / S, 8, bp, bq);
-        if (o == 0 && i < S - 1) {
-          // pairs inside a block: round m pairs a with a ^ m; slot j of a
-          // block takes the j-th a without the top bit of m
-          const int m = i + 1, blk = sl < S / 2 ? bp : bq, j = sl % (S / 2);
-          const int h = 31 - __builtin_clz(m);
-          const int a = ((j >> h) << (h + 1)) | (j & ((1 << h) - 1));
-          p = S * blk + a;
-          q = S * blk + (a ^ m);
+        if (N == 32) {
+          const unsigned pq = ptab[r * NP + k];
+          p = (int)(pq & 0xffu);
+          q = (int)(pq >> 8);
         } else {
-          p = S * bp + sl;
-          q = S * bq + (sl ^ (o == 0 ? i - (S - 1) : i));
+          block_order_pair<N>(r, k, p, q);
         }
         block_sync = o == 0 ? i == 2 * S - 2 : i == S - 1;
       } else {
