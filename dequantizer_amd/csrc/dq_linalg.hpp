@@ -31,6 +31,23 @@ __device__ __forceinline__ void rr_pair(int r, int k, int n2, int& p, int& q) {
   }
 }
 
+// The pairs rr_pair(r, k, n2) of rounds r = 0, 1, ... advanced by one step
+// per round: both players move up by one modulo n2 - 1, except player n2 - 1
+// of pair 0 (k = 0).  Same pairs, without a division per round.
+struct RoundRobin {
+  int a, b;
+  int k, m;  // m = n2 - 1
+  __device__ __forceinline__ RoundRobin(int k_, int n2)
+      : a(k_ == 0 ? n2 - 1 : k_), b(k_ == 0 ? 0 : n2 - 1 - k_), k(k_), m(n2 - 1) {}
+  // this round's pair (p < q), then the state of the next round
+  __device__ __forceinline__ void take(int& p, int& q) {
+    p = a < b ? a : b;
+    q = a < b ? b : a;
+    a = k == 0 ? a : (a + 1 == m ? 0 : a + 1);
+    b = b + 1 == m ? 0 : b + 1;
+  }
+};
+
 // 1/sqrt(x) and 1/x for normal positive x: the hardware estimates
 // (V_RSQ_F64, V_RCP_F64) refined by one Newton step, y <- y (3 - x y^2) / 2
 // and y <- y (2 - x y), which takes them to about an ulp.  The rotation
@@ -211,9 +228,10 @@ __device__ inline void wave_jacobi_eigh_n(cd* A, cd* V, bool loose = false) {
   __builtin_amdgcn_wave_barrier();
   for (int sweep = 0; sweep < 60; ++sweep) {
     int rotated = 0, large = 0;
+    RoundRobin rrs(k, N);
     for (int r = 0; r < N - 1; ++r) {
       int p, q;
-      rr_pair(r, k, N, p, q);
+      rrs.take(p, q);
       double c = 1.0, s = 0.0, er = 1.0, ei = 0.0;
       const double app = A[p * N + p].re, aqq = A[q * N + q].re;
       const cd apq = A[p * N + q];
@@ -498,9 +516,10 @@ __device__ inline void wave_jacobi_svd_n(cd* G, cd* U, cd* V, double* sigma, int
   __builtin_amdgcn_wave_barrier();
   for (int sweep = 0; sweep < 60; ++sweep) {
     int rotated = 0, large = 0;
+    RoundRobin rrs(k, N);
     for (int r = 0; r < N - 1; ++r) {
       int p, q;
-      rr_pair(r, k, N, p, q);
+      rrs.take(p, q);
       cd x[RPL], y[RPL];
 #pragma unroll
       for (int t = 0; t < RPL; ++t) {
@@ -602,9 +621,10 @@ __device__ inline double wave_nuclear_norm_n(cd* G, bool loose = false) {
   __builtin_amdgcn_wave_barrier();
   for (int sweep = 0; sweep < 60; ++sweep) {
     int rotated = 0, large = 0;
+    RoundRobin rrs(k, N);
     for (int r = 0; r < N - 1; ++r) {
       int p, q;
-      rr_pair(r, k, N, p, q);
+      rrs.take(p, q);
       cd x[RPL], y[RPL];
 #pragma unroll
       for (int t = 0; t < RPL; ++t) {
@@ -729,6 +749,7 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
     // 1e-13 threshold of the rotation test: the sweep that would only
     // confirm convergence is skipped.
     int rotated = 0, large = 0;
+    RoundRobin rrs(k, N);  // (the round-robin ordering)
     for (int r = 0; r < N - 1; ++r) {
       int p, q;
       bool block_sync = true;  // uniform over the block: depends on r only
@@ -745,7 +766,7 @@ __device__ inline void block_jacobi_svd_n(P* G, cd* U, P* V, double* sigma, int*
         }
         block_sync = o == 0 ? i == 2 * S - 2 : i == S - 1;
       } else {
-        rr_pair(r, k, N, p, q);
+        rrs.take(p, q);
       }
       cd x[RPL], y[RPL], vx[RPL], vy[RPL];
 #pragma unroll
