@@ -1003,6 +1003,19 @@ __device__ __attribute__((noinline)) int wave_pivoted_cholesky(P* G, int n, cd* 
     const int ii = row ? i : 0;
     cd a0 = mk<double>(0.0, 0.0), a1 = a0;
     int m = h;
+    // two steps of the loop below at a time, every load issued first (same
+    // order of additions in each chain)
+    for (; m + 6 < k; m += 8) {
+      const int p0 = piv[m], p1 = piv[m + 2], p2 = piv[m + 4], p3 = piv[m + 6];
+      const cd x0 = cld(G, p0 * n + ii), y0 = cld(G, p0 * n + j);
+      const cd x1 = cld(G, p1 * n + ii), y1 = cld(G, p1 * n + j);
+      const cd x2 = cld(G, p2 * n + ii), y2 = cld(G, p2 * n + j);
+      const cd x3 = cld(G, p3 * n + ii), y3 = cld(G, p3 * n + j);
+      fma_acc(a0, x0, conj(y0));
+      fma_acc(a1, x1, conj(y1));
+      fma_acc(a0, x2, conj(y2));
+      fma_acc(a1, x3, conj(y3));
+    }
     for (; m + 2 < k; m += 4) {
       const int p0 = piv[m], p1 = piv[m + 2];
       fma_acc(a0, cld(G, p0 * n + ii), conj(cld(G, p0 * n + j)));
