@@ -133,6 +133,12 @@ __global__ void __launch_bounds__(kBigThreads)
 typedef double big_d4 __attribute__((ext_vector_type(4)));
 typedef float big_f4 __attribute__((ext_vector_type(4)));
 
+// First tile of block b of n in a node's grid-stride tile loop, dealt so that
+// neighbouring tiles run on one XCD: workgroups go to the 8 XCDs in turn
+// (blocks b and b + 8 share an L2), and with 64-byte runs per tile row two
+// neighbouring tiles read and write the two halves of each 128-byte line.
+__device__ inline unsigned big_xcd_tile(unsigned b, unsigned n) { return n % 8 == 0 ? (b % 8) * (n / 8) + b / 8 : b; }
+
 template <typename R, int CHI>
 __global__ void __launch_bounds__(kBigThreads)
     k_big_gram(DevGraph g, const int* nodes, int idx, int src_id, int nb, const cx<R>* T, const cx<R>* buf,
@@ -289,8 +295,9 @@ __global__ void __launch_bounds__(kBigThreads)
   // two slices per wave, each with its own accumulators (four independent
   // chains of matrix-core steps in each phase)
   big_d4 gr0 = {0.0, 0.0, 0.0, 0.0}, gi0 = gr0, gr1 = gr0, gi1 = gr0;
-  if (blockIdx.x < ntiles) load_tile(blockIdx.x);
-  for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  const unsigned tb = big_xcd_tile(blockIdx.x, gridDim.x);
+  if (tb < ntiles) load_tile(tb);
+  for (long long tile = tb; tile < ntiles; tile += gridDim.x) {
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int q = tid + u * kBigThreads;
@@ -417,12 +424,13 @@ __global__ void __launch_bounds__(kBigThreads)
     const unsigned t2 = cp / nbot;
     return (t2 / nmid) * stop + (t2 % nmid) * smid + cp % nbot + offc[u] + (whole ? (unsigned)wwv[u] : 0u);
   };
-  if (blockIdx.x < ntiles) {
+  const unsigned tb = big_xcd_tile(blockIdx.x, gridDim.x);
+  if (tb < ntiles) {
 #pragma unroll
-    for (int u = 0; u < PER; ++u) np[u] = src[tile_off(blockIdx.x, u)];
+    for (int u = 0; u < PER; ++u) np[u] = src[tile_off(tb, u)];
   }
   const int s0 = 2 * w, s1 = 2 * w + 1;
-  for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  for (long long tile = tb; tile < ntiles; tile += gridDim.x) {
 #pragma unroll
     for (int u = 0; u < PER; ++u) St[wwv[u]][xav[u]][xbv[u]] = np[u];
     __syncthreads();
