@@ -25,6 +25,12 @@ namespace dq {
 constexpr int kBigThreads = 256;
 constexpr int kBigTile = 64;  // Gram columns per LDS tile
 
+// First tile of block b of n in a node's grid-stride tile loop, dealt so that
+// neighbouring tiles run on one XCD: workgroups go to the 8 XCDs in turn
+// (blocks b and b + 8 share an L2), and with 64-byte runs per tile row two
+// neighbouring tiles read and write the two halves of each 128-byte line.
+__device__ inline unsigned big_xcd_tile(unsigned b, unsigned n) { return n % 8 == 0 ? (b % 8) * (n / 8) + b / 8 : b; }
+
 // Absorb product t of a node (axis t): dst = src x_t X^T, ping-ponging
 // between the node's first two buffers (stride nb buffers per node).
 template <typename R, int CHI>
@@ -133,12 +139,6 @@ __global__ void __launch_bounds__(kBigThreads)
 typedef double big_d4 __attribute__((ext_vector_type(4)));
 typedef float big_f4 __attribute__((ext_vector_type(4)));
 
-// First tile of block b of n in a node's grid-stride tile loop, dealt so that
-// neighbouring tiles run on one XCD: workgroups go to the 8 XCDs in turn
-// (blocks b and b + 8 share an L2), and with 64-byte runs per tile row two
-// neighbouring tiles read and write the two halves of each 128-byte line.
-__device__ inline unsigned big_xcd_tile(unsigned b, unsigned n) { return n % 8 == 0 ? (b % 8) * (n / 8) + b / 8 : b; }
-
 template <typename R, int CHI>
 __global__ void __launch_bounds__(kBigThreads)
     k_big_gram(DevGraph g, const int* nodes, int idx, int src_id, int nb, const cx<R>* T, const cx<R>* buf,
@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(kBigThreads)
 // read U and T.  complex64 only (the complex128 levels keep the separate
 // launches, which run faster there).
 template <typename R>
-__global__ void __launch_bounds__(kBigThreads)
+__global__ void __launch_bounds__(kBigThreads, 3)
     k_big_prod_gram16(DevGraph g, const int* nodes, int j, int idx, int src_id, int nb, const cx<R>* T,
                       const cx<R>* n2e_old, const cx<R>* e2n_old, const cx<R>* n2e_new, const uint8_t* node_src,
                       const cx<R>* buf, long long smax, cd* part, const int* inst_active) {
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(kBigThreads)
   const cx<R>* P = src_id < 0 ? t : buf + ((long long)nb * k + src_id) * smax;
   // M_j[a][b] as the A operand of the product's four k-steps: lane l holds
   // M[l % 16][4 ks + l / 16]
-  cd mv[4];
+  float mvr[4], mvi[4];
   {
     const int base = g.adj_ptr[node];
     const int e = g.adj_edge[base + j], sd = g.adj_side[base + j];
@@ -250,25 +250,26 @@ __global__ void __launch_bounds__(kBigThreads)
     const cx<R>* m = code == 0 ? e2n_old + (2LL * e + sd) * CC
                                : (code == 1 ? n2e_new : n2e_old) + (2LL * e + 1 - sd) * CC;
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) mv[ks] = cvt<double>(m[mi * CHI + 4 * ks + kq]);
+    for (int ks = 0; ks < 4; ++ks) {
+      const cx<R> mm = m[mi * CHI + 4 * ks + kq];
+      mvr[ks] = (float)mm.re;
+      mvi[ks] = (float)mm.im;
+    }
   }
   // the next tile's elements are loaded into registers while the current
   // tile is multiplied (8 of P and 8 of T per thread)
   // Index arithmetic in 32 bits (the host fuses only tensors below 2^31
   // elements): the (x_hi, x_lo) part of each of the thread's PER elements is
   // fixed; when the contiguous run n_bot holds whole tiles the c' part is one
-  // offset per tile.
+  // offset per tile.  Element u of thread tid is q = tid + 256 u of the tile:
+  // slice ww = q % W and x_lo = (q / W) % CHI are the thread's own, x_hi =
+  // q / (W CHI) steps by two per u (one offset and a uniform stride).
   constexpr int PER = W * CC / kBigThreads;
+  static_assert(kBigThreads % (W * CHI) == 0, "element u of a thread keeps its slice and x_lo");
+  constexpr int XH_STEP = kBigThreads / (W * CHI);
   cx<R> np[PER], nt[PER];
-  unsigned offc[PER];
-  int wwv[PER];
-#pragma unroll
-  for (int u = 0; u < PER; ++u) {
-    const int q = tid + u * kBigThreads;
-    wwv[u] = q % W;
-    const int rest = q / W, xl = rest % CHI, xh = rest / CHI;
-    offc[u] = (unsigned)(xh * st_hi + xl * st_lo);
-  }
+  const int ww = tid % W, xl = (tid / W) % CHI, xh0 = tid / (W * CHI);
+  const unsigned offx = (unsigned)(xh0 * st_hi + xl * st_lo), ostep = (unsigned)(XH_STEP * st_hi);
   const unsigned nbot = (unsigned)n_bot, nmid = (unsigned)n_mid, smid = (unsigned)st_mid, stop = (unsigned)st_top;
   const bool whole = nbot % W == 0;
   auto load_tile = [&](long long tile) {
@@ -277,16 +278,17 @@ __global__ void __launch_bounds__(kBigThreads)
       const unsigned t2 = cp0 / nbot, base = (t2 / nmid) * stop + (t2 % nmid) * smid + cp0 % nbot;
 #pragma unroll
       for (int u = 0; u < PER; ++u) {
-        const unsigned off = base + offc[u] + wwv[u];
+        const unsigned off = base + offx + u * ostep + ww;
         np[u] = P[off];
         nt[u] = t[off];
       }
     } else {
+      const unsigned cp = cp0 + ww;
+      const unsigned t2 = cp / nbot;
+      const unsigned base = (t2 / nmid) * stop + (t2 % nmid) * smid + cp % nbot + offx;
 #pragma unroll
       for (int u = 0; u < PER; ++u) {
-        const unsigned cp = cp0 + wwv[u];
-        const unsigned t2 = cp / nbot;
-        const unsigned off = (t2 / nmid) * stop + (t2 % nmid) * smid + cp % nbot + offc[u];
+        const unsigned off = base + u * ostep;
         np[u] = P[off];
         nt[u] = t[off];
       }
@@ -300,8 +302,7 @@ __global__ void __launch_bounds__(kBigThreads)
   for (long long tile = tb; tile < ntiles; tile += gridDim.x) {
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      const int q = tid + u * kBigThreads;
-      const int ww = q % W, rest = q / W, xl = rest % CHI, xh = rest / CHI;
+      const int xh = xh0 + XH_STEP * u;
       const int xj = j < idx ? xh : xl, xi = j < idx ? xl : xh;
       Pt[ww][xj][xi] = np[u];
       Tt[ww][xj][xi] = nt[u];
@@ -317,7 +318,7 @@ __global__ void __launch_bounds__(kBigThreads)
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       const cx<R> p0 = Pt[s0][4 * ks + kq][mi], p1 = Pt[s1][4 * ks + kq][mi];
-      const float mr = (float)mv[ks].re, mim = (float)mv[ks].im;
+      const float mr = mvr[ks], mim = mvi[ks];
       const float p0r = (float)p0.re, p0i = (float)p0.im, p1r = (float)p1.re, p1i = (float)p1.im;
       ur0 = __builtin_amdgcn_mfma_f32_16x16x4f32(mr, p0r, ur0, 0, 0, 0);
       ur1 = __builtin_amdgcn_mfma_f32_16x16x4f32(mr, p1r, ur1, 0, 0, 0);
@@ -404,39 +405,41 @@ __global__ void __launch_bounds__(kBigThreads)
   // each thread's PER elements of a tile: slice q % W, (x_hi, x_lo) from q / W
   // (as k_big_prod_gram16; 32-bit offsets: the host fuses only tensors below
   // 2^31 elements)
+  // (element u: x_hi = q / (W CHI) steps by two per u, slice and x_lo fixed)
   constexpr int PER = W * CC / kBigThreads;
+  static_assert(kBigThreads % (W * CHI) == 0, "element u of a thread keeps its slice and x_lo");
+  constexpr int XH_STEP = kBigThreads / (W * CHI);
   cx<R> np[PER];
-  unsigned offc[PER];
-  int wwv[PER], xav[PER], xbv[PER];
-#pragma unroll
-  for (int u = 0; u < PER; ++u) {
-    const int q = tid + u * kBigThreads;
-    wwv[u] = q % W;
-    const int rest = q / W, xl = rest % CHI, xh = rest / CHI;
-    offc[u] = (unsigned)(xh * st_hi + xl * st_lo);
-    xav[u] = ax_a == a_hi ? xh : xl;
-    xbv[u] = ax_b == a_hi ? xh : xl;
-  }
+  const int ww = tid % W, xl = (tid / W) % CHI, xh0 = tid / (W * CHI);
+  const bool a_first = ax_a == a_hi;
+  const unsigned offx = (unsigned)(xh0 * st_hi + xl * st_lo), ostep = (unsigned)(XH_STEP * st_hi);
   const unsigned nbot = (unsigned)n_bot, nmid = (unsigned)n_mid, smid = (unsigned)st_mid, stop = (unsigned)st_top;
-  const bool whole = nbot % W == 0;
-  auto tile_off = [&](long long tile, int u) -> unsigned {
-    const unsigned cp = (unsigned)(tile * W) + (whole ? 0u : (unsigned)wwv[u]);
+  // offset of the thread's element 0 of a tile (element u: + u ostep)
+  auto tile_off = [&](long long tile) -> unsigned {
+    const unsigned cp = (unsigned)(tile * W) + ww;
     const unsigned t2 = cp / nbot;
-    return (t2 / nmid) * stop + (t2 % nmid) * smid + cp % nbot + offc[u] + (whole ? (unsigned)wwv[u] : 0u);
+    return (t2 / nmid) * stop + (t2 % nmid) * smid + cp % nbot + offx;
+  };
+  // the thread's element u in the slice tile St[ww][x_a][x_b]
+  auto st_at = [&](int u) -> cx<R>& {
+    const int xh = xh0 + XH_STEP * u;
+    return St[ww][a_first ? xh : xl][a_first ? xl : xh];
   };
   const unsigned tb = big_xcd_tile(blockIdx.x, gridDim.x);
   if (tb < ntiles) {
+    const unsigned o = tile_off(tb);
 #pragma unroll
-    for (int u = 0; u < PER; ++u) np[u] = src[tile_off(tb, u)];
+    for (int u = 0; u < PER; ++u) np[u] = src[o + u * ostep];
   }
   const int s0 = 2 * w, s1 = 2 * w + 1;
   for (long long tile = tb; tile < ntiles; tile += gridDim.x) {
 #pragma unroll
-    for (int u = 0; u < PER; ++u) St[wwv[u]][xav[u]][xbv[u]] = np[u];
+    for (int u = 0; u < PER; ++u) st_at(u) = np[u];
     __syncthreads();
     if (tile + gridDim.x < ntiles) {
+      const unsigned o = tile_off(tile + gridDim.x);
 #pragma unroll
-      for (int u = 0; u < PER; ++u) np[u] = src[tile_off(tile + gridDim.x, u)];
+      for (int u = 0; u < PER; ++u) np[u] = src[o + u * ostep];
     }
     // U1 = S M_b^T
     big_f4 ur0 = {0.f, 0.f, 0.f, 0.f}, ui0 = ur0, ur1 = ur0, ui1 = ur0;
@@ -476,8 +479,9 @@ __global__ void __launch_bounds__(kBigThreads)
       St[s1][4 * kq + v][mi] = mk<R>((R)vr1[v], (R)vi1[v]);
     }
     __syncthreads();
+    const unsigned o = tile_off(tile);
 #pragma unroll
-    for (int u = 0; u < PER; ++u) dst[tile_off(tile, u)] = St[wwv[u]][xav[u]][xbv[u]];
+    for (int u = 0; u < PER; ++u) dst[o + u * ostep] = st_at(u);
     __syncthreads();
   }
 }
