@@ -355,21 +355,24 @@ __global__ void __launch_bounds__(kBigThreads, 3)
 
 // Two products of a node update in one pass, chi = 16, complex64 data:
 //   dst = src x_a M_a x_b M_b   (a != b),
-// over the tiles of k_big_prod_gram16: the 16 x 16 (x_a, x_b) slices S of src
-// for W = 8 consecutive c' (the remaining indices, contiguous ones fastest),
-// wave w taking slices 2w and 2w + 1.  Per slice on FP32 matrix cores (as the
+// over tiles as in k_big_prod_gram16 but twice as wide: the 16 x 16 (x_a,
+// x_b) slices S of src for W = 16 consecutive c' (the remaining indices,
+// contiguous ones fastest; whole 128-byte lines where the tile rows are runs
+// in memory, so no write covers half a line), 512 threads, wave w taking
+// slices 2w and 2w + 1.  Per slice on FP32 matrix cores (as the
 // separate products' FP32 FMAs): U1 = S M_b^T with A = S (lane l: S[l % 16]
 // [4 ks + l / 16]) and B = M_b^T (lane l: M_b[l % 16][4 ks + l / 16]); the
 // result registers hold U1 rows 4 (l / 16) + v, column l % 16, which are the
 // B operands of U = M_a U1 when the k-steps run over those rows (step v:
 // A = M_a[l % 16][4 (l / 16) + v]).  U goes back over the slice in LDS and
 // out along memory: one HBM pass where two separate products take two.
+constexpr int kProd2Threads = 512;
 template <typename R>
-__global__ void __launch_bounds__(kBigThreads)
+__global__ void __launch_bounds__(kProd2Threads)
     k_big_prod2_16(DevGraph g, const int* nodes, int ax_a, int ax_b, int src_id, int dst_id, int nb, const cx<R>* T,
                    const cx<R>* n2e_old, const cx<R>* e2n_old, const cx<R>* n2e_new, const uint8_t* node_src,
                    cx<R>* buf, long long smax, const int* inst_active) {
-  constexpr int CHI = 16, CC = CHI * CHI, W = 8;
+  constexpr int CHI = 16, CC = CHI * CHI, W = 16;
   __shared__ cx<R> St[W][CHI][CHI + 1];  // [slice][x_a][x_b]
   const int k = blockIdx.y;
   const int node = nodes[k];
@@ -406,9 +409,10 @@ __global__ void __launch_bounds__(kBigThreads)
   // (as k_big_prod_gram16; 32-bit offsets: the host fuses only tensors below
   // 2^31 elements)
   // (element u: x_hi = q / (W CHI) steps by two per u, slice and x_lo fixed)
-  constexpr int PER = W * CC / kBigThreads;
-  static_assert(kBigThreads % (W * CHI) == 0, "element u of a thread keeps its slice and x_lo");
-  constexpr int XH_STEP = kBigThreads / (W * CHI);
+  constexpr int PER = W * CC / kProd2Threads;
+  static_assert(kProd2Threads % (W * CHI) == 0, "element u of a thread keeps its slice and x_lo");
+  static_assert(kProd2Threads / 64 * 2 == W, "two slices per wave");
+  constexpr int XH_STEP = kProd2Threads / (W * CHI);
   cx<R> np[PER];
   const int ww = tid % W, xl = (tid / W) % CHI, xh0 = tid / (W * CHI);
   const bool a_first = ax_a == a_hi;
@@ -676,7 +680,7 @@ static hipError_t big_bp_level(const DevGraph& g, int min_deg, int max_deg, cons
     // passes (22 tensor reads and writes per node against 32 for the plan
     // above, same 16 products); degree 4: Q_{01} = T x23, Q_{23} = T x01.
     auto prod2 = [&](int a, int b, int src, int dst) {
-      hipLaunchKernelGGL((k_big_prod2_16<R>), dim3(chunks, count), dim3(kBigThreads), 0, s, g, nodes, a, b, src, dst,
+      hipLaunchKernelGGL((k_big_prod2_16<R>), dim3(chunks, count), dim3(kProd2Threads), 0, s, g, nodes, a, b, src, dst,
                          nb, (const cx<R>*)T, (const cx<R>*)n2e_old, (const cx<R>*)e2n_old, (const cx<R>*)n2e_new,
                          node_src, buf, smax, inst_active);
     };
