@@ -37,6 +37,15 @@ TRAFFIC_BP_SWEEP_DEFAULT = 2 * 38304.5 * 1024 + 12000.0 * 1024
 # dispatch; the 256 MiB copy reads 131089 KiB by FETCH_SIZE and writes
 # 262144 KiB by WRITE_SIZE): 84093.6 KiB fetched (x 2) + 112942 KiB written.
 TRAFFIC_SU_LAUNCH_DEFAULT = int((2 * 84093.6 + 112942.0) * 1024)
+# HBM bytes of one complex64 BP sweep of the periodic 4^3 lattice leg (64
+# degree-6 nodes of 256 MiB): the streaming kernels' counter passes over the
+# leg (FETCH_SIZE x 2 + WRITE_SIZE, calibrated as above), summed over a
+# sweep's 5 two-product and 6 product-Gram launches: 160.0 GiB for the
+# two-product passes (their algorithmic 32 GiB each) and 218.7 GiB for the
+# product-Gram passes (192 GiB algorithmic; the 64-byte tile rows of the
+# non-contiguous axis pairs are partly fetched twice).
+TRAFFIC_LATTICE_SWEEP_L4 = 406597664796
+ALGO_BYTES_LATTICE_SWEEP_L4 = 22 * 64 * 256 * 2 ** 20
 
 
 def eu_flops(d: int, chi: int, p: int) -> int:
@@ -623,7 +632,10 @@ def lattice_run(args, chi=16, tol=1e-8, max_sweeps=60):
     out["algorithmic_tflops_c64"] = eu * eu_flops(6, chi, 2) / (out["sweep_ms_c64"] * 1e-3) / 1e12
     out["roofline"] = {"bound": "mfma", "kernel": "BP sweep (streaming node update, degree 6, chi=16, complex64)",
                        "achieved": out["algorithmic_tflops_c64"], "peak": 157.3, "unit": "TFLOP/s",
-                       "frac": out["algorithmic_tflops_c64"] / 157.3, "traffic": None,
+                       "frac": out["algorithmic_tflops_c64"] / 157.3,
+                       "traffic": TRAFFIC_LATTICE_SWEEP_L4 if args.lattice_size == 4 else None,
+                       "traffic_unit": "B per sweep (the streaming kernels, counter passes)",
+                       "algorithmic_bytes_per_sweep": ALGO_BYTES_LATTICE_SWEEP_L4 if args.lattice_size == 4 else None,
                        "algorithmic_flops_per_sweep": eu * eu_flops(6, chi, 2),
                        "timing": "wall time of the sweeps after the first, synchronised (one sweep per map call)"}
     (m64, l64), (m128, l128) = res["c64"], res["c128"]
